@@ -1,0 +1,146 @@
+/*
+ * pica_amd.h -- C ABI of the MI355X (gfx950) particle-in-cell step.
+ *
+ * Drop-in boundary for pica's hot path.  pica itself is a header-only C++ template library
+ * whose per-particle functors (FieldInterpolatorCIC::get, BorisPusher::push,
+ * CurrentDepositorCIC::deposit) cannot cross a host->device boundary one particle at a time,
+ * so this ABI works at grid x ensemble granularity.  The C++ surface in
+ * include/pica_amd/pica.hpp keeps pica's type and method names on top of it.
+ *
+ * Conventions
+ *  - Every call returns PK_OK (0) or a negative PK_E_* code; pk_last_error() gives the text.
+ *    No exception crosses the ABI.
+ *  - Arrays are pica's Array3d layout: z fastest, data[(i*ny + j)*nz + k] (Array.h:140-144).
+ *  - Units are pica's CGS units (Constants.h:8-19); pi is pica's truncated literal.
+ *  - One HIP stream per context; calls on one context are ordered; a context is not
+ *    thread-safe.  Device buffers are owned by the library.
+ */
+#ifndef PICA_AMD_H
+#define PICA_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ---- */
+#define PK_OK 0
+#define PK_E_ARG (-1)         /* invalid argument or mismatched geometry */
+#define PK_E_HIP (-2)         /* HIP runtime error */
+#define PK_E_ALLOC (-3)       /* device or host allocation failed */
+#define PK_E_STATE (-4)       /* a particle left its tile (non-finite or faster than the tile allows) */
+#define PK_E_UNSUPPORTED (-5) /* combination not implemented */
+#define PK_E_CAPACITY (-6)    /* fixed-capacity buffer too small */
+
+/* ---- enums ---- */
+enum { PK_F64 = 0, PK_F32 = 1 };                  /* arithmetic type of grid / ensemble */
+enum { PK_POS_ABS = 0, PK_POS_SPLIT = 1 };        /* absolute positions, or int32 cell + float offset */
+enum { PK_CIC = 0, PK_TSC = 1 };                  /* particle form factor */
+enum { PK_DIRECT = 0, PK_ESIRKEPOV = 1 };         /* current depositor */
+enum { PK_EX = 0, PK_EY, PK_EZ, PK_BX, PK_BY, PK_BZ, PK_JX, PK_JY, PK_JZ, PK_RHO, PK_NCOMP };
+enum { PK_H2D = 0, PK_D2H = 1 };
+enum { PK_FILL_E = 1, PK_FILL_B = 2, PK_FILL_J = 4, PK_FOLD_J = 8, PK_FOLD_RHO = 16 };
+
+/* pk_diag output slots (all FP64) */
+enum {
+    PK_DIAG_FIELD_ENERGY = 0,   /* sum_cells (E^2 + B^2) V / (8 pi) */
+    PK_DIAG_KINETIC_ENERGY,     /* sum_p w m c^2 (gamma - 1) */
+    PK_DIAG_CHARGE,             /* sum_cells rho V */
+    PK_DIAG_GAUSS_MAX,          /* max |div E - 4 pi rho| over cells */
+    PK_DIAG_GAUSS_DRIFT,        /* max |(div E - 4 pi rho) - reference residual| (0 without a reference) */
+    PK_DIAG_RHO_MAX,            /* max |4 pi rho| (scale for the residuals) */
+    PK_DIAG_N
+};
+
+typedef struct pk_ctx pk_ctx;
+typedef struct pk_grid pk_grid;
+typedef struct pk_ens pk_ens;
+
+/* ---- context ---- */
+int pk_ctx_create(int device, pk_ctx** out);
+void pk_ctx_destroy(pk_ctx* ctx);
+const char* pk_last_error(const pk_ctx* ctx);
+int pk_sync(pk_ctx* ctx);
+/* the hipStream_t all work of this context is issued on (for events / interop) */
+void* pk_ctx_stream(pk_ctx* ctx);
+/* per-kernel HIP-event timing: when enabled, every kernel launch of this context is bracketed
+ * by events on its stream; pk_timer_get returns the accumulated ms and launch count for a
+ * kernel name ("particles", "insert", "yee_e", "yee_b", "ghost", "rebuild", ...). */
+int pk_timer_enable(pk_ctx* ctx, int on);
+int pk_timer_get(pk_ctx* ctx, const char* kernel, double* total_ms, long* launches);
+int pk_timer_reset(pk_ctx* ctx);
+
+/* ParticleTypes table (Particle.h:21-31): species masses and charges (CGS). */
+int pk_species_set(pk_ctx* ctx, int ntypes, const double* mass, const double* charge);
+
+/* ---- grid: replaces YeeGrid<Three, Real>(origin, step, size), YeeGrid.h:191-256 ----
+ * size = array size incl. ghosts; ghost = periodic ghost width used by pk_ghost and the
+ * particle stage (2 for the periodic PIC step, origin = -2 step; 0 for a plain pica grid). */
+int pk_grid_create(pk_ctx* ctx, int prec, const int size[3], const double origin[3], const double step[3],
+                   int ghost, pk_grid** out);
+void pk_grid_destroy(pk_grid* g);
+/* copy one component (PK_EX..PK_RHO) between host and device; host element type = grid prec */
+int pk_grid_copy(pk_grid* g, int comp, void* host, int dir);
+int pk_grid_zero(pk_grid* g, int comp);
+/* YeeSolver::updateE / updateB (YeeSolver.h:14-18,132-213): pica's semantics on the whole
+ * array, including its open-edge rules. */
+int pk_yee_update_e(pk_grid* g, double dt);
+int pk_yee_update_b(pk_grid* g, double dt);
+/* periodic ghost fill / fold (PK_FILL_* | PK_FOLD_*), width = the grid's ghost */
+int pk_ghost(pk_grid* g, int ops);
+
+/* ---- ensemble: replaces Ensemble<ParticleArraySoA<Particle3d>, Supercells>::Type,
+ * EnsembleSupercells(min, max, numCells, numCellsPerSupercell) (EnsembleSupercells.h:25-34).
+ * The domain is [0, ncell*step); ncell must be a multiple of ncps.  One species per ensemble.
+ * prec/posmode: (PK_F64, PK_POS_ABS) or (PK_F32, PK_POS_SPLIT). */
+int pk_ens_create(pk_ctx* ctx, int prec, int posmode, const int ncell[3], const double step[3],
+                  const int ncps[3], int type_index, pk_ens** out);
+void pk_ens_destroy(pk_ens* e);
+/* slack policy of the fixed per-supercell capacity: cap = count + count*frac + add */
+int pk_ens_set_slack(pk_ens* e, double frac, int add);
+/* EnsembleSupercells::add for n particles (any order); replaces the current contents.
+ * Positions absolute (CGS), u = p/(mc) as ParticleRef::setP takes it, w = factor. */
+int pk_ens_upload(pk_ens* e, size_t n, const double* x, const double* y, const double* z, const double* ux,
+                  const double* uy, const double* uz, const double* w);
+/* particles in supercell order; arrays must hold pk_ens_size() entries; supercell offsets
+ * (nsupercells + 1 entries) written when off != NULL */
+int pk_ens_download(pk_ens* e, double* x, double* y, double* z, double* ux, double* uy, double* uz,
+                    double* w, int64_t* off);
+int64_t pk_ens_size(pk_ens* e);
+int64_t pk_ens_num_supercells(pk_ens* e);
+/* per-cell particle counts (physical cells, z fastest) and per-particle cell keys in
+ * supercell order (linear physical cell index), either may be NULL */
+int pk_ens_cells(pk_ens* e, int32_t* cell_counts, int32_t* cell_of_particle);
+/* seeded thermal loader on the device: ppc particles per cell at uniform random positions,
+ * u ~ N(0, uth) per component, plus +drift_x on even and -drift_x on odd particle ids. */
+int pk_ens_load_thermal(pk_ens* e, int ppc, double uth, double drift_x, double weight, uint64_t seed);
+
+/* ---- the step ----
+ * Fused interpolation + Boris push + deposition for every particle (one workgroup per
+ * supercell: FieldInterpolatorCICSupercell, BorisPusher::push, CurrentDepositorCICSupercell,
+ * FieldInterpolatorYeeGrid.h:187-283, BorisPusher.h:32-47, CurrentDepositorYeeGrid.h:168-264).
+ * J is accumulated into the grid (not zeroed); particles that leave their supercell are
+ * extracted for pk_ens_resort. */
+int pk_particle_step(pk_ens* e, pk_grid* g, double dt, int shape, int dep);
+/* moves extracted particles into their new supercells (deterministic order) */
+int pk_ens_resort(pk_ens* e);
+/* a whole periodic PIC step: J = 0, particle step, fold J, B(dt/2), E(dt), B(dt/2) with ghost
+ * fills, re-sort */
+int pk_step(pk_ens* e, pk_grid* g, double dt, int shape, int dep);
+int pk_run(pk_ens* e, pk_grid* g, double dt, int shape, int dep, int nsteps);
+
+/* ---- diagnostics (FP64 reductions on the device) ----
+ * rho of the ensemble with the given shape at cell centres into the grid's PK_RHO component
+ * (folded and filled), then energies, charge and Gauss-law residual; out has PK_DIAG_N slots */
+int pk_deposit_rho(pk_ens* e, pk_grid* g, int shape);
+int pk_diag(pk_ens* e, pk_grid* g, int shape, double* out);
+/* store the current div E - 4 pi rho as the reference for PK_DIAG_GAUSS_DRIFT */
+int pk_diag_set_reference(pk_ens* e, pk_grid* g, int shape);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PICA_AMD_H */

@@ -1,0 +1,358 @@
+"""pica_amd -- the pica particle-in-cell step on MI355X (gfx950), behind pica's own names.
+
+The compute lives in ``libpica_amd.so`` (hand-written HIP, built in-tree by
+``pica_amd/build.py``); this module is a thin ctypes layer over its C ABI
+(``include/pica_amd.h``) with pica's type and method names:
+
+    YeeGrid(ctx, size, origin, step)          pica::YeeGrid<Three, Real>   (YeeGrid.h:191-256)
+    YeeSolver().updateE(grid, dt)/updateB     pica::YeeSolver              (YeeSolver.h:11-38)
+    EnsembleSupercells(ctx, ncell, step, ncps) pica::EnsembleSupercells    (EnsembleSupercells.h:15-69)
+    particle_step(ens, grid, dt, shape, dep)  FieldInterpolatorCICSupercell::get + BorisPusher::push +
+                                              CurrentDepositorCICSupercell::deposit for every particle
+
+There is no CPU fallback: without the native library every entry point raises.
+"""
+import atexit
+import ctypes
+import os
+import weakref
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libpica_amd.so")
+
+PK_OK, PK_E_ARG, PK_E_HIP, PK_E_ALLOC, PK_E_STATE, PK_E_UNSUPPORTED, PK_E_CAPACITY = 0, -1, -2, -3, -4, -5, -6
+F64, F32 = 0, 1
+POS_ABS, POS_SPLIT = 0, 1
+CIC, TSC = 0, 1
+DIRECT, ESIRKEPOV = 0, 1
+EX, EY, EZ, BX, BY, BZ, JX, JY, JZ, RHO = range(10)
+H2D, D2H = 0, 1
+FILL_E, FILL_B, FILL_J, FOLD_J, FOLD_RHO = 1, 2, 4, 8, 16
+DIAG_FIELD_ENERGY, DIAG_KINETIC_ENERGY, DIAG_CHARGE, DIAG_GAUSS_MAX, DIAG_GAUSS_DRIFT, DIAG_RHO_MAX = range(6)
+DIAG_N = 6
+
+# pica's CGS constants (Constants.h:10-13)
+PI = 3.14159265358
+C = 29979245800.0
+ELECTRON_CHARGE = -4.80320427e-10
+ELECTRON_MASS = 9.10938215e-28
+
+# every symbol include/pica_amd.h declares
+ABI_SYMBOLS = (
+    "pk_ctx_create", "pk_ctx_destroy", "pk_last_error", "pk_sync", "pk_ctx_stream", "pk_timer_enable",
+    "pk_timer_get", "pk_timer_reset", "pk_species_set", "pk_grid_create", "pk_grid_destroy", "pk_grid_copy",
+    "pk_grid_zero", "pk_yee_update_e", "pk_yee_update_b", "pk_ghost", "pk_ens_create", "pk_ens_destroy",
+    "pk_ens_set_slack", "pk_ens_upload", "pk_ens_download", "pk_ens_size", "pk_ens_num_supercells",
+    "pk_ens_cells", "pk_ens_load_thermal", "pk_particle_step", "pk_ens_resort", "pk_step", "pk_run",
+    "pk_deposit_rho", "pk_diag", "pk_diag_set_reference",
+)
+
+_LIB = None
+_CONTEXTS = weakref.WeakSet()
+
+
+@atexit.register
+def _close_all():
+    # release device objects before the interpreter (and then the HIP runtime) shut down
+    for c in list(_CONTEXTS):
+        try:
+            c.close()
+        except Exception:
+            pass
+
+
+class PicaError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("pica_amd error %d: %s" % (code, msg))
+        self.code = code
+
+
+def lib():
+    """The native library; raises if it has not been built (there is no fallback)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIBPATH):
+            raise RuntimeError("pica_amd: native library %s is missing; run `python -m pica_amd.build`" % LIBPATH)
+        L = ctypes.CDLL(LIBPATH)
+        vp = ctypes.c_void_p
+        L.pk_last_error.restype = ctypes.c_char_p
+        L.pk_last_error.argtypes = [vp]
+        L.pk_ctx_stream.restype = vp
+        L.pk_ctx_stream.argtypes = [vp]
+        L.pk_ctx_destroy.argtypes = [vp]
+        L.pk_grid_destroy.argtypes = [vp]
+        L.pk_ens_destroy.argtypes = [vp]
+        L.pk_ens_size.restype = ctypes.c_int64
+        L.pk_ens_size.argtypes = [vp]
+        L.pk_ens_num_supercells.restype = ctypes.c_int64
+        L.pk_ens_num_supercells.argtypes = [vp]
+        for name in ("pk_sync", "pk_timer_reset"):
+            getattr(L, name).argtypes = [vp]
+        L.pk_timer_enable.argtypes = [vp, ctypes.c_int]
+        L.pk_timer_get.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long)]
+        L.pk_yee_update_e.argtypes = [vp, ctypes.c_double]
+        L.pk_yee_update_b.argtypes = [vp, ctypes.c_double]
+        L.pk_ghost.argtypes = [vp, ctypes.c_int]
+        L.pk_grid_copy.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int]
+        L.pk_grid_zero.argtypes = [vp, ctypes.c_int]
+        L.pk_ens_set_slack.argtypes = [vp, ctypes.c_double, ctypes.c_int]
+        L.pk_ens_upload.argtypes = [vp, ctypes.c_size_t] + [vp] * 7
+        L.pk_ens_download.argtypes = [vp] * 9
+        L.pk_ens_cells.argtypes = [vp, vp, vp]
+        L.pk_ens_load_thermal.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                          ctypes.c_uint64]
+        L.pk_particle_step.argtypes = [vp, vp, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+        L.pk_ens_resort.argtypes = [vp]
+        L.pk_step.argtypes = [vp, vp, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+        L.pk_run.argtypes = [vp, vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.pk_deposit_rho.argtypes = [vp, vp, ctypes.c_int]
+        L.pk_diag.argtypes = [vp, vp, ctypes.c_int, vp]
+        L.pk_diag_set_reference.argtypes = [vp, vp, ctypes.c_int]
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def _i3(v):
+    return (ctypes.c_int * 3)(*[int(x) for x in v])
+
+
+def _d3(v):
+    return (ctypes.c_double * 3)(*[float(x) for x in v])
+
+
+class Context:
+    """One device and one HIP stream (pk_ctx).  Grids and ensembles created on a context are
+    released before it (close() and interpreter exit)."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        r = lib().pk_ctx_create(int(device), ctypes.byref(h))
+        if r:
+            raise PicaError(r, "pk_ctx_create(device=%d) failed" % device)
+        self.h = h
+        self._children = weakref.WeakSet()
+        _CONTEXTS.add(self)
+
+    def _adopt(self, obj):
+        self._children.add(obj)
+
+    def check(self, r):
+        if r:
+            raise PicaError(r, lib().pk_last_error(self.h).decode())
+
+    def sync(self):
+        self.check(lib().pk_sync(self.h))
+
+    def stream(self):
+        return lib().pk_ctx_stream(self.h)
+
+    def set_species(self, mass, charge):
+        m = np.ascontiguousarray(mass, np.float64)
+        q = np.ascontiguousarray(charge, np.float64)
+        self.check(lib().pk_species_set(self.h, len(m), _p(m), _p(q)))
+
+    def timer_enable(self, on=True):
+        self.check(lib().pk_timer_enable(self.h, 1 if on else 0))
+
+    def timer_reset(self):
+        self.check(lib().pk_timer_reset(self.h))
+
+    def timer(self, kernel):
+        ms, n = ctypes.c_double(), ctypes.c_long()
+        self.check(lib().pk_timer_get(self.h, kernel.encode(), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            for child in list(self._children):
+                child.close()
+            lib().pk_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class YeeGrid:
+    """pica::YeeGrid<Three, Real>(origin, step, size) on the device (+ rho), z-fastest arrays."""
+
+    def __init__(self, ctx, size, origin, step, prec=F64, ghost=0):
+        self.ctx = ctx
+        self.size = tuple(int(v) for v in size)
+        self.origin = tuple(float(v) for v in origin)
+        self.step = tuple(float(v) for v in step)
+        self.prec = prec
+        self.ghost = ghost
+        self.dtype = np.float64 if prec == F64 else np.float32
+        h = ctypes.c_void_p()
+        ctx.check(lib().pk_grid_create(ctx.h, prec, _i3(self.size), _d3(self.origin), _d3(self.step), int(ghost),
+                                       ctypes.byref(h)))
+        self.h = h
+        ctx._adopt(self)
+
+    @classmethod
+    def periodic(cls, ctx, ncell, step=(1.0, 1.0, 1.0), prec=F64):
+        """ncell physical cells with 2 periodic ghost layers per side, origin -2 step (the
+        convention pica's supercell interpolator and depositor assume)."""
+        return cls(ctx, [n + 4 for n in ncell], [-2.0 * h for h in step], step, prec, ghost=2)
+
+    def get(self, comp):
+        a = np.empty(self.size, self.dtype)
+        self.ctx.check(lib().pk_grid_copy(self.h, comp, _p(a), D2H))
+        return a
+
+    def set(self, comp, arr):
+        a = np.ascontiguousarray(arr, self.dtype)
+        assert a.shape == self.size
+        self.ctx.check(lib().pk_grid_copy(self.h, comp, _p(a), H2D))
+
+    def get_all(self):
+        return np.stack([self.get(c) for c in range(9)])
+
+    def set_all(self, f):
+        for c in range(9):
+            self.set(c, f[c])
+
+    def zero(self, comp):
+        self.ctx.check(lib().pk_grid_zero(self.h, comp))
+
+    def ghost_ops(self, ops):
+        self.ctx.check(lib().pk_ghost(self.h, ops))
+
+    def close(self):
+        if getattr(self, "h", None):
+            if self.ctx.h:
+                lib().pk_grid_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class YeeSolver:
+    """pica::YeeSolver (YeeSolver.h:11-38): stateless, pica's method names."""
+
+    def updateE(self, grid, dt):
+        grid.ctx.check(lib().pk_yee_update_e(grid.h, float(dt)))
+
+    def updateB(self, grid, dt):
+        grid.ctx.check(lib().pk_yee_update_b(grid.h, float(dt)))
+
+
+class EnsembleSupercells:
+    """pica::EnsembleSupercells over ParticleArraySoA on the device, one species per ensemble.
+    The domain is [0, ncell*step); ncell must be a multiple of ncps."""
+
+    def __init__(self, ctx, ncell, step, ncps, prec=F64, posmode=None, type_index=0):
+        self.ctx = ctx
+        self.ncell = tuple(int(v) for v in ncell)
+        self.step = tuple(float(v) for v in step)
+        self.ncps = tuple(int(v) for v in ncps)
+        self.prec = prec
+        self.posmode = (POS_ABS if prec == F64 else POS_SPLIT) if posmode is None else posmode
+        h = ctypes.c_void_p()
+        ctx.check(lib().pk_ens_create(ctx.h, prec, self.posmode, _i3(self.ncell), _d3(self.step), _i3(self.ncps),
+                                      int(type_index), ctypes.byref(h)))
+        self.h = h
+        ctx._adopt(self)
+
+    def set_slack(self, frac, add):
+        self.ctx.check(lib().pk_ens_set_slack(self.h, float(frac), int(add)))
+
+    def add(self, x, u, w):
+        """EnsembleSupercells::add for every row of x (n x 3 positions), u (n x 3, p/(mc)), w."""
+        x = np.asarray(x, np.float64)
+        u = np.asarray(u, np.float64)
+        cols = [np.ascontiguousarray(x[:, d]) for d in range(3)] + [np.ascontiguousarray(u[:, d]) for d in range(3)]
+        cols.append(np.ascontiguousarray(w, np.float64))
+        self.ctx.check(lib().pk_ens_upload(self.h, x.shape[0], *[_p(c) for c in cols]))
+
+    def load_thermal(self, ppc, uth, drift_x, weight, seed):
+        self.ctx.check(lib().pk_ens_load_thermal(self.h, int(ppc), float(uth), float(drift_x), float(weight),
+                                                 int(seed) & 0xFFFFFFFFFFFFFFFF))
+
+    def size(self):
+        n = lib().pk_ens_size(self.h)
+        if n < 0:
+            self.ctx.check(PK_E_STATE)
+        return int(n)
+
+    def num_supercells(self):
+        return int(lib().pk_ens_num_supercells(self.h))
+
+    def particles(self):
+        """(x n x 3, u n x 3, w n, off nsc+1) in supercell order."""
+        n = self.size()
+        cols = [np.empty(n) for _ in range(7)]
+        off = np.empty(self.num_supercells() + 1, np.int64)
+        self.ctx.check(lib().pk_ens_download(self.h, *[_p(c) for c in cols], _p(off)))
+        return np.stack(cols[:3], 1), np.stack(cols[3:6], 1), cols[6], off
+
+    def cells(self):
+        """(per-cell counts over the physical cells, linear cell index of every particle)."""
+        n = self.size()
+        counts = np.empty(int(np.prod(self.ncell)), np.int32)
+        cells = np.empty(max(n, 1), np.int32)
+        self.ctx.check(lib().pk_ens_cells(self.h, _p(counts), _p(cells)))
+        return counts.reshape(self.ncell), cells[:n]
+
+    def resort(self):
+        self.ctx.check(lib().pk_ens_resort(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            if self.ctx.h:
+                lib().pk_ens_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def particle_step(ens, grid, dt, shape=CIC, dep=DIRECT):
+    ens.ctx.check(lib().pk_particle_step(ens.h, grid.h, float(dt), shape, dep))
+
+
+def step(ens, grid, dt, shape=CIC, dep=DIRECT):
+    ens.ctx.check(lib().pk_step(ens.h, grid.h, float(dt), shape, dep))
+
+
+def run(ens, grid, dt, nsteps, shape=CIC, dep=DIRECT):
+    ens.ctx.check(lib().pk_run(ens.h, grid.h, float(dt), shape, dep, int(nsteps)))
+
+
+def deposit_rho(ens, grid, shape=CIC):
+    ens.ctx.check(lib().pk_deposit_rho(ens.h, grid.h, shape))
+
+
+def diag(ens, grid, shape=CIC):
+    out = np.zeros(DIAG_N)
+    ens.ctx.check(lib().pk_diag(ens.h, grid.h, shape, _p(out)))
+    return out
+
+
+def diag_set_reference(ens, grid, shape=CIC):
+    ens.ctx.check(lib().pk_diag_set_reference(ens.h, grid.h, shape))
+
+
+def plasma_parameters(dt, ppc, omega_p_dt=0.1, step=1.0):
+    """Electron density n0 = omega_p^2 m / (4 pi e^2) for omega_p * dt = omega_p_dt, and the
+    macro-particle weight n0 h^3 / ppc (CGS, pica's constants)."""
+    wp = omega_p_dt / dt
+    n0 = wp * wp * ELECTRON_MASS / (4.0 * PI * ELECTRON_CHARGE * ELECTRON_CHARGE)
+    return n0, n0 * step ** 3 / ppc

@@ -1,0 +1,759 @@
+// pk_ensemble.hip -- supercell-ordered particle storage on the device (gfx950).
+//
+// Layout: supercell s owns slots [off[s], off[s+1]) of every SoA field array; cnt[s] of them
+// are live.  The slack (capacity - count) absorbs the particles that arrive each step, so the
+// per-step re-sort moves only the few particles that changed supercell:
+//   - the particle kernel leaves stayers compacted in [0, nstay[s]) and writes leavers to the
+//     outbox (same slot layout) with their direction class (0..26, 13 = none);
+//   - k_insert pulls, for each direction class d in a fixed order, the class-d leavers of the
+//     neighbour s - delta(d) and appends them.  No atomics decide the order: the result is
+//     deterministic and equals the oracle's re-sort rule.
+//   - if a supercell's slack is exhausted, a rebuild (stable rocPRIM radix sort on
+//     dest * 28 + class rank) recomputes all capacities; it yields the very same order.
+// pica's EnsembleSupercells (EnsembleSupercells.h:15-69) never moves particles between
+// supercells; this is the device counterpart that the periodic PIC step needs.
+#include <string.h>
+
+#include "pk_internal.h"
+
+#include <rocprim/rocprim.hpp>
+
+namespace {
+
+inline unsigned nblk(long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+template <class W>
+struct Fields {
+    W* f[8];
+};
+
+template <class W>
+Fields<W> fields_of(const pk_ens* e, void* buf)
+{
+    Fields<W> r;
+    for (int k = 0; k < 8; k++) r.f[k] = k < e->nfield() ? (W*)e->field(buf, k) : nullptr;
+    return r;
+}
+
+__device__ __forceinline__ long sc_neighbour(const EnsGeom& eg, long s, int code, int sign)
+{
+    int c[3] = {(int)(s / ((long)eg.nsc[1] * eg.nsc[2])), (int)((s / eg.nsc[2]) % eg.nsc[1]), (int)(s % eg.nsc[2])};
+    const int dd[3] = {code / 9 - 1, (code / 3) % 3 - 1, code % 3 - 1};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        c[d] += sign * dd[d];
+        if (c[d] < 0) c[d] += eg.nsc[d];
+        if (c[d] >= eg.nsc[d]) c[d] -= eg.nsc[d];
+    }
+    return ((long)c[0] * eg.nsc[1] + c[1]) * eg.nsc[2] + c[2];
+}
+
+// one wavefront per supercell: append incoming movers, class by class
+template <class W, int NF>
+__global__ void __launch_bounds__(64) k_insert(EnsGeom eg, Fields<W> pm, Fields<W> po, const uint8_t* __restrict__ odir,
+                                               const int64_t* __restrict__ off, int32_t* __restrict__ cnt,
+                                               const int32_t* __restrict__ nstay, const int32_t* __restrict__ nmove,
+                                               const int32_t* __restrict__ mvc, int32_t* __restrict__ flags)
+{
+    const long s = pk_xcd_supercell(blockIdx.x, eg.xcd_chunk);
+    if (s >= eg.nsct) return;
+    const int lane = threadIdx.x;
+    const int nfinal = nstay[s];
+    const long cap = off[s + 1] - off[s];
+    int term = 0;
+    if (lane < 27 && lane != 13) term = mvc[sc_neighbour(eg, s, lane, -1) * 27 + lane];
+    int nin = term;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) nin += __shfl_xor(nin, o);
+    if ((long)nfinal + nin > cap) {
+        if (lane == 0) atomicOr(&flags[PK_FLAG_OVERFLOW], 1);
+        return;
+    }
+    if (lane == 0) cnt[s] = nfinal + nin;
+    long p = off[s] + nfinal;
+    for (int d = 0; d < 27; d++) {
+        if (d == 13) continue;
+        const long src = sc_neighbour(eg, s, d, -1);
+        const int m = mvc[src * 27 + d];
+        if (m == 0) continue;
+        const int nm = nmove[src];
+        const long ob = off[src];
+        int found = 0;
+        for (int k0 = 0; k0 < nm && found < m; k0 += 64) {
+            const int k = k0 + lane;
+            const bool f = k < nm && odir[ob + k] == d;
+            const uint64_t b = __ballot(f);
+            if (f) {
+                const long dst = p + found + __popcll(b & ((1ull << lane) - 1ull));
+#pragma unroll
+                for (int q = 0; q < NF; q++) pm.f[q][dst] = po.f[q][ob + k];
+            }
+            found += __popcll(b);
+        }
+        p += m;
+    }
+}
+
+// ---- rebuild ----
+constexpr uint64_t SRC_MAIN = 0, SRC_OUT = 1ull << 62, SRC_STAGE = 2ull << 62;
+constexpr uint64_t SRC_MASK = (1ull << 62) - 1;
+
+__global__ void k_entries_per_sc(long nsct, const int32_t* nstay, const int32_t* nmove, int64_t* n)
+{
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nsct) n[s] = (int64_t)nstay[s] + nmove[s];
+}
+
+__global__ void __launch_bounds__(64) k_keys_main(EnsGeom eg, const int64_t* off, const int32_t* nstay,
+                                                  const int32_t* nmove, const uint8_t* odir, const int64_t* ebase,
+                                                  uint32_t* keys, uint64_t* srcs)
+{
+    const long s = blockIdx.x;
+    if (s >= eg.nsct) return;
+    const long b = ebase[s];
+    const int ns = nstay[s], nm = nmove[s];
+    for (int j = threadIdx.x; j < ns; j += 64) {
+        keys[b + j] = (uint32_t)(s * 28);
+        srcs[b + j] = SRC_MAIN | (uint64_t)(off[s] + j);
+    }
+    for (int k = threadIdx.x; k < nm; k += 64) {
+        const int code = odir[off[s] + k];
+        const long dst = sc_neighbour(eg, s, code, +1);
+        keys[b + ns + k] = (uint32_t)(dst * 28 + code + 1);
+        srcs[b + ns + k] = SRC_OUT | (uint64_t)(off[s] + k);
+    }
+}
+
+struct StageGeom {
+    double cellinv[3], scinv[3], L[3];
+};
+
+// supercell key of a host-supplied particle: pica's getSupercellIndex for absolute positions
+// (EnsembleSupercells.h:56); for the split form the supercell of the particle's cell.
+__global__ void k_keys_stage(EnsGeom eg, StageGeom sg, int split, const double* x, const double* y, const double* z,
+                             long n, uint32_t* keys, uint64_t* srcs, int32_t* flags)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double p[3] = {x[i], y[i], z[i]};
+    int sc[3];
+    bool ok = true;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        if (!(p[d] >= 0.0 && p[d] < sg.L[d])) ok = false;
+        if (split) {
+            int c = (int)floor(p[d] * sg.cellinv[d]);
+            c = c < 0 ? 0 : (c >= eg.ncell[d] ? eg.ncell[d] - 1 : c);
+            sc[d] = c / eg.ncps[d];
+        } else {
+            sc[d] = (int)floor((p[d] - 0.0) * sg.scinv[d]);
+            if (sc[d] < 0 || sc[d] >= eg.nsc[d]) ok = false;
+            sc[d] = sc[d] < 0 ? 0 : (sc[d] >= eg.nsc[d] ? eg.nsc[d] - 1 : sc[d]);
+        }
+    }
+    if (!ok) atomicOr(&flags[PK_FLAG_ERR], 1);
+    keys[i] = (uint32_t)((((long)sc[0] * eg.nsc[1] + sc[1]) * eg.nsc[2] + sc[2]) * 28);
+    srcs[i] = SRC_STAGE | (uint64_t)i;
+}
+
+__global__ void k_count_keys(long n, const uint32_t* keys, int32_t* cnt)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) atomicAdd(&cnt[keys[i] / 28], 1);
+}
+
+__global__ void k_caps(long nsct, const int32_t* cnt, double frac, int add, int64_t* cap64, int64_t* cnt64,
+                       int32_t* maxcap)
+{
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsct) return;
+    const int c = cnt[s];
+    const long cap = (long)c + (long)(c * frac) + add;
+    cap64[s] = cap;
+    cnt64[s] = c;
+    atomicMax(maxcap, (int)cap);
+}
+
+template <class W, int NF>
+__global__ void k_gather(long n, const uint32_t* keys, const uint64_t* srcs, const int64_t* newoff,
+                         const int64_t* runstart, Fields<W> dst, Fields<W> from_main, Fields<W> from_out)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long d = keys[i] / 28;
+    const long slot = newoff[d] + (i - runstart[d]);
+    const uint64_t sv = srcs[i];
+    const long si = (long)(sv & SRC_MASK);
+    const Fields<W>& from = (sv >> 62) == 0 ? from_main : from_out;
+#pragma unroll
+    for (int q = 0; q < NF; q++) dst.f[q][slot] = from.f[q][si];
+}
+
+// staged FP64 particles -> FP64 absolute storage
+__global__ void k_gather_stage_f64(long n, const uint32_t* keys, const uint64_t* srcs, const int64_t* newoff,
+                                   const int64_t* runstart, PartF64 dst, Fields<double> st)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long d = keys[i] / 28;
+    const long slot = newoff[d] + (i - runstart[d]);
+    const long si = (long)(srcs[i] & SRC_MASK);
+    dst.x[slot] = st.f[0][si]; dst.y[slot] = st.f[1][si]; dst.z[slot] = st.f[2][si];
+    dst.ux[slot] = st.f[3][si]; dst.uy[slot] = st.f[4][si]; dst.uz[slot] = st.f[5][si];
+    dst.w[slot] = st.f[6][si];
+}
+
+// cell index and in-cell offset of an absolute position for the split form
+__device__ __forceinline__ void to_split(double x, double cellinv, int ncell, int& c, float& o)
+{
+    const double t = x * cellinv;
+    c = (int)floor(t);
+    c = c < 0 ? 0 : (c >= ncell ? ncell - 1 : c);
+    float f = (float)(t - (double)c);
+    if (!(f >= 0.0f)) f = 0.0f;
+    if (f >= 1.0f) f = 0x1.fffffep-1f;   // largest float below 1
+    o = f;
+}
+
+__global__ void k_gather_stage_split(EnsGeom eg, StageGeom sg, long n, const uint32_t* keys, const uint64_t* srcs,
+                                     const int64_t* newoff, const int64_t* runstart, PartSplit dst, Fields<double> st)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long d = keys[i] / 28;
+    const long slot = newoff[d] + (i - runstart[d]);
+    const long si = (long)(srcs[i] & SRC_MASK);
+    int c[3];
+    float o[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) to_split(st.f[k][si], sg.cellinv[k], eg.ncell[k], c[k], o[k]);
+    dst.cell[slot] = (c[0] * eg.ncell[1] + c[1]) * eg.ncell[2] + c[2];
+    dst.ox[slot] = o[0]; dst.oy[slot] = o[1]; dst.oz[slot] = o[2];
+    dst.ux[slot] = (float)st.f[3][si]; dst.uy[slot] = (float)st.f[4][si]; dst.uz[slot] = (float)st.f[5][si];
+    dst.w[slot] = (float)st.f[6][si];
+}
+
+__global__ void k_reset_after_rebuild(long nsct, int32_t* nstay, int32_t* nmove, const int32_t* cnt)
+{
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nsct) {
+        nstay[s] = cnt[s];
+        nmove[s] = 0;
+    }
+}
+
+// ---- loader ----
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// uniform in [0, 1) from the counter (seed, particle id, stream j)
+__device__ __forceinline__ double urand(uint64_t key, int j)
+{
+    return (double)(mix64(key + (uint64_t)j * 0xD1B54A32D192ED03ull) >> 11) * 0x1.0p-53;
+}
+
+struct LoadParams {
+    int ppc;
+    double uth, drift, weight;
+    uint64_t seed;
+    double step[3];
+};
+
+__device__ __forceinline__ void thermal_particle(const EnsGeom& eg, const LoadParams& lp, long s, int l, int gc[3],
+                                                 double U[3], double u[3])
+{
+    const int sidx[3] = {(int)(s / ((long)eg.nsc[1] * eg.nsc[2])), (int)((s / eg.nsc[2]) % eg.nsc[1]),
+                         (int)(s % eg.nsc[2])};
+    const int lc = l / lp.ppc, k = l % lp.ppc;
+    const int a = lc / (eg.ncps[1] * eg.ncps[2]), b = (lc / eg.ncps[2]) % eg.ncps[1], c = lc % eg.ncps[2];
+    gc[0] = sidx[0] * eg.ncps[0] + a;
+    gc[1] = sidx[1] * eg.ncps[1] + b;
+    gc[2] = sidx[2] * eg.ncps[2] + c;
+    const uint64_t id = ((((uint64_t)gc[0] * eg.ncell[1] + gc[1]) * eg.ncell[2] + gc[2]) * lp.ppc) + k;
+    const uint64_t key = mix64(lp.seed ^ mix64(id));
+    U[0] = urand(key, 0); U[1] = urand(key, 1); U[2] = urand(key, 2);
+    // Box-Muller normals from streams 3..6
+    const double r1 = sqrt(-2.0 * log(1.0 - urand(key, 3))), a1 = 2.0 * PK_PI * urand(key, 4);
+    const double r2 = sqrt(-2.0 * log(1.0 - urand(key, 5))), a2 = 2.0 * PK_PI * urand(key, 6);
+    u[0] = lp.uth * r1 * cos(a1) + ((id & 1ull) ? -lp.drift : lp.drift);
+    u[1] = lp.uth * r1 * sin(a1);
+    u[2] = lp.uth * r2 * cos(a2);
+}
+
+__global__ void __launch_bounds__(256) k_load_f64(EnsGeom eg, LoadParams lp, const int64_t* off, int per_sc, PartF64 pm)
+{
+    const long s = blockIdx.x;
+    for (int l = threadIdx.x; l < per_sc; l += blockDim.x) {
+        int gc[3];
+        double U[3], u[3];
+        thermal_particle(eg, lp, s, l, gc, U, u);
+        const long p = off[s] + l;
+        pm.x[p] = ((double)gc[0] + U[0]) * lp.step[0];
+        pm.y[p] = ((double)gc[1] + U[1]) * lp.step[1];
+        pm.z[p] = ((double)gc[2] + U[2]) * lp.step[2];
+        pm.ux[p] = u[0]; pm.uy[p] = u[1]; pm.uz[p] = u[2];
+        pm.w[p] = lp.weight;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_load_split(EnsGeom eg, LoadParams lp, const int64_t* off, int per_sc,
+                                                    PartSplit pm)
+{
+    const long s = blockIdx.x;
+    for (int l = threadIdx.x; l < per_sc; l += blockDim.x) {
+        int gc[3];
+        double U[3], u[3];
+        thermal_particle(eg, lp, s, l, gc, U, u);
+        const long p = off[s] + l;
+        float o[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            // the same rounding as the absolute form's (gc + U) * h, taken relative to the cell
+            const double xa = ((double)gc[d] + U[d]) * lp.step[d];
+            int c;
+            to_split(xa, 1.0 / lp.step[d], eg.ncell[d], c, o[d]);
+            if (c != gc[d]) o[d] = c < gc[d] ? 0.0f : 0x1.fffffep-1f;
+        }
+        pm.cell[p] = (gc[0] * eg.ncell[1] + gc[1]) * eg.ncell[2] + gc[2];
+        pm.ox[p] = o[0]; pm.oy[p] = o[1]; pm.oz[p] = o[2];
+        pm.ux[p] = (float)u[0]; pm.uy[p] = (float)u[1]; pm.uz[p] = (float)u[2];
+        pm.w[p] = (float)lp.weight;
+    }
+}
+
+__global__ void k_fill_i32(int32_t* p, long n, int32_t v)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_uniform_off(long nsct, int64_t cap, int64_t* off)
+{
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s <= nsct) off[s] = s * cap;
+}
+
+// ---- download / cells ----
+__global__ void __launch_bounds__(256) k_compact_f64(EnsGeom eg, PartF64 pm, const int64_t* off, const int32_t* cnt,
+                                                     const int64_t* runstart, Fields<double> out)
+{
+    const long s = blockIdx.x;
+    const int n = cnt[s];
+    for (int l = threadIdx.x; l < n; l += blockDim.x) {
+        const long p = off[s] + l, o = runstart[s] + l;
+        out.f[0][o] = pm.x[p]; out.f[1][o] = pm.y[p]; out.f[2][o] = pm.z[p];
+        out.f[3][o] = pm.ux[p]; out.f[4][o] = pm.uy[p]; out.f[5][o] = pm.uz[p];
+        out.f[6][o] = pm.w[p];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_compact_split(EnsGeom eg, PartSplit pm, const int64_t* off, const int32_t* cnt,
+                                                       const int64_t* runstart, Fields<double> out)
+{
+    const long s = blockIdx.x;
+    const int n = cnt[s];
+    for (int l = threadIdx.x; l < n; l += blockDim.x) {
+        const long p = off[s] + l, o = runstart[s] + l;
+        const int c = pm.cell[p];
+        const int c0 = c / (eg.ncell[1] * eg.ncell[2]), c1 = (c / eg.ncell[2]) % eg.ncell[1], c2 = c % eg.ncell[2];
+        out.f[0][o] = ((double)c0 + (double)pm.ox[p]) * eg.step[0];
+        out.f[1][o] = ((double)c1 + (double)pm.oy[p]) * eg.step[1];
+        out.f[2][o] = ((double)c2 + (double)pm.oz[p]) * eg.step[2];
+        out.f[3][o] = pm.ux[p]; out.f[4][o] = pm.uy[p]; out.f[5][o] = pm.uz[p];
+        out.f[6][o] = pm.w[p];
+    }
+}
+
+// linear physical cell of every particle (supercell order) and per-cell counts.  For the
+// absolute form the key is floor((x - min) * inverse(cellSize)) per dimension, pica's
+// getSupercellIndex with one cell per supercell (EnsembleSupercells.h:31-33,56).
+__global__ void __launch_bounds__(256) k_cells(EnsGeom eg, StageGeom sg, int split, Fields<double> pf,
+                                               const int32_t* scell, const int64_t* off, const int32_t* cnt,
+                                               const int64_t* runstart, int32_t* counts, int32_t* cells)
+{
+    const long s = blockIdx.x;
+    const int n = cnt[s];
+    for (int l = threadIdx.x; l < n; l += blockDim.x) {
+        const long p = off[s] + l;
+        int c;
+        if (split) {
+            c = scell[p];
+        } else {
+            int k[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) k[d] = (int)floor((pf.f[d][p] - 0.0) * sg.cellinv[d]);
+            c = (k[0] * eg.ncell[1] + k[1]) * eg.ncell[2] + k[2];
+        }
+        if (cells) cells[runstart[s] + l] = c;
+        if (counts) atomicAdd(&counts[c], 1);
+    }
+}
+
+StageGeom stage_geom(const pk_ens* e)
+{
+    StageGeom sg;
+    for (int d = 0; d < 3; d++) {
+        sg.L[d] = (double)e->ncell[d] * e->step[d];
+        const double cellSize = (sg.L[d] - 0.0) / (double)e->ncell[d];   // EnsembleSupercells.h:31
+        sg.cellinv[d] = 1.0 / cellSize;
+        sg.scinv[d] = 1.0 / ((double)e->ncps[d] * cellSize);            // :32-33
+    }
+    return sg;
+}
+
+// RAII device buffer
+struct DBuf {
+    void* p = nullptr;
+    ~DBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t alloc(size_t bytes)
+    {
+        return hipMalloc(&p, bytes ? bytes : 16);
+    }
+    template <class T>
+    T* as() const
+    {
+        return (T*)p;
+    }
+};
+
+template <class T>
+int exclusive_scan(pk_ctx* c, const T* in, T* out, long n)
+{
+    size_t bytes = 0;
+    PK_HIP(c, rocprim::exclusive_scan(nullptr, bytes, in, out, T(0), (size_t)n, rocprim::plus<T>(), c->stream));
+    DBuf tmp;
+    PK_HIP(c, tmp.alloc(bytes));
+    PK_HIP(c, rocprim::exclusive_scan(tmp.p, bytes, in, out, T(0), (size_t)n, rocprim::plus<T>(), c->stream));
+    return PK_OK;
+}
+
+}  // namespace
+
+EnsGeom ens_geom(const pk_ens* e)
+{
+    EnsGeom g;
+    for (int d = 0; d < 3; d++) {
+        g.nsc[d] = e->nsc[d];
+        g.ncps[d] = e->ncps[d];
+        g.ncell[d] = e->ncell[d];
+        g.step[d] = e->step[d];
+    }
+    g.nsct = e->nsct;
+    g.xcd_chunk = (int)((e->nsct + PK_XCD - 1) / PK_XCD);
+    return g;
+}
+
+int pk_ens_alloc(pk_ens* e, size_t capacity)
+{
+    pk_ctx* c = e->ctx;
+    void *m = nullptr, *o = nullptr;
+    uint8_t* od = nullptr;
+    const size_t bytes = capacity * e->esize() * e->nfield();
+    if (hipMalloc(&m, bytes ? bytes : 16) != hipSuccess || hipMalloc(&o, bytes ? bytes : 16) != hipSuccess ||
+        hipMalloc((void**)&od, capacity ? capacity : 16) != hipSuccess) {
+        if (m) (void)hipFree(m);
+        if (o) (void)hipFree(o);
+        if (od) (void)hipFree(od);
+        (void)hipGetLastError();
+        return pk_fail(c, PK_E_ALLOC, "particle buffers: hipMalloc failed");
+    }
+    if (e->main) (void)hipFree(e->main);
+    if (e->outbox) (void)hipFree(e->outbox);
+    if (e->odir) (void)hipFree(e->odir);
+    e->main = m;
+    e->outbox = o;
+    e->odir = od;
+    e->capacity = capacity;
+    return PK_OK;
+}
+
+int pk_ens_insert_launch(pk_ens* e)
+{
+    EnsGeom eg = ens_geom(e);
+    const unsigned nb = (unsigned)(PK_XCD * (long)eg.xcd_chunk);
+    pk_ktimer tm(e->ctx, "insert");
+    if (e->prec == PK_F64)
+        hipLaunchKernelGGL((k_insert<uint64_t, 7>), dim3(nb), dim3(64), 0, e->ctx->stream, eg,
+                           fields_of<uint64_t>(e, e->main), fields_of<uint64_t>(e, e->outbox), e->odir, e->off, e->cnt,
+                           e->nstay, e->nmove, e->mvc, e->flags);
+    else
+        hipLaunchKernelGGL((k_insert<uint32_t, 8>), dim3(nb), dim3(64), 0, e->ctx->stream, eg,
+                           fields_of<uint32_t>(e, e->main), fields_of<uint32_t>(e, e->outbox), e->odir, e->off, e->cnt,
+                           e->nstay, e->nmove, e->mvc, e->flags);
+    PK_HIP(e->ctx, hipGetLastError());
+    return PK_OK;
+}
+
+// Stable counting rebuild.  source 0: current main (stayers) + outbox (movers);
+// source 1: staged host upload (stage pointer in e->outbox, n entries in np).
+static int rebuild_from(pk_ens* e, long nent, DBuf& keys_in, DBuf& srcs_in, Fields<double>* stage)
+{
+    pk_ctx* c = e->ctx;
+    const long nsct = e->nsct;
+    DBuf keys_out, srcs_out, tmp, cnt64, cap64, newoff, runstart, newcnt, maxcap;
+    PK_HIP(c, keys_out.alloc(sizeof(uint32_t) * nent));
+    PK_HIP(c, srcs_out.alloc(sizeof(uint64_t) * nent));
+    unsigned bits = 1;
+    while (((uint64_t)1 << bits) < (uint64_t)nsct * 28) bits++;
+    size_t bytes = 0;
+    PK_HIP(c, rocprim::radix_sort_pairs(nullptr, bytes, keys_in.as<uint32_t>(), keys_out.as<uint32_t>(),
+                                        srcs_in.as<uint64_t>(), srcs_out.as<uint64_t>(), (size_t)nent, 0u, bits,
+                                        c->stream));
+    PK_HIP(c, tmp.alloc(bytes));
+    PK_HIP(c, rocprim::radix_sort_pairs(tmp.p, bytes, keys_in.as<uint32_t>(), keys_out.as<uint32_t>(),
+                                        srcs_in.as<uint64_t>(), srcs_out.as<uint64_t>(), (size_t)nent, 0u, bits,
+                                        c->stream));
+    PK_HIP(c, newcnt.alloc(sizeof(int32_t) * nsct));
+    PK_HIP(c, cnt64.alloc(sizeof(int64_t) * (nsct + 1)));
+    PK_HIP(c, cap64.alloc(sizeof(int64_t) * (nsct + 1)));
+    PK_HIP(c, newoff.alloc(sizeof(int64_t) * (nsct + 1)));
+    PK_HIP(c, runstart.alloc(sizeof(int64_t) * (nsct + 1)));
+    PK_HIP(c, maxcap.alloc(sizeof(int32_t)));
+    hipLaunchKernelGGL(k_fill_i32, dim3(nblk(nsct, 256)), dim3(256), 0, c->stream, newcnt.as<int32_t>(), nsct, 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, c->stream, maxcap.as<int32_t>(), 1L, 0);
+    if (nent > 0)
+        hipLaunchKernelGGL(k_count_keys, dim3(nblk(nent, 256)), dim3(256), 0, c->stream, nent, keys_out.as<uint32_t>(),
+                           newcnt.as<int32_t>());
+    hipLaunchKernelGGL(k_caps, dim3(nblk(nsct, 256)), dim3(256), 0, c->stream, nsct, newcnt.as<int32_t>(),
+                       e->slack_frac, e->slack_add, cap64.as<int64_t>(), cnt64.as<int64_t>(), maxcap.as<int32_t>());
+    PK_HIP(c, hipGetLastError());
+    // one extra zero element so that the scans also give the totals at index nsct
+    PK_HIP(c, hipMemsetAsync(cap64.as<int64_t>() + nsct, 0, sizeof(int64_t), c->stream));
+    PK_HIP(c, hipMemsetAsync(cnt64.as<int64_t>() + nsct, 0, sizeof(int64_t), c->stream));
+    int r = exclusive_scan<int64_t>(c, cap64.as<int64_t>(), newoff.as<int64_t>(), nsct + 1);
+    if (r) return r;
+    r = exclusive_scan<int64_t>(c, cnt64.as<int64_t>(), runstart.as<int64_t>(), nsct + 1);
+    if (r) return r;
+    int64_t total_cap = 0;
+    int32_t mc = 0;
+    PK_HIP(c, hipMemcpyAsync(&total_cap, newoff.as<int64_t>() + nsct, sizeof(int64_t), hipMemcpyDeviceToHost,
+                             c->stream));
+    PK_HIP(c, hipMemcpyAsync(&mc, maxcap.p, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    PK_HIP(c, hipStreamSynchronize(c->stream));
+    if (mc > PK_MAX_SC_PARTICLES)
+        return pk_fail(c, PK_E_CAPACITY, "supercell holds more than 65536 particles; use smaller supercells");
+    // new storage
+    void *nmain = nullptr, *nout = nullptr;
+    uint8_t* nodir = nullptr;
+    const size_t fbytes = (size_t)total_cap * e->esize();
+    const size_t bbytes = fbytes * e->nfield();
+    if (hipMalloc(&nmain, bbytes ? bbytes : 16) != hipSuccess || hipMalloc(&nout, bbytes ? bbytes : 16) != hipSuccess ||
+        hipMalloc((void**)&nodir, total_cap ? total_cap : 16) != hipSuccess) {
+        if (nmain) (void)hipFree(nmain);
+        if (nout) (void)hipFree(nout);
+        if (nodir) (void)hipFree(nodir);
+        (void)hipGetLastError();
+        return pk_fail(c, PK_E_ALLOC, "rebuild: hipMalloc of particle buffers failed");
+    }
+    pk_ens tmpe = *e;
+    tmpe.capacity = (size_t)total_cap;
+    EnsGeom eg = ens_geom(e);
+    if (nent > 0) {
+        if (stage) {
+            if (e->prec == PK_F64)
+                hipLaunchKernelGGL(k_gather_stage_f64, dim3(nblk(nent, 256)), dim3(256), 0, c->stream, nent,
+                                   keys_out.as<uint32_t>(), srcs_out.as<uint64_t>(), newoff.as<int64_t>(),
+                                   runstart.as<int64_t>(), part_f64(&tmpe, nmain), *stage);
+            else
+                hipLaunchKernelGGL(k_gather_stage_split, dim3(nblk(nent, 256)), dim3(256), 0, c->stream, eg,
+                                   stage_geom(e), nent, keys_out.as<uint32_t>(), srcs_out.as<uint64_t>(),
+                                   newoff.as<int64_t>(), runstart.as<int64_t>(), part_split(&tmpe, nmain), *stage);
+        } else {
+            if (e->prec == PK_F64)
+                hipLaunchKernelGGL((k_gather<uint64_t, 7>), dim3(nblk(nent, 256)), dim3(256), 0, c->stream, nent,
+                                   keys_out.as<uint32_t>(), srcs_out.as<uint64_t>(), newoff.as<int64_t>(),
+                                   runstart.as<int64_t>(), fields_of<uint64_t>(&tmpe, nmain),
+                                   fields_of<uint64_t>(e, e->main), fields_of<uint64_t>(e, e->outbox));
+            else
+                hipLaunchKernelGGL((k_gather<uint32_t, 8>), dim3(nblk(nent, 256)), dim3(256), 0, c->stream, nent,
+                                   keys_out.as<uint32_t>(), srcs_out.as<uint64_t>(), newoff.as<int64_t>(),
+                                   runstart.as<int64_t>(), fields_of<uint32_t>(&tmpe, nmain),
+                                   fields_of<uint32_t>(e, e->main), fields_of<uint32_t>(e, e->outbox));
+        }
+        PK_HIP(c, hipGetLastError());
+    }
+    PK_HIP(c, hipMemcpyAsync(e->off, newoff.p, sizeof(int64_t) * (nsct + 1), hipMemcpyDeviceToDevice, c->stream));
+    PK_HIP(c, hipMemcpyAsync(e->cnt, newcnt.p, sizeof(int32_t) * nsct, hipMemcpyDeviceToDevice, c->stream));
+    hipLaunchKernelGGL(k_reset_after_rebuild, dim3(nblk(nsct, 256)), dim3(256), 0, c->stream, nsct, e->nstay, e->nmove,
+                       e->cnt);
+    PK_HIP(c, hipGetLastError());
+    PK_HIP(c, hipStreamSynchronize(c->stream));
+    if (e->main) (void)hipFree(e->main);
+    if (e->outbox) (void)hipFree(e->outbox);
+    if (e->odir) (void)hipFree(e->odir);
+    e->main = nmain;
+    e->outbox = nout;
+    e->odir = nodir;
+    e->capacity = (size_t)total_cap;
+    e->maxcap = mc;
+    e->np = nent;
+    e->pending_movers = false;
+    return PK_OK;
+}
+
+int pk_ens_rebuild(pk_ens* e, int source)
+{
+    (void)source;
+    pk_ctx* c = e->ctx;
+    const long nsct = e->nsct;
+    pk_ktimer tm(c, "rebuild");
+    DBuf ecount, ebase;
+    PK_HIP(c, ecount.alloc(sizeof(int64_t) * (nsct + 1)));
+    PK_HIP(c, ebase.alloc(sizeof(int64_t) * (nsct + 1)));
+    hipLaunchKernelGGL(k_entries_per_sc, dim3(nblk(nsct, 256)), dim3(256), 0, c->stream, nsct, e->nstay, e->nmove,
+                       ecount.as<int64_t>());
+    PK_HIP(c, hipMemsetAsync(ecount.as<int64_t>() + nsct, 0, sizeof(int64_t), c->stream));
+    int r = exclusive_scan<int64_t>(c, ecount.as<int64_t>(), ebase.as<int64_t>(), nsct + 1);
+    if (r) return r;
+    int64_t nent = 0;
+    PK_HIP(c, hipMemcpyAsync(&nent, ebase.as<int64_t>() + nsct, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+    PK_HIP(c, hipStreamSynchronize(c->stream));
+    DBuf keys, srcs;
+    PK_HIP(c, keys.alloc(sizeof(uint32_t) * nent));
+    PK_HIP(c, srcs.alloc(sizeof(uint64_t) * nent));
+    EnsGeom eg = ens_geom(e);
+    hipLaunchKernelGGL(k_keys_main, dim3((unsigned)nsct), dim3(64), 0, c->stream, eg, e->off, e->nstay, e->nmove,
+                       e->odir, ebase.as<int64_t>(), keys.as<uint32_t>(), srcs.as<uint64_t>());
+    PK_HIP(c, hipGetLastError());
+    return rebuild_from(e, nent, keys, srcs, nullptr);
+}
+
+int pk_ens_upload_impl(pk_ens* e, size_t n, const double* const* host)
+{
+    pk_ctx* c = e->ctx;
+    DBuf stage, keys, srcs;
+    PK_HIP(c, stage.alloc(sizeof(double) * 7 * n));
+    Fields<double> st;
+    for (int k = 0; k < 8; k++) st.f[k] = k < 7 ? stage.as<double>() + (size_t)k * n : nullptr;
+    for (int k = 0; k < 7; k++)
+        PK_HIP(c, hipMemcpyAsync(st.f[k], host[k], sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    PK_HIP(c, keys.alloc(sizeof(uint32_t) * n));
+    PK_HIP(c, srcs.alloc(sizeof(uint64_t) * n));
+    PK_HIP(c, hipMemsetAsync(e->flags, 0, sizeof(int32_t) * PK_NFLAGS, c->stream));
+    if (n > 0)
+        hipLaunchKernelGGL(k_keys_stage, dim3(nblk((long)n, 256)), dim3(256), 0, c->stream, ens_geom(e), stage_geom(e),
+                           e->posmode == PK_POS_SPLIT ? 1 : 0, st.f[0], st.f[1], st.f[2], (long)n, keys.as<uint32_t>(),
+                           srcs.as<uint64_t>(), e->flags);
+    PK_HIP(c, hipGetLastError());
+    int32_t fl[PK_NFLAGS];
+    PK_HIP(c, hipMemcpyAsync(fl, e->flags, sizeof(fl), hipMemcpyDeviceToHost, c->stream));
+    PK_HIP(c, hipStreamSynchronize(c->stream));
+    if (fl[PK_FLAG_ERR]) {
+        PK_HIP(c, hipMemsetAsync(e->flags, 0, sizeof(int32_t) * PK_NFLAGS, c->stream));
+        return pk_fail(c, PK_E_ARG, "pk_ens_upload: particle outside the domain [0, ncell*step) or not finite");
+    }
+    pk_ktimer tm(c, "rebuild");
+    return rebuild_from(e, (long)n, keys, srcs, &st);
+}
+
+int pk_ens_load_impl(pk_ens* e, int ppc, double uth, double drift, double weight, uint64_t seed)
+{
+    pk_ctx* c = e->ctx;
+    const int per_sc = e->ncps[0] * e->ncps[1] * e->ncps[2] * ppc;
+    const long cap = (long)per_sc + (long)(per_sc * e->slack_frac) + e->slack_add;
+    if (cap > PK_MAX_SC_PARTICLES)
+        return pk_fail(c, PK_E_CAPACITY, "pk_ens_load_thermal: more than 65536 particles per supercell");
+    const size_t capacity = (size_t)cap * (size_t)e->nsct;
+    int r = pk_ens_alloc(e, capacity);
+    if (r) return r;
+    hipLaunchKernelGGL(k_uniform_off, dim3(nblk(e->nsct + 1, 256)), dim3(256), 0, c->stream, e->nsct, (int64_t)cap,
+                       e->off);
+    hipLaunchKernelGGL(k_fill_i32, dim3(nblk(e->nsct, 256)), dim3(256), 0, c->stream, e->cnt, e->nsct, per_sc);
+    hipLaunchKernelGGL(k_fill_i32, dim3(nblk(e->nsct, 256)), dim3(256), 0, c->stream, e->nstay, e->nsct, per_sc);
+    hipLaunchKernelGGL(k_fill_i32, dim3(nblk(e->nsct, 256)), dim3(256), 0, c->stream, e->nmove, e->nsct, 0);
+    LoadParams lp;
+    lp.ppc = ppc;
+    lp.uth = uth;
+    lp.drift = drift;
+    lp.weight = weight;
+    lp.seed = seed;
+    for (int d = 0; d < 3; d++) lp.step[d] = e->step[d];
+    EnsGeom eg = ens_geom(e);
+    pk_ktimer tm(c, "load");
+    if (e->prec == PK_F64)
+        hipLaunchKernelGGL(k_load_f64, dim3((unsigned)e->nsct), dim3(256), 0, c->stream, eg, lp, e->off, per_sc,
+                           part_f64(e, e->main));
+    else
+        hipLaunchKernelGGL(k_load_split, dim3((unsigned)e->nsct), dim3(256), 0, c->stream, eg, lp, e->off, per_sc,
+                           part_split(e, e->main));
+    PK_HIP(c, hipGetLastError());
+    e->maxcap = (int)cap;
+    e->np = (int64_t)per_sc * e->nsct;
+    e->pending_movers = false;
+    return PK_OK;
+}
+
+static int runstart_of(pk_ens* e, DBuf& runstart)
+{
+    pk_ctx* c = e->ctx;
+    const long nsct = e->nsct;
+    DBuf c64;
+    PK_HIP(c, c64.alloc(sizeof(int64_t) * (nsct + 1)));
+    PK_HIP(c, runstart.alloc(sizeof(int64_t) * (nsct + 1)));
+    DBuf dummy_cap, dummy_max;
+    PK_HIP(c, dummy_cap.alloc(sizeof(int64_t) * (nsct + 1)));
+    PK_HIP(c, dummy_max.alloc(sizeof(int32_t)));
+    hipLaunchKernelGGL(k_caps, dim3(nblk(nsct, 256)), dim3(256), 0, c->stream, nsct, e->cnt, 0.0, 0,
+                       dummy_cap.as<int64_t>(), c64.as<int64_t>(), dummy_max.as<int32_t>());
+    PK_HIP(c, hipMemsetAsync(c64.as<int64_t>() + nsct, 0, sizeof(int64_t), c->stream));
+    return exclusive_scan<int64_t>(c, c64.as<int64_t>(), runstart.as<int64_t>(), nsct + 1);
+}
+
+int pk_ens_download_impl(pk_ens* e, double* const* host, int64_t* off)
+{
+    pk_ctx* c = e->ctx;
+    DBuf runstart, out;
+    int r = runstart_of(e, runstart);
+    if (r) return r;
+    const long n = (long)e->np;
+    PK_HIP(c, out.alloc(sizeof(double) * 7 * (n > 0 ? n : 1)));
+    Fields<double> of;
+    for (int k = 0; k < 8; k++) of.f[k] = k < 7 ? out.as<double>() + (size_t)k * n : nullptr;
+    EnsGeom eg = ens_geom(e);
+    if (e->prec == PK_F64)
+        hipLaunchKernelGGL(k_compact_f64, dim3((unsigned)e->nsct), dim3(256), 0, c->stream, eg, part_f64(e, e->main),
+                           e->off, e->cnt, runstart.as<int64_t>(), of);
+    else
+        hipLaunchKernelGGL(k_compact_split, dim3((unsigned)e->nsct), dim3(256), 0, c->stream, eg,
+                           part_split(e, e->main), e->off, e->cnt, runstart.as<int64_t>(), of);
+    PK_HIP(c, hipGetLastError());
+    for (int k = 0; k < 7; k++)
+        if (host[k]) PK_HIP(c, hipMemcpyAsync(host[k], of.f[k], sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    if (off)
+        PK_HIP(c, hipMemcpyAsync(off, runstart.p, sizeof(int64_t) * (e->nsct + 1), hipMemcpyDeviceToHost, c->stream));
+    PK_HIP(c, hipStreamSynchronize(c->stream));
+    return PK_OK;
+}
+
+int pk_ens_cells_impl(pk_ens* e, int32_t* counts, int32_t* cells)
+{
+    pk_ctx* c = e->ctx;
+    DBuf runstart, dcounts, dcells;
+    int r = runstart_of(e, runstart);
+    if (r) return r;
+    const long ncells = (long)e->ncell[0] * e->ncell[1] * e->ncell[2];
+    const long n = (long)e->np;
+    PK_HIP(c, dcounts.alloc(sizeof(int32_t) * ncells));
+    PK_HIP(c, dcells.alloc(sizeof(int32_t) * (n > 0 ? n : 1)));
+    hipLaunchKernelGGL(k_fill_i32, dim3(nblk(ncells, 256)), dim3(256), 0, c->stream, dcounts.as<int32_t>(), ncells, 0);
+    Fields<double> pf;
+    for (int k = 0; k < 8; k++) pf.f[k] = (e->prec == PK_F64 && k < 7) ? (double*)e->field(e->main, k) : nullptr;
+    const int32_t* scell = e->prec == PK_F64 ? nullptr : (const int32_t*)e->field(e->main, 0);
+    hipLaunchKernelGGL(k_cells, dim3((unsigned)e->nsct), dim3(256), 0, c->stream, ens_geom(e), stage_geom(e),
+                       e->prec == PK_F64 ? 0 : 1, pf, scell, e->off, e->cnt, runstart.as<int64_t>(),
+                       dcounts.as<int32_t>(), dcells.as<int32_t>());
+    PK_HIP(c, hipGetLastError());
+    if (counts)
+        PK_HIP(c, hipMemcpyAsync(counts, dcounts.p, sizeof(int32_t) * ncells, hipMemcpyDeviceToHost, c->stream));
+    if (cells) PK_HIP(c, hipMemcpyAsync(cells, dcells.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->stream));
+    PK_HIP(c, hipStreamSynchronize(c->stream));
+    return PK_OK;
+}

@@ -1,0 +1,223 @@
+// pk_field.hip -- Yee FDTD and periodic ghost kernels (gfx950).
+//
+// Yee: YeeSolver::Implementation<Three>::updateE / updateB (YeeSolver.h:132-213).  pica's main
+// loop plus its three edge loops update every component wherever its curl stencil is inside
+// the array, with one formula; each thread here does exactly that for one cell, in pica's
+// operation order (this file is compiled with -ffp-contract=off, so no FMA is formed and the
+// FP64 result is bit-identical to pica's).  The kernels are streaming stencils: consecutive
+// threads walk the z-fastest storage, the +1/-1 neighbours of a row come from L1/L2.
+#include "pk_internal.h"
+
+namespace {
+
+template <class T>
+struct YeeCoef {
+    T cc, kx, ky, kz;
+};
+
+template <class T>
+__global__ void __launch_bounds__(256) k_yee_e(GridView<T> g, YeeCoef<T> k, long n)
+{
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const int s2 = g.s2;
+    const long s12 = (long)g.s1 * g.s2;
+    const int kk = (int)(c % s2);
+    const int jj = (int)((c / s2) % g.s1);
+    const int ii = (int)(c / s12);
+    T* ex = g.f[PK_EX];
+    T* ey = g.f[PK_EY];
+    T* ez = g.f[PK_EZ];
+    const T* bx = g.f[PK_BX];
+    const T* by = g.f[PK_BY];
+    const T* bz = g.f[PK_BZ];
+    if (jj < g.s1 - 1 && kk < s2 - 1)
+        ex[c] += k.cc * g.f[PK_JX][c] + k.ky * (bz[c + s2] - bz[c]) - k.kz * (by[c + 1] - by[c]);
+    if (ii < g.s0 - 1 && kk < s2 - 1)
+        ey[c] += k.cc * g.f[PK_JY][c] + k.kz * (bx[c + 1] - bx[c]) - k.kx * (bz[c + s12] - bz[c]);
+    if (ii < g.s0 - 1 && jj < g.s1 - 1)
+        ez[c] += k.cc * g.f[PK_JZ][c] + k.kx * (by[c + s12] - by[c]) - k.ky * (bx[c + s2] - bx[c]);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_yee_b(GridView<T> g, YeeCoef<T> k, long n)
+{
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const int s2 = g.s2;
+    const long s12 = (long)g.s1 * g.s2;
+    const int kk = (int)(c % s2);
+    const int jj = (int)((c / s2) % g.s1);
+    const int ii = (int)(c / s12);
+    const T* ex = g.f[PK_EX];
+    const T* ey = g.f[PK_EY];
+    const T* ez = g.f[PK_EZ];
+    if (jj >= 1 && kk >= 1)
+        g.f[PK_BX][c] += k.kz * (ey[c] - ey[c - 1]) - k.ky * (ez[c] - ez[c - s2]);
+    if (ii >= 1 && kk >= 1)
+        g.f[PK_BY][c] += k.kx * (ez[c] - ez[c - s12]) - k.kz * (ex[c] - ex[c - 1]);
+    if (ii >= 1 && jj >= 1)
+        g.f[PK_BZ][c] += k.ky * (ex[c] - ex[c - s2]) - k.kx * (ey[c] - ey[c - s12]);
+}
+
+// Periodic fill: every ghost cell takes the value of its wrapped physical image.  The ghost
+// cells are enumerated as x slabs (full y,z), then y slabs (physical x, full z), then z slabs
+// (physical x,y); the closed-form image equals the dimension-by-dimension copy.
+template <class T>
+__global__ void __launch_bounds__(256) k_ghost_fill(GridView<T> g, unsigned mask, long nx_, long ny_, long nz_)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int G = g.G;
+    const int N0 = g.s0 - 2 * G, N1 = g.s1 - 2 * G, N2 = g.s2 - 2 * G;
+    int i, j, k;
+    if (t < nx_) {
+        const int a = (int)(t / ((long)g.s1 * g.s2));
+        const long r = t % ((long)g.s1 * g.s2);
+        i = a < G ? a : N0 + a;
+        j = (int)(r / g.s2);
+        k = (int)(r % g.s2);
+    } else if ((t -= nx_) < ny_) {
+        const int b = (int)((t / g.s2) % (2 * G));
+        i = G + (int)(t / ((long)2 * G * g.s2));
+        j = b < G ? b : N1 + b;
+        k = (int)(t % g.s2);
+    } else if ((t -= ny_) < nz_) {
+        const int cc = (int)(t % (2 * G));
+        const long r = t / (2 * G);
+        j = G + (int)(r % N1);
+        i = G + (int)(r / N1);
+        k = cc < G ? cc : N2 + cc;
+    } else {
+        return;
+    }
+    const int si = i < G ? i + N0 : (i >= G + N0 ? i - N0 : i);
+    const int sj = j < G ? j + N1 : (j >= G + N1 ? j - N1 : j);
+    const int sk = k < G ? k + N2 : (k >= G + N2 ? k - N2 : k);
+    const long dst = ((long)i * g.s1 + j) * g.s2 + k;
+    const long src = ((long)si * g.s1 + sj) * g.s2 + sk;
+    for (int c = 0; c < PK_NCOMP; c++)
+        if (mask & (1u << c)) g.f[c][dst] = g.f[c][src];
+}
+
+// Periodic fold along dimension d: physical += ghost image, low ghosts first then high ones,
+// over the full extent of the other two dimensions (the oracle's order exactly).
+template <class T>
+__global__ void __launch_bounds__(256) k_ghost_fold(GridView<T> g, unsigned mask, int d)
+{
+    const int s[3] = {g.s0, g.s1, g.s2};
+    const int o1 = (d + 1) % 3, o2 = (d + 2) % 3;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)s[o1] * s[o2]) return;
+    const int p = (int)(t / s[o2]), q = (int)(t % s[o2]);
+    const int G = g.G;
+    const int N = s[d] - 2 * G;
+    long stride[3];
+    stride[2] = 1;
+    stride[1] = g.s2;
+    stride[0] = (long)g.s1 * g.s2;
+    const long base = p * stride[o1] + q * stride[o2];
+    for (int c = 0; c < PK_NCOMP; c++) {
+        if (!(mask & (1u << c))) continue;
+        T* f = g.f[c];
+        for (int a = 0; a < G; a++) f[base + (a + N) * stride[d]] += f[base + a * stride[d]];
+        for (int a = 0; a < G; a++) f[base + (G + a) * stride[d]] += f[base + (G + N + a) * stride[d]];
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_fill_zero(T* p, long n)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = T(0);
+}
+
+inline unsigned nblk(long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+template <class T>
+int yee_e(pk_grid* g, double dt_in)
+{
+    const T dt = (T)dt_in;
+    YeeCoef<T> k;
+    k.cc = -(T)4 * (T)PK_PI * dt;                 // YeeSolver.h:135
+    const T cdt = (T)PK_C * dt;                   // :137
+    k.kx = cdt / (T)g->step[0];                   // :138
+    k.ky = cdt / (T)g->step[1];
+    k.kz = cdt / (T)g->step[2];
+    const long n = (long)g->n;
+    pk_ktimer tm(g->ctx, "yee_e");
+    hipLaunchKernelGGL(k_yee_e<T>, dim3(nblk(n, 256)), dim3(256), 0, g->ctx->stream, grid_view<T>(g), k, n);
+    PK_HIP(g->ctx, hipGetLastError());
+    return PK_OK;
+}
+
+template <class T>
+int yee_b(pk_grid* g, double dt_in)
+{
+    const T dt = (T)dt_in;
+    YeeCoef<T> k;
+    k.cc = 0;
+    const T cdt = (T)PK_C * dt;                   // YeeSolver.h:181-182
+    k.kx = cdt / (T)g->step[0];
+    k.ky = cdt / (T)g->step[1];
+    k.kz = cdt / (T)g->step[2];
+    const long n = (long)g->n;
+    pk_ktimer tm(g->ctx, "yee_b");
+    hipLaunchKernelGGL(k_yee_b<T>, dim3(nblk(n, 256)), dim3(256), 0, g->ctx->stream, grid_view<T>(g), k, n);
+    PK_HIP(g->ctx, hipGetLastError());
+    return PK_OK;
+}
+
+template <class T>
+int ghost(pk_grid* g, int ops)
+{
+    const int G = g->ghost;
+    if (G <= 0) return pk_fail(g->ctx, PK_E_ARG, "pk_ghost: grid has no ghost layers");
+    for (int d = 0; d < 3; d++)
+        if (g->s[d] - 2 * G < G)
+            return pk_fail(g->ctx, PK_E_ARG, "pk_ghost: physical extent smaller than the ghost width");
+    unsigned fold = 0, fill = 0;
+    if (ops & PK_FOLD_J) fold |= (1u << PK_JX) | (1u << PK_JY) | (1u << PK_JZ);
+    if (ops & PK_FOLD_RHO) fold |= (1u << PK_RHO);
+    if (ops & PK_FILL_E) fill |= (1u << PK_EX) | (1u << PK_EY) | (1u << PK_EZ);
+    if (ops & PK_FILL_B) fill |= (1u << PK_BX) | (1u << PK_BY) | (1u << PK_BZ);
+    if (ops & PK_FILL_J) fill |= (1u << PK_JX) | (1u << PK_JY) | (1u << PK_JZ);
+    if (ops & PK_FOLD_RHO) fill |= (1u << PK_RHO);
+    GridView<T> v = grid_view<T>(g);
+    pk_ktimer tm(g->ctx, "ghost");
+    if (fold) {
+        for (int d = 0; d < 3; d++) {
+            const int o1 = (d + 1) % 3, o2 = (d + 2) % 3;
+            const long n = (long)g->s[o1] * g->s[o2];
+            hipLaunchKernelGGL(k_ghost_fold<T>, dim3(nblk(n, 256)), dim3(256), 0, g->ctx->stream, v, fold, d);
+        }
+    }
+    if (fill) {
+        const long nx = (long)2 * G * g->s[1] * g->s[2];
+        const long ny = (long)(g->s[0] - 2 * G) * 2 * G * g->s[2];
+        const long nz = (long)(g->s[0] - 2 * G) * (g->s[1] - 2 * G) * 2 * G;
+        hipLaunchKernelGGL(k_ghost_fill<T>, dim3(nblk(nx + ny + nz, 256)), dim3(256), 0, g->ctx->stream, v, fill,
+                           nx, ny, nz);
+    }
+    PK_HIP(g->ctx, hipGetLastError());
+    return PK_OK;
+}
+
+}  // namespace
+
+int pk_field_zero(pk_grid* g, int comp)
+{
+    const long n = (long)g->n;
+    pk_ktimer tm(g->ctx, "zero");
+    if (g->prec == PK_F64)
+        hipLaunchKernelGGL(k_fill_zero<double>, dim3(nblk(n, 256)), dim3(256), 0, g->ctx->stream,
+                           (double*)g->comp(comp), n);
+    else
+        hipLaunchKernelGGL(k_fill_zero<float>, dim3(nblk(n, 256)), dim3(256), 0, g->ctx->stream,
+                           (float*)g->comp(comp), n);
+    PK_HIP(g->ctx, hipGetLastError());
+    return PK_OK;
+}
+
+int pk_field_yee_e(pk_grid* g, double dt) { return g->prec == PK_F64 ? yee_e<double>(g, dt) : yee_e<float>(g, dt); }
+int pk_field_yee_b(pk_grid* g, double dt) { return g->prec == PK_F64 ? yee_b<double>(g, dt) : yee_b<float>(g, dt); }
+int pk_field_ghost(pk_grid* g, int ops) { return g->prec == PK_F64 ? ghost<double>(g, ops) : ghost<float>(g, ops); }

@@ -1,0 +1,188 @@
+// pk_internal.h -- internal structures of libpica_amd (not part of the ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/pica_amd.h"
+
+// pica's constants (Constants.h:10-13), pi deliberately the truncated literal pica uses.
+#define PK_PI 3.14159265358
+#define PK_C 29979245800.0
+
+#define PK_BLOCK 256           // threads per particle workgroup (4 wavefronts of 64)
+#define PK_XCD 8               // MI355X: 8 XCDs, workgroups are dealt round-robin over them
+#define PK_MAX_SC_PARTICLES 65536
+
+struct pk_timer_rec {
+    double ms = 0.0;
+    long launches = 0;
+};
+
+struct pk_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::vector<double> mass, charge;   // ParticleTypes table
+    bool timing = false;
+    struct pending {
+        std::string name;
+        hipEvent_t a, b;
+    };
+    std::vector<pending> pend;
+    std::vector<hipEvent_t> evpool;
+    std::map<std::string, pk_timer_rec> timers;
+    int32_t* flags_host = nullptr;     // pinned, PK_NFLAGS
+};
+
+enum { PK_FLAG_ERR = 0, PK_FLAG_OVERFLOW = 1, PK_NFLAGS = 4 };
+
+struct pk_grid {
+    pk_ctx* ctx = nullptr;
+    int prec = PK_F64;
+    int s[3] = {0, 0, 0};
+    double origin[3] = {0, 0, 0};
+    double step[3] = {1, 1, 1};
+    int ghost = 0;
+    size_t n = 0;             // elements per component
+    size_t pitch = 0;         // elements between components (aligned)
+    void* data = nullptr;     // PK_NCOMP components
+    double* scratch = nullptr;  // FP64 reduction scratch (partials + results)
+    size_t scratch_n = 0;
+    double* gauss_ref = nullptr;  // reference residual for PK_DIAG_GAUSS_DRIFT
+    size_t esize() const { return prec == PK_F64 ? 8 : 4; }
+    void* comp(int c) const { return (char*)data + (size_t)c * pitch * esize(); }
+};
+
+struct pk_ens {
+    pk_ctx* ctx = nullptr;
+    int prec = PK_F64;
+    int posmode = PK_POS_ABS;
+    int type = 0;
+    int ncell[3] = {0, 0, 0}, ncps[3] = {0, 0, 0}, nsc[3] = {0, 0, 0};
+    double step[3] = {1, 1, 1};
+    long nsct = 0;
+    double slack_frac = 0.25;
+    int slack_add = 32;
+    // particle buffers (SoA, nfield arrays of `capacity` elements of esize bytes)
+    size_t capacity = 0;
+    void* main = nullptr;
+    void* outbox = nullptr;
+    uint8_t* odir = nullptr;     // direction class of each outbox entry
+    int64_t* off = nullptr;      // nsct + 1 (capacity prefix)
+    int32_t* cnt = nullptr;      // particles per supercell
+    int32_t* nstay = nullptr;    // after the particle step: stayers (compacted)
+    int32_t* nmove = nullptr;    // after the particle step: movers in the outbox
+    int32_t* mvc = nullptr;      // nsct * 27 movers per direction class
+    int32_t* flags = nullptr;    // PK_NFLAGS device flags
+    int maxcap = 0;
+    int64_t np = 0;
+    bool pending_movers = false;  // particle step done, re-sort not yet done
+    int nfield() const { return prec == PK_F64 ? 7 : 8; }
+    size_t esize() const { return prec == PK_F64 ? 8 : 4; }
+    void* field(void* buf, int f) const { return (char*)buf + (size_t)f * capacity * esize(); }
+};
+
+// ---- error helpers ----
+int pk_fail(pk_ctx* c, int code, const std::string& msg);
+#define PK_HIP(ctx, call)                                                                      \
+    do {                                                                                       \
+        hipError_t _e = (call);                                                                \
+        if (_e != hipSuccess)                                                                  \
+            return pk_fail((ctx), PK_E_HIP, std::string(#call) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+// ---- per-kernel event timing (RAII around a launch) ----
+struct pk_ktimer {
+    pk_ctx* c;
+    const char* name;
+    hipEvent_t a = nullptr, b = nullptr;
+    pk_ktimer(pk_ctx* ctx, const char* nm);
+    ~pk_ktimer();
+};
+
+// ---- device views passed to kernels by value ----
+template <class T>
+struct GridView {
+    T* f[PK_NCOMP];
+    int s0, s1, s2;
+    int G;
+    double step[3];
+    double origin[3];
+};
+
+template <class T>
+GridView<T> grid_view(const pk_grid* g)
+{
+    GridView<T> v;
+    for (int c = 0; c < PK_NCOMP; c++) v.f[c] = (T*)g->comp(c);
+    v.s0 = g->s[0]; v.s1 = g->s[1]; v.s2 = g->s[2];
+    v.G = g->ghost;
+    for (int d = 0; d < 3; d++) { v.step[d] = g->step[d]; v.origin[d] = g->origin[d]; }
+    return v;
+}
+
+// particle arrays of one buffer (main, outbox or staging)
+struct PartF64 {
+    double *x, *y, *z, *ux, *uy, *uz, *w;
+};
+struct PartSplit {
+    int32_t* cell;
+    float *ox, *oy, *oz, *ux, *uy, *uz, *w;
+};
+
+inline PartF64 part_f64(const pk_ens* e, void* buf)
+{
+    PartF64 p;
+    p.x = (double*)e->field(buf, 0); p.y = (double*)e->field(buf, 1); p.z = (double*)e->field(buf, 2);
+    p.ux = (double*)e->field(buf, 3); p.uy = (double*)e->field(buf, 4); p.uz = (double*)e->field(buf, 5);
+    p.w = (double*)e->field(buf, 6);
+    return p;
+}
+inline PartSplit part_split(const pk_ens* e, void* buf)
+{
+    PartSplit p;
+    p.cell = (int32_t*)e->field(buf, 0);
+    p.ox = (float*)e->field(buf, 1); p.oy = (float*)e->field(buf, 2); p.oz = (float*)e->field(buf, 3);
+    p.ux = (float*)e->field(buf, 4); p.uy = (float*)e->field(buf, 5); p.uz = (float*)e->field(buf, 6);
+    p.w = (float*)e->field(buf, 7);
+    return p;
+}
+
+// supercell geometry shared by the ensemble kernels
+struct EnsGeom {
+    int nsc[3], ncps[3], ncell[3];
+    long nsct;
+    double step[3];
+    int xcd_chunk;   // ceil(nsct / PK_XCD)
+};
+EnsGeom ens_geom(const pk_ens* e);
+
+// XCD-aware supercell of a workgroup: the workgroups one XCD receives (b % 8 == x) get one
+// contiguous slab of supercells, so tile halos and J flushes stay in that XCD's L2.
+__device__ __forceinline__ long pk_xcd_supercell(long b, int chunk)
+{
+    return (b % PK_XCD) * (long)chunk + b / PK_XCD;
+}
+
+// ---- launchers implemented in the .hip files ----
+int pk_field_zero(pk_grid* g, int comp);
+int pk_field_yee_e(pk_grid* g, double dt);
+int pk_field_yee_b(pk_grid* g, double dt);
+int pk_field_ghost(pk_grid* g, int ops);
+int pk_particles_launch(pk_ens* e, pk_grid* g, double dt, int shape, int dep);
+int pk_ens_insert_launch(pk_ens* e);
+int pk_ens_rebuild(pk_ens* e, int source);   // source: 0 = main+outbox, 1 = staging (upload)
+int pk_ens_upload_impl(pk_ens* e, size_t n, const double* const* host);
+int pk_ens_download_impl(pk_ens* e, double* const* host, int64_t* off);
+int pk_ens_cells_impl(pk_ens* e, int32_t* counts, int32_t* cells);
+int pk_ens_load_impl(pk_ens* e, int ppc, double uth, double drift, double weight, uint64_t seed);
+int pk_ens_alloc(pk_ens* e, size_t capacity);
+int pk_check_flags(pk_ens* e, bool* overflow);
+int pk_diag_impl(pk_ens* e, pk_grid* g, int shape, double* out, bool set_ref);
+int pk_rho_impl(pk_ens* e, pk_grid* g, int shape);

@@ -1,0 +1,633 @@
+// pk_particles_f64.hip -- fused interpolation + Boris push + current deposition, FP64,
+// absolute positions, one workgroup per supercell (gfx950).
+//
+// Per workgroup (supercell s, XCD-aware numbering):
+//   1. gather the E/B stencil of the supercell into LDS (FieldInterpolatorCICSupercell ctor,
+//      FieldInterpolatorYeeGrid.h:197-223) and zero an LDS J tile
+//      (CurrentDepositorCICSupercell ctor, CurrentDepositorYeeGrid.h:178-199);
+//   2. stream the supercell's SoA particles, 256 at a time, coalesced: interpolate
+//      (:225-242, 257-281), push (BorisPusher.h:32-47), deposit into the LDS tile with LDS
+//      atomics (CurrentDepositorYeeGrid.h:215-262, or Esirkepov);
+//   3. particles that stay write back x, u in place; particles that leave go to the outbox
+//      (with their direction class), and the stayers found past the final count fill the
+//      holes movers left below it (ascending order on both sides);
+//   4. flush the J tile x 1/V with one pass of global FP64 atomics (the destructor, :201-213).
+// This file is compiled with -ffp-contract=off: per particle, the arithmetic is pica's, in
+// pica's order, so positions and momenta agree with pica to the last bit given the same
+// fields.  The only FP difference left is the order of J sums (atomics).
+#include "pk_internal.h"
+
+namespace {
+
+constexpr int WAVES = PK_BLOCK / 64;
+static_assert(WAVES + 27 + 1 <= 32, "LDS bookkeeping layout");
+
+struct ScGeom {
+    double inv[3], no[3], nos[3], nom[3];
+    int ibase[3], its[3], dbase[3], dts[3];
+};
+
+template <int SHAPE, int DEP>
+__device__ __forceinline__ void sc_geom(const GridView<double>& g, const EnsGeom& eg, const int sidx[3], ScGeom& q)
+{
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const double step = g.step[d];
+        const double scMin = (double)(sidx[d] * eg.ncps[d]) * step;
+        q.inv[d] = 1.0 / step;
+        q.no[d] = (scMin - step) / step;                 // FieldInterpolatorYeeGrid.h:201
+        q.nos[d] = q.no[d] + 0.5;                        // :202
+        q.nom[d] = (scMin - 2.0 * step) / step;
+        const int base = (int)(scMin / step) + 1;       // :205
+        q.ibase[d] = base;
+        q.its[d] = eg.ncps[d] + (SHAPE == PK_CIC ? 2 : 3);
+        if (SHAPE == PK_CIC && DEP == PK_DIRECT) {
+            q.dbase[d] = base;                           // CurrentDepositorYeeGrid.h:185-186
+            q.dts[d] = eg.ncps[d] + 3;
+        } else {
+            q.dbase[d] = base - 1;
+            q.dts[d] = eg.ncps[d] + 4;
+        }
+    }
+}
+
+__device__ __forceinline__ int tidx(const int ts[3], int i, int j, int k) { return (i * ts[1] + j) * ts[2] + k; }
+
+// E/J_x collocated in x; E/J_y in y; E/J_z in z; B_x staggered only in x, etc.
+// (FieldInterpolatorYeeGrid.h:155-166)
+__device__ __forceinline__ bool colloc(int comp, int d)
+{
+    // bit d of the mask of component comp
+    const unsigned m[6] = {1u, 2u, 4u, 6u, 5u, 3u};
+    return (m[comp] >> d) & 1u;
+}
+
+// pica's 8-point CIC sum, term order of FieldInterpolatorYeeGrid.h:273-280
+__device__ __forceinline__ double cic8(const double* t, const int ts[3], int i, int j, int k, const double c[3])
+{
+    return t[tidx(ts, i, j, k)] * (1.0 - c[0]) * (1.0 - c[1]) * (1.0 - c[2]) +
+           t[tidx(ts, i, j, k + 1)] * (1.0 - c[0]) * (1.0 - c[1]) * c[2] +
+           t[tidx(ts, i, j + 1, k)] * (1.0 - c[0]) * c[1] * (1.0 - c[2]) +
+           t[tidx(ts, i, j + 1, k + 1)] * (1.0 - c[0]) * c[1] * c[2] +
+           t[tidx(ts, i + 1, j, k)] * c[0] * (1.0 - c[1]) * (1.0 - c[2]) +
+           t[tidx(ts, i + 1, j, k + 1)] * c[0] * (1.0 - c[1]) * c[2] +
+           t[tidx(ts, i + 1, j + 1, k)] * c[0] * c[1] * (1.0 - c[2]) +
+           t[tidx(ts, i + 1, j + 1, k + 1)] * c[0] * c[1] * c[2];
+}
+
+// pica's 8-point CIC scatter, CurrentDepositorYeeGrid.h:254-261, into LDS with atomics
+__device__ __forceinline__ void cic8_add(double* t, const int ts[3], int i, int j, int k, const double c[3], double v)
+{
+    atomicAdd(&t[tidx(ts, i, j, k)], (1.0 - c[0]) * (1.0 - c[1]) * (1.0 - c[2]) * v);
+    atomicAdd(&t[tidx(ts, i, j, k + 1)], (1.0 - c[0]) * (1.0 - c[1]) * c[2] * v);
+    atomicAdd(&t[tidx(ts, i, j + 1, k)], (1.0 - c[0]) * c[1] * (1.0 - c[2]) * v);
+    atomicAdd(&t[tidx(ts, i, j + 1, k + 1)], (1.0 - c[0]) * c[1] * c[2] * v);
+    atomicAdd(&t[tidx(ts, i + 1, j, k)], c[0] * (1.0 - c[1]) * (1.0 - c[2]) * v);
+    atomicAdd(&t[tidx(ts, i + 1, j, k + 1)], c[0] * (1.0 - c[1]) * c[2] * v);
+    atomicAdd(&t[tidx(ts, i + 1, j + 1, k)], c[0] * c[1] * (1.0 - c[2]) * v);
+    atomicAdd(&t[tidx(ts, i + 1, j + 1, k + 1)], c[0] * c[1] * c[2] * v);
+}
+
+__device__ __forceinline__ void tsc_w(double d, double w[3])
+{
+    const double a = 0.5 - d, b = 0.5 + d;
+    w[0] = 0.5 * a * a;
+    w[1] = 0.75 - d * d;
+    w[2] = 0.5 * b * b;
+}
+
+// nearest node i0 of xi; weights on i0-1..i0+1; returns i0-1
+__device__ __forceinline__ int tsc_node(double xi, double w[3])
+{
+    const int i0 = (int)floor(xi + 0.5);
+    tsc_w(xi - (double)i0, w);
+    return i0 - 1;
+}
+
+__device__ __forceinline__ double tsc27(const double* t, const int ts[3], const int b[3], const double w[3][3])
+{
+    double sa = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        double sb = 0.0;
+#pragma unroll
+        for (int bb = 0; bb < 3; bb++) {
+            double sc = 0.0;
+#pragma unroll
+            for (int c = 0; c < 3; c++) sc += w[2][c] * t[tidx(ts, b[0] + a, b[1] + bb, b[2] + c)];
+            sb += w[1][bb] * sc;
+        }
+        sa += w[0][a] * sb;
+    }
+    return sa;
+}
+
+__device__ __forceinline__ void tsc27_add(double* t, const int ts[3], const int b[3], const double w[3][3], double v)
+{
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int bb = 0; bb < 3; bb++)
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+                atomicAdd(&t[tidx(ts, b[0] + a, b[1] + bb, b[2] + c)], w[0][a] * w[1][bb] * w[2][c] * v);
+}
+
+__device__ __forceinline__ bool in_tile(const int b[3], int span, const int ts[3])
+{
+    return b[0] >= 0 && b[0] + span <= ts[0] && b[1] >= 0 && b[1] + span <= ts[1] && b[2] >= 0 &&
+           b[2] + span <= ts[2];
+}
+
+template <int SHAPE>
+__device__ __forceinline__ bool interp(const ScGeom& q, const double* tile, const double x[3], double e[3], double b[3])
+{
+    const int tn = q.its[0] * q.its[1] * q.its[2];
+    if (SHAPE == PK_CIC) {
+        int ic[3], is[3];
+        double cc[3], cs[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {               // getIndexCoeff, :257-267
+            const double np = x[d] * q.inv[d];
+            double internal = np - q.no[d];
+            ic[d] = (int)internal;
+            cc[d] = internal - (double)ic[d];
+            internal = np - q.nos[d];
+            is[d] = (int)internal;
+            cs[d] = internal - (double)is[d];
+        }
+        bool ok = true;
+#pragma unroll
+        for (int comp = 0; comp < 6; comp++) {
+            int bi[3];
+            double c[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                bi[d] = colloc(comp, d) ? ic[d] : is[d];
+                c[d] = colloc(comp, d) ? cc[d] : cs[d];
+            }
+            ok = ok && in_tile(bi, 2, q.its);
+            const double v = ok ? cic8(tile + comp * tn, q.its, bi[0], bi[1], bi[2], c) : 0.0;
+            if (comp < 3) e[comp] = v; else b[comp - 3] = v;
+        }
+        return ok;
+    } else {
+        int nc[3], ns[3];
+        double wc[3][3], ws[3][3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const double np = x[d] * q.inv[d];
+            nc[d] = tsc_node(np - q.no[d], wc[d]);
+            ns[d] = tsc_node(np - q.nos[d], ws[d]);
+        }
+        bool ok = true;
+#pragma unroll
+        for (int comp = 0; comp < 6; comp++) {
+            int bi[3];
+            double w[3][3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const bool col = colloc(comp, d);
+                bi[d] = col ? nc[d] : ns[d];
+#pragma unroll
+                for (int k = 0; k < 3; k++) w[d][k] = col ? wc[d][k] : ws[d][k];
+            }
+            ok = ok && in_tile(bi, 3, q.its);
+            const double v = ok ? tsc27(tile + comp * tn, q.its, bi, w) : 0.0;
+            if (comp < 3) e[comp] = v; else b[comp - 3] = v;
+        }
+        return ok;
+    }
+}
+
+// BorisPusher::push (BorisPusher.h:32-47) with ParticleRef::setP / getVelocity
+// (ParticleArray.h:209,257-263); returns v
+__device__ __forceinline__ void boris(double coeff, double dt, const double e[3], const double b[3], double u[3],
+                                      double x[3], double v[3])
+{
+    double eM[3], um[3], t[3], up[3], s[3], cr[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) eM[d] = e[d] * coeff;
+#pragma unroll
+    for (int d = 0; d < 3; d++) um[d] = u[d] + eM[d];
+    const double um2 = um[0] * um[0] + um[1] * um[1] + um[2] * um[2];
+    const double tk = coeff / sqrt(1.0 + um2);
+#pragma unroll
+    for (int d = 0; d < 3; d++) t[d] = b[d] * tk;
+    cr[0] = um[1] * t[2] - um[2] * t[1];
+    cr[1] = um[2] * t[0] - um[0] * t[2];
+    cr[2] = um[0] * t[1] - um[1] * t[0];
+#pragma unroll
+    for (int d = 0; d < 3; d++) up[d] = um[d] + cr[d];
+    const double t2 = t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
+    const double sk = 2.0 / (1.0 + t2);
+#pragma unroll
+    for (int d = 0; d < 3; d++) s[d] = t[d] * sk;
+    cr[0] = up[1] * s[2] - up[2] * s[1];
+    cr[1] = up[2] * s[0] - up[0] * s[2];
+    cr[2] = up[0] * s[1] - up[1] * s[0];
+#pragma unroll
+    for (int d = 0; d < 3; d++) u[d] = um[d] + cr[d] + eM[d];
+    const double u2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    const double invGamma = 1.0 / sqrt(1.0 + u2);
+#pragma unroll
+    for (int d = 0; d < 3; d++) v[d] = u[d] * PK_C * invGamma;
+#pragma unroll
+    for (int d = 0; d < 3; d++) x[d] = x[d] + v[d] * dt;
+}
+
+template <int SHAPE>
+__device__ __forceinline__ bool deposit_direct(const ScGeom& q, double* jt, const double x[3], const double cur[3])
+{
+    const int tn = q.dts[0] * q.dts[1] * q.dts[2];
+    if (SHAPE == PK_CIC) {
+        int ic[3], is[3];
+        double cc[3], cs[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {               // CurrentDepositorYeeGrid.h:239-249
+            const double np = x[d] * q.inv[d];
+            double internal = np - q.no[d];
+            ic[d] = (int)internal;
+            cc[d] = internal - (double)ic[d];
+            internal = np - q.nos[d];
+            is[d] = (int)internal;
+            cs[d] = internal - (double)is[d];
+        }
+        bool ok = true;
+#pragma unroll
+        for (int comp = 0; comp < 3; comp++) {
+            int bi[3];
+            double c[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                bi[d] = colloc(comp, d) ? ic[d] : is[d];
+                c[d] = colloc(comp, d) ? cc[d] : cs[d];
+            }
+            ok = ok && in_tile(bi, 2, q.dts);
+        }
+        if (!ok) return false;
+#pragma unroll
+        for (int comp = 0; comp < 3; comp++) {
+            int bi[3];
+            double c[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                bi[d] = colloc(comp, d) ? ic[d] : is[d];
+                c[d] = colloc(comp, d) ? cc[d] : cs[d];
+            }
+            cic8_add(jt + comp * tn, q.dts, bi[0], bi[1], bi[2], c, cur[comp]);
+        }
+        return true;
+    } else {
+        int nc[3], ns[3];
+        double wc[3][3], ws[3][3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const double np = x[d] * q.inv[d];
+            nc[d] = tsc_node(np - q.nom[d], wc[d]);
+            ns[d] = tsc_node(np - (q.nom[d] + 0.5), ws[d]);
+        }
+        bool ok = true;
+#pragma unroll
+        for (int comp = 0; comp < 3; comp++) {
+            int bi[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) bi[d] = colloc(comp, d) ? nc[d] : ns[d];
+            ok = ok && in_tile(bi, 3, q.dts);
+        }
+        if (!ok) return false;
+#pragma unroll
+        for (int comp = 0; comp < 3; comp++) {
+            int bi[3];
+            double w[3][3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const bool col = colloc(comp, d);
+                bi[d] = col ? nc[d] : ns[d];
+#pragma unroll
+                for (int k = 0; k < 3; k++) w[d][k] = col ? wc[d][k] : ws[d][k];
+            }
+            tsc27_add(jt + comp * tn, q.dts, bi, w, cur[comp]);
+        }
+        return true;
+    }
+}
+
+// 1D old/new shapes on a common window of K nodes (cell centres, base-m tile); -1 on failure
+template <int SHAPE>
+__device__ __forceinline__ int esk_1d(double xi0, double xi1, double S0[4], double DS[4])
+{
+    double S1[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) { S0[k] = 0.0; S1[k] = 0.0; }
+    int w;
+    if (SHAPE == PK_CIC) {
+        const int i0 = (int)floor(xi0), i1 = (int)floor(xi1);
+        const double f0 = xi0 - (double)i0, f1 = xi1 - (double)i1;
+        w = i0 < i1 ? i0 : i1;
+        if (i0 - w > 1 || i1 - w > 1) return -1;
+        const int a = i0 - w, b = i1 - w;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            S0[k] = (k == a) ? 1.0 - f0 : ((k == a + 1) ? f0 : 0.0);
+            S1[k] = (k == b) ? 1.0 - f1 : ((k == b + 1) ? f1 : 0.0);
+        }
+    } else {
+        double w0[3], w1[3];
+        const int a0 = tsc_node(xi0, w0), a1 = tsc_node(xi1, w1);
+        w = a0 < a1 ? a0 : a1;
+        if (a0 - w > 1 || a1 - w > 1) return -1;
+        const int a = a0 - w, b = a1 - w;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            S0[k] = (k == a) ? w0[0] : (k == a + 1) ? w0[1] : (k == a + 2) ? w0[2] : 0.0;
+            S1[k] = (k == b) ? w1[0] : (k == b + 1) ? w1[1] : (k == b + 2) ? w1[2] : 0.0;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) DS[k] = S1[k] - S0[k];
+    return w;
+}
+
+// Esirkepov (CPC 135 (2001) 144) deposition of a particle moving x0 -> x1 with charge qw
+template <int SHAPE>
+__device__ __forceinline__ bool deposit_esirkepov(const ScGeom& q, double* jt, const double x0[3], const double x1[3],
+                                                  double qw, double dt, const double step[3])
+{
+    constexpr int K = SHAPE == PK_CIC ? 3 : 4;
+    const int tn = q.dts[0] * q.dts[1] * q.dts[2];
+    double S0[3][4], DS[3][4];
+    int w[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const double off = q.nom[d] + 0.5;
+        const double xi0 = x0[d] * q.inv[d] - off;
+        const double xi1 = x1[d] * q.inv[d] - off;
+        w[d] = esk_1d<SHAPE>(xi0, xi1, S0[d], DS[d]);
+        if (w[d] < 0 || w[d] + K > q.dts[d]) return false;
+    }
+    const double third = 1.0 / 3.0;
+#pragma unroll
+    for (int comp = 0; comp < 3; comp++) {
+        const int d1 = (comp + 1) % 3, d2 = (comp + 2) % 3;
+        const double kq = -qw * step[comp] / dt;
+        double* t = jt + comp * tn;
+#pragma unroll
+        for (int p = 0; p < K; p++)
+#pragma unroll
+            for (int r = 0; r < K; r++) {
+                double cum = 0.0;
+#pragma unroll
+                for (int a = 0; a < K - 1; a++) {
+                    const double Wv = DS[comp][a] * (S0[d1][p] * S0[d2][r] + 0.5 * DS[d1][p] * S0[d2][r] +
+                                                     0.5 * S0[d1][p] * DS[d2][r] + third * DS[d1][p] * DS[d2][r]);
+                    cum += Wv;
+                    int ix[3];
+                    ix[comp] = w[comp] + a + 1;
+                    ix[d1] = w[d1] + p;
+                    ix[d2] = w[d2] + r;
+                    const double val = kq * cum;
+                    if (val != 0.0) atomicAdd(&t[tidx(q.dts, ix[0], ix[1], ix[2])], val);
+                }
+            }
+    }
+    return true;
+}
+
+struct StepParams {
+    double dt, coeff, q;
+    double L[3], scinv[3];
+    int mask_words;
+};
+
+template <int SHAPE, int DEP>
+__global__ void __launch_bounds__(PK_BLOCK) k_particles(GridView<double> g, EnsGeom eg, PartF64 pm, PartF64 po,
+                                                        uint8_t* __restrict__ odir, const int64_t* __restrict__ off,
+                                                        const int32_t* __restrict__ cnt, int32_t* __restrict__ nstay,
+                                                        int32_t* __restrict__ nmove, int32_t* __restrict__ mvc,
+                                                        int32_t* __restrict__ flags, StepParams sp)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const long s = pk_xcd_supercell(blockIdx.x, eg.xcd_chunk);
+    if (s >= eg.nsct) return;
+    const int sidx[3] = {(int)(s / ((long)eg.nsc[1] * eg.nsc[2])), (int)((s / eg.nsc[2]) % eg.nsc[1]),
+                         (int)(s % eg.nsc[2])};
+    ScGeom q;
+    sc_geom<SHAPE, DEP>(g, eg, sidx, q);
+    const int itn = q.its[0] * q.its[1] * q.its[2];
+    const int dtn = q.dts[0] * q.dts[1] * q.dts[2];
+    double* itile = (double*)smem;
+    double* jtile = itile + 6 * itn;
+    uint64_t* mask = (uint64_t*)(jtile + 3 * dtn);
+    int* wtot = (int*)(mask + sp.mask_words);
+    int* dircnt = wtot + WAVES;            // 27
+    int* bad = dircnt + 27;                // 1 (WAVES + 27 + 1 = 32 ints keeps wpre 8-byte aligned)
+    long* wpre = (long*)(wtot + 32);       // prefix of mover bits per mask word (mask_words + 1)
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    // 1. gather the field stencil (6 components), zero J tile and bookkeeping
+    for (int t = tid; t < 6 * itn; t += PK_BLOCK) {
+        const int comp = t / itn, r = t % itn;
+        const int i = r / (q.its[1] * q.its[2]), j = (r / q.its[2]) % q.its[1], k = r % q.its[2];
+        itile[t] = g.f[PK_EX + comp][((long)(q.ibase[0] + i) * g.s1 + (q.ibase[1] + j)) * g.s2 + (q.ibase[2] + k)];
+    }
+    for (int t = tid; t < 3 * dtn; t += PK_BLOCK) jtile[t] = 0.0;
+    for (int t = tid; t < 27; t += PK_BLOCK) dircnt[t] = 0;
+    if (tid == 0) bad[0] = 0;
+    __syncthreads();
+
+    const long base = off[s];
+    const int n = cnt[s];
+    int running = 0;   // movers so far
+    for (int c0 = 0; c0 < n; c0 += PK_BLOCK) {
+        const int i = c0 + tid;
+        const bool act = i < n;
+        double x[3], u[3], w = 0.0;
+        int code = 13;
+        bool ok = true;
+        if (act) {
+            const long p = base + i;
+            x[0] = pm.x[p]; x[1] = pm.y[p]; x[2] = pm.z[p];
+            u[0] = pm.ux[p]; u[1] = pm.uy[p]; u[2] = pm.uz[p];
+            w = pm.w[p];
+            const double x0[3] = {x[0], x[1], x[2]};
+            double ef[3], bf[3], v[3];
+            ok = isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2]) && interp<SHAPE>(q, itile, x, ef, bf);
+            if (ok) {
+                boris(sp.coeff, sp.dt, ef, bf, u, x, v);
+                ok = isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2]);
+            }
+            if (ok) {
+                const double qw = sp.q * w;
+                if (DEP == PK_DIRECT) {
+                    double cur[3];
+#pragma unroll
+                    for (int d = 0; d < 3; d++) cur[d] = v[d] * qw;
+                    ok = deposit_direct<SHAPE>(q, jtile, x, cur);
+                } else {
+                    ok = deposit_esirkepov<SHAPE>(q, jtile, x0, x, qw, sp.dt, g.step);
+                }
+            }
+            if (ok) {
+                // periodic wrap and direction class of the destination supercell
+                code = 0;
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    if (x[d] < 0.0) x[d] += sp.L[d];
+                    if (x[d] >= sp.L[d]) x[d] -= sp.L[d];
+                    int dst = (int)floor((x[d] - 0.0) * sp.scinv[d]);
+                    dst = dst < 0 ? 0 : (dst > eg.nsc[d] - 1 ? eg.nsc[d] - 1 : dst);
+                    int dd = dst - sidx[d];
+                    if (dd > 1) dd -= eg.nsc[d];
+                    if (dd < -1) dd += eg.nsc[d];
+                    if (dd < -1 || dd > 1) ok = false;
+                    code = code * 3 + (dd + 1);
+                }
+            }
+            if (!ok) {
+                // leave the particle untouched and report; nothing outside the tiles is written
+                x[0] = x0[0]; x[1] = x0[1]; x[2] = x0[2];
+                u[0] = pm.ux[p]; u[1] = pm.uy[p]; u[2] = pm.uz[p];
+                code = 13;
+                bad[0] = 1;
+            }
+        }
+        const bool mover = act && code != 13;
+        const uint64_t mb = __ballot(mover);
+        if (lane == 0) {
+            wtot[wave] = __popcll(mb);
+            mask[(c0 >> 6) + wave] = mb;
+        }
+        __syncthreads();
+        int pre = running, tot = 0;
+#pragma unroll
+        for (int v = 0; v < WAVES; v++) {
+            pre += (v < wave) ? wtot[v] : 0;
+            tot += wtot[v];
+        }
+        if (act) {
+            const long p = base + i;
+            if (!mover) {
+                pm.x[p] = x[0]; pm.y[p] = x[1]; pm.z[p] = x[2];
+                pm.ux[p] = u[0]; pm.uy[p] = u[1]; pm.uz[p] = u[2];
+            } else {
+                const int rank = pre + __popcll(mb & ((1ull << lane) - 1ull));
+                const long o = base + rank;
+                po.x[o] = x[0]; po.y[o] = x[1]; po.z[o] = x[2];
+                po.ux[o] = u[0]; po.uy[o] = u[1]; po.uz[o] = u[2];
+                po.w[o] = w;
+                odir[o] = (uint8_t)code;
+                atomicAdd(&dircnt[code], 1);
+            }
+        }
+        running += tot;
+        __syncthreads();
+    }
+    const int nmv = running;
+    const int nfinal = n - nmv;
+    // 3. swap-fill: the k-th stayer at a slot >= nfinal fills the k-th hole below nfinal
+    const int words = (n + 63) >> 6;
+    if (nmv > 0) {
+        if (tid == 0) {
+            long acc = 0;
+            for (int wd = 0; wd < words; wd++) {
+                wpre[wd] = acc;
+                acc += __popcll(mask[wd]);
+            }
+            wpre[words] = acc;
+        }
+        __syncthreads();
+        // movers strictly below slot y
+        auto movers_below = [&](int y) -> long {
+            const int wd = y >> 6, bt = y & 63;
+            return wpre[wd] + (bt ? __popcll(mask[wd] & ((1ull << bt) - 1ull)) : 0);
+        };
+        const long below_final = movers_below(nfinal);
+        for (int t = tid; t < nmv; t += PK_BLOCK) {
+            const int j = nfinal + t;
+            if ((mask[j >> 6] >> (j & 63)) & 1ull) continue;    // mover slot, nothing to move
+            const long r = (long)t - (movers_below(j) - below_final);   // rank among tail stayers
+            // select the r-th mover slot (ascending): word by binary search on wpre, then bit
+            int lo = 0, hi = words - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (wpre[mid] <= r) lo = mid; else hi = mid - 1;
+            }
+            uint64_t m = mask[lo];
+            for (long z = r - wpre[lo]; z > 0; z--) m &= m - 1ull;
+            if (m == 0ull) {            // cannot happen for a consistent mask; never write outside
+                bad[0] = 1;
+                continue;
+            }
+            const int h = (lo << 6) + (__ffsll((long long)m) - 1);
+            const long src = base + j, dst = base + h;
+            pm.x[dst] = pm.x[src]; pm.y[dst] = pm.y[src]; pm.z[dst] = pm.z[src];
+            pm.ux[dst] = pm.ux[src]; pm.uy[dst] = pm.uy[src]; pm.uz[dst] = pm.uz[src];
+            pm.w[dst] = pm.w[src];
+        }
+    }
+    __syncthreads();
+    // 4. flush the J tile x 1/V (CurrentDepositorYeeGrid.h:201-213)
+    const double normalization = 1.0 / (g.step[0] * g.step[1] * g.step[2]);
+    for (int t = tid; t < 3 * dtn; t += PK_BLOCK) {
+        const double v = jtile[t];
+        if (v == 0.0) continue;
+        const int comp = t / dtn, r = t % dtn;
+        const int i = r / (q.dts[1] * q.dts[2]), j = (r / q.dts[2]) % q.dts[1], k = r % q.dts[2];
+        atomicAdd(&g.f[PK_JX + comp][((long)(q.dbase[0] + i) * g.s1 + (q.dbase[1] + j)) * g.s2 + (q.dbase[2] + k)],
+                  v * normalization);
+    }
+    if (tid < 27) mvc[s * 27 + tid] = dircnt[tid];
+    if (tid == 0) {
+        nstay[s] = nfinal;
+        nmove[s] = nmv;
+        if (bad[0]) atomicOr(&flags[PK_FLAG_ERR], 1);
+    }
+}
+
+template <int SHAPE, int DEP>
+int launch(pk_ens* e, pk_grid* g, double dt)
+{
+    EnsGeom eg = ens_geom(e);
+    StepParams sp;
+    sp.dt = dt;
+    const double q = e->ctx->charge[e->type], m = e->ctx->mass[e->type];
+    sp.coeff = q * dt / (2.0 * m * PK_C);        // BorisPusher.h:29
+    sp.q = q;
+    for (int d = 0; d < 3; d++) {
+        sp.L[d] = (double)e->ncell[d] * e->step[d];
+        const double cellSize = (sp.L[d] - 0.0) / (double)e->ncell[d];      // EnsembleSupercells.h:31-33
+        sp.scinv[d] = 1.0 / ((double)e->ncps[d] * cellSize);
+    }
+    sp.mask_words = (e->maxcap + PK_BLOCK + 63) / 64 + 1;
+    int its[3], dts[3];
+    for (int d = 0; d < 3; d++) {
+        its[d] = e->ncps[d] + (SHAPE == PK_CIC ? 2 : 3);
+        dts[d] = e->ncps[d] + ((SHAPE == PK_CIC && DEP == PK_DIRECT) ? 3 : 4);
+    }
+    const size_t lds = sizeof(double) * (6 * its[0] * its[1] * its[2] + 3 * dts[0] * dts[1] * dts[2]) +
+                       sizeof(uint64_t) * sp.mask_words + sizeof(int) * 32 +
+                       sizeof(long) * (sp.mask_words + 1) + 16;
+    if (lds > 160 * 1024)
+        return pk_fail(e->ctx, PK_E_CAPACITY, "particle step: supercell too large for LDS");
+    const unsigned nblocks = (unsigned)(PK_XCD * (long)eg.xcd_chunk);
+    PartF64 pm = part_f64(e, e->main), po = part_f64(e, e->outbox);
+    GridView<double> gv = grid_view<double>(g);
+    pk_ktimer tm(e->ctx, "particles");
+    hipLaunchKernelGGL((k_particles<SHAPE, DEP>), dim3(nblocks), dim3(PK_BLOCK), lds, e->ctx->stream, gv, eg, pm, po,
+                       e->odir, e->off, e->cnt, e->nstay, e->nmove, e->mvc, e->flags, sp);
+    PK_HIP(e->ctx, hipGetLastError());
+    return PK_OK;
+}
+
+}  // namespace
+
+int pk_particles_f64(pk_ens* e, pk_grid* g, double dt, int shape, int dep)
+{
+    if (shape == PK_CIC && dep == PK_DIRECT) return launch<PK_CIC, PK_DIRECT>(e, g, dt);
+    if (shape == PK_CIC && dep == PK_ESIRKEPOV) return launch<PK_CIC, PK_ESIRKEPOV>(e, g, dt);
+    if (shape == PK_TSC && dep == PK_DIRECT) return launch<PK_TSC, PK_DIRECT>(e, g, dt);
+    if (shape == PK_TSC && dep == PK_ESIRKEPOV) return launch<PK_TSC, PK_ESIRKEPOV>(e, g, dt);
+    return pk_fail(e->ctx, PK_E_ARG, "particle step: unknown shape/depositor");
+}
