@@ -1,0 +1,89 @@
+"""FP32 split-position form (int32 cell + float offset, FP32 fields/J) against the FP64 oracle.
+
+The two runs start from identical seeded inputs.  Particles are matched through unique
+weights (w_k = w0 (1 + k 2^-20), exact in FP32).  Tolerances are stated per quantity; they are
+FP32 round-off bounds, not parity in the bit sense.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import pica_amd as P
+from helpers import courant_dt, plasma, random_particles, rel_err
+
+pytestmark = pytest.mark.gpu
+
+# per-quantity bounds after 10 steps (measured errors are printed by the test)
+TOL = {"x": 2e-6, "u": 2e-5, "E": 2e-4, "B": 2e-4, "J": 2e-4, "energy": 1e-5}
+
+
+def _say(msg):
+    print("[case] " + msg, flush=True)
+
+
+def _inputs(seed, ncell, ppc, uth=0.05, drift=0.1):
+    rng = np.random.default_rng(seed)
+    dt = courant_dt()
+    n0, w0 = plasma(dt, ppc)
+    x, u = random_particles(rng, ncell, ppc, uth=uth, drift=drift)
+    n = x.shape[0]
+    w = (w0 * (1.0 + np.arange(n) * 2.0 ** -20)).astype(np.float32).astype(np.float64)
+    # the FP32 form stores u and the in-cell offset in FP32: give the FP64 run the same values
+    u = u.astype(np.float32).astype(np.float64)
+    c = np.floor(x)
+    x = c + (x - c).astype(np.float32).astype(np.float64)
+    return dt, x, u, w
+
+
+@pytest.mark.parametrize("shape,dep", [(P.CIC, P.DIRECT), (P.CIC, P.ESIRKEPOV), (P.TSC, P.DIRECT),
+                                       (P.TSC, P.ESIRKEPOV)])
+def test_split_matches_fp64_oracle(ctx, shape, dep):
+    _say("split vs fp64 shape=%d dep=%d" % (shape, dep))
+    ncell, ncps, ppc, nsteps = (16, 16, 16), (4, 4, 4), 8, 10
+    dt, x, u, w = _inputs(41 + 2 * shape + dep, ncell, ppc)
+    og = O.Grid.periodic(ncell)
+    oe = O.Ensemble(ncell, ncps, (1.0, 1.0, 1.0), x, u, w)
+    for _ in range(nsteps):
+        O.step(og, oe, dt, shape, dep)
+    e0 = sum(O.energy(og, oe))
+    g = P.YeeGrid.periodic(ctx, ncell, prec=P.F32)
+    ens = P.EnsembleSupercells(ctx, ncell, (1.0, 1.0, 1.0), ncps, prec=P.F32)
+    ens.add(x, u, w)
+    P.run(ens, g, dt, nsteps, shape, dep)
+    gx, gu, gw, goff = ens.particles()
+    assert gx.shape[0] == oe.np
+    d = P.diag(ens, g, shape)
+    e1 = d[P.DIAG_FIELD_ENERGY] + d[P.DIAG_KINETIC_ENERGY]
+    ia, ib = np.argsort(gw, kind="stable"), np.argsort(oe.w, kind="stable")
+    assert np.array_equal(gw[ia], oe.w[ib]), "particle sets differ"
+    L = np.array(ncell, np.float64)
+    dx = gx[ia] - oe.positions()[ib]
+    dx -= L * np.round(dx / L)                          # periodic images
+    errs = {
+        "x": float(np.abs(dx).max() / L.max()),
+        "u": rel_err(gu[ia], oe.momenta()[ib]),
+        "E": max(rel_err(g.get(c), og.f[c]) for c in (P.EX, P.EY, P.EZ)),
+        "B": max(rel_err(g.get(c), og.f[c]) for c in (P.BX, P.BY, P.BZ)),
+        "J": max(rel_err(g.get(c), og.f[c]) for c in (P.JX, P.JY, P.JZ)),
+        "energy": abs(e1 - e0) / abs(e0),
+    }
+    print("split errors vs FP64 oracle (shape=%d dep=%d): %s" % (shape, dep, errs), flush=True)
+    for k, v in errs.items():
+        assert v < TOL[k], "%s error %.3g above %.3g" % (k, v, TOL[k])
+
+
+def test_split_cells_and_counts(ctx):
+    """the cell of every particle and the per-cell counts match the FP64 oracle's positions"""
+    _say("split cells")
+    ncell, ncps, ppc = (8, 12, 8), (4, 4, 4), 6
+    dt, x, u, w = _inputs(77, ncell, ppc)
+    ens = P.EnsembleSupercells(ctx, ncell, (1.0, 1.0, 1.0), ncps, prec=P.F32)
+    ens.add(x, u, w)
+    counts, cells = ens.cells()
+    gx, gu, gw, goff = ens.particles()
+    oe = O.Ensemble(ncell, ncps, (1.0, 1.0, 1.0), x, u, w)
+    assert np.array_equal(goff, oe.off)
+    ci = np.floor(oe.positions()).astype(np.int64)
+    lin = (ci[:, 0] * ncell[1] + ci[:, 1]) * ncell[2] + ci[:, 2]
+    assert np.array_equal(cells, lin)
+    assert np.array_equal(counts.ravel(), np.bincount(lin, minlength=int(np.prod(ncell))))
