@@ -269,7 +269,11 @@ __device__ __forceinline__ void thermal_particle(const EnsGeom& eg, const LoadPa
 {
     const int sidx[3] = {(int)(s / ((long)eg.nsc[1] * eg.nsc[2])), (int)((s / eg.nsc[2]) % eg.nsc[1]),
                          (int)(s % eg.nsc[2])};
-    const int lc = l / lp.ppc, k = l % lp.ppc;
+    // slots interleave the cells of the supercell (slot = k * cells + cell): consecutive
+    // particles, hence the lanes of a wavefront, sit in different cells and their LDS
+    // deposits do not collide on one address
+    const int ncsc = eg.ncps[0] * eg.ncps[1] * eg.ncps[2];
+    const int lc = l % ncsc, k = l / ncsc;
     const int a = lc / (eg.ncps[1] * eg.ncps[2]), b = (lc / eg.ncps[2]) % eg.ncps[1], c = lc % eg.ncps[2];
     gc[0] = sidx[0] * eg.ncps[0] + a;
     gc[1] = sidx[1] * eg.ncps[1] + b;

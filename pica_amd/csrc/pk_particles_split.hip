@@ -3,10 +3,16 @@
 // A particle is an int32 linear physical cell index, a float offset in [0, 1) per dimension,
 // float u = p/(mc) and a float weight (32 bytes).  Every stencil index is exact small-integer
 // arithmetic relative to the particle's cell; only the in-cell offset carries rounding, so
-// the position resolution is 2^-24 of a cell everywhere in the domain.  Fields, J and the LDS
-// tiles are FP32.  The kernel is the FP64 kernel's pipeline (pk_particles_f64.hip): stencil
-// in LDS -> interpolate -> Boris push (BorisPusher.h:32-47) -> deposit into an LDS J tile ->
-// movers to the outbox, tail stayers fill holes -> one flush of global atomics.
+// the position resolution is 2^-24 of a cell everywhere in the domain.  Fields and J in HBM
+// are FP32.  The kernel is the FP64 kernel's pipeline (pk_particles_f64.hip): stencil in LDS
+// -> interpolate -> Boris push (BorisPusher.h:32-47) -> deposit into an LDS J tile -> movers
+// to the outbox, tail stayers fill holes -> one flush of global FP32 atomics.
+//
+// The LDS J tile is accumulated in FP64: on gfx950 ds_add_f64 sustains ~12x the rate of
+// ds_add_f32 (measured 2.4e12 vs 2.0e11 LDS atomics/s on MI355X), and the extra precision
+// is free.  Esirkepov deposition uses a per-wavefront window: 4 (TSC) or 3 (CIC) nodes in a
+// dimension only when some lane of the wavefront changes its nearest node there, else 3 / 2,
+// so a typical wavefront issues about half of the worst-case atomics.
 //
 // Stencil conventions (grid origin -2h, two ghost layers; l = cell - supercell start):
 //   interpolation tile base = array index m+1 (pica's baseIdx), CIC n+2 / TSC n+3 wide;
@@ -17,8 +23,6 @@
 namespace {
 
 constexpr int WAVES = PK_BLOCK / 64;
-
-__device__ __forceinline__ int tix(const int ts[3], int i, int j, int k) { return (i * ts[1] + j) * ts[2] + k; }
 
 __device__ __forceinline__ bool colloc(int comp, int d)
 {
@@ -56,16 +60,17 @@ __device__ __forceinline__ void interp_stencil(int l, float o, int& bc, float wc
 }
 
 template <int SHAPE>
-__device__ __forceinline__ float gather(const float* t, const int ts[3], const int b[3], const float w[3][3])
+__device__ __forceinline__ float gather(const float* t, int sx, int sy, const int b[3], const float w[3][3])
 {
     constexpr int K = SHAPE == PK_TSC ? 3 : 2;
+    const float* base = t + b[0] * sx + b[1] * sy + b[2];
     float sa = 0.0f;
 #pragma unroll
     for (int a = 0; a < K; a++) {
         float sb = 0.0f;
 #pragma unroll
         for (int bb = 0; bb < K; bb++) {
-            const float* row = t + tix(ts, b[0] + a, b[1] + bb, b[2]);
+            const float* row = base + a * sx + bb * sy;
             float sc = 0.0f;
 #pragma unroll
             for (int c = 0; c < K; c++) sc += w[2][c] * row[c];
@@ -76,25 +81,60 @@ __device__ __forceinline__ float gather(const float* t, const int ts[3], const i
     return sa;
 }
 
-// shape of the cell-centre (rho) grid on a K-node window [w, w+K) for a particle in tile
-// coordinates: node index of its cell centre nc = l + 2 (deposition tile base m), offset o
+// shape on the cell-centre (rho) grid, 4-entry window starting at node wb, of a particle whose
+// own cell centre is node nc (deposition tile coordinates) and offset o
 template <int SHAPE>
-__device__ __forceinline__ void rho_shape(int nc, float o, int w, float S[4])
+__device__ __forceinline__ void rho_shape(int nc, float o, int wb, float S[4])
 {
-    S[0] = S[1] = S[2] = S[3] = 0.0f;
     if (SHAPE == PK_TSC) {
         float ww[3];
         tsc_w(o - 0.5f, ww);          // nodes nc-1, nc, nc+1
-        const int a = nc - 1 - w;
+        const int a = nc - 1 - wb;
 #pragma unroll
         for (int k = 0; k < 4; k++) S[k] = (k == a) ? ww[0] : (k == a + 1) ? ww[1] : (k == a + 2) ? ww[2] : 0.0f;
     } else {
         const float s = o - 0.5f;     // lower node nc-1 (s < 0) or nc
         const int lo = s < 0.0f ? nc - 1 : nc;
         const float f = s < 0.0f ? s + 1.0f : s;
-        const int a = lo - w;
+        const int a = lo - wb;
 #pragma unroll
         for (int k = 0; k < 4; k++) S[k] = (k == a) ? 1.0f - f : (k == a + 1) ? f : 0.0f;
+    }
+}
+
+// Esirkepov W decomposition (CPC 135 (2001) 144) on a KX x KY x KZ window, accumulated in
+// FP64 LDS.  W_c(a,p,r) = DS_c[a] (S0_1 S0_2 + DS_1 S0_2/2 + S0_1 DS_2/2 + DS_1 DS_2/3), written
+// as DS_c[a] (A[p] S0_2[r] + B[p] DS_2[r]); J_c on face a+1 is the running sum over a.
+template <int KX, int KY, int KZ>
+__device__ __forceinline__ void esirkepov(double* jt, int dtn, int sx, int sy, const int wb[3], const float S0[3][4],
+                                          const float DS[3][4], const float kq[3])
+{
+    constexpr int K[3] = {KX, KY, KZ};
+    const int st[3] = {sx, sy, 1};
+    const int orig = wb[0] * sx + wb[1] * sy + wb[2];
+    const float third = 1.0f / 3.0f;
+#pragma unroll
+    for (int comp = 0; comp < 3; comp++) {
+        const int d1 = (comp + 1) % 3, d2 = (comp + 2) % 3;
+        float A[4], B[4];
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            A[p] = kq[comp] * (S0[d1][p] + 0.5f * DS[d1][p]);
+            B[p] = kq[comp] * (0.5f * S0[d1][p] + third * DS[d1][p]);
+        }
+        double* t = jt + comp * dtn + orig + st[comp];
+#pragma unroll
+        for (int p = 0; p < K[d1]; p++)
+#pragma unroll
+            for (int r = 0; r < K[d2]; r++) {
+                const float coef = A[p] * S0[d2][r] + B[p] * DS[d2][r];
+                float cum = 0.0f;
+#pragma unroll
+                for (int a = 0; a < K[comp] - 1; a++) {
+                    cum += DS[comp][a] * coef;
+                    atomicAdd(&t[a * st[comp] + p * st[d1] + r * st[d2]], (double)cum);
+                }
+            }
     }
 }
 
@@ -107,7 +147,8 @@ struct SplitParams {
     int mask_words;
 };
 
-template <int SHAPE, int DEP>
+// NC > 0: cubic supercells of NC cells (compile-time tile strides); NC == 0: any shape
+template <int SHAPE, int DEP, int NC>
 __global__ void __launch_bounds__(PK_BLOCK) k_particles_split(GridView<float> g, EnsGeom eg, PartSplit pm, PartSplit po,
                                                               uint8_t* __restrict__ odir,
                                                               const int64_t* __restrict__ off,
@@ -119,23 +160,28 @@ __global__ void __launch_bounds__(PK_BLOCK) k_particles_split(GridView<float> g,
     extern __shared__ __align__(16) unsigned char smem[];
     const long s = pk_xcd_supercell(blockIdx.x, eg.xcd_chunk);
     if (s >= eg.nsct) return;
+    constexpr bool PICA_CIC = SHAPE == PK_CIC && DEP == PK_DIRECT;
+    const int nn[3] = {NC ? NC : eg.ncps[0], NC ? NC : eg.ncps[1], NC ? NC : eg.ncps[2]};
     const int sidx[3] = {(int)(s / ((long)eg.nsc[1] * eg.nsc[2])), (int)((s / eg.nsc[2]) % eg.nsc[1]),
                          (int)(s % eg.nsc[2])};
     int its[3], dts[3], ibase[3], dbase[3], m0[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        m0[d] = sidx[d] * eg.ncps[d];
-        its[d] = eg.ncps[d] + (SHAPE == PK_TSC ? 3 : 2);
+        m0[d] = sidx[d] * nn[d];
+        its[d] = nn[d] + (SHAPE == PK_TSC ? 3 : 2);
         ibase[d] = m0[d] + 1;
-        const bool pica_cic = SHAPE == PK_CIC && DEP == PK_DIRECT;
-        dts[d] = eg.ncps[d] + (pica_cic ? 3 : 4);
-        dbase[d] = pica_cic ? m0[d] + 1 : m0[d];
+        dts[d] = nn[d] + (PICA_CIC ? 3 : 4);
+        dbase[d] = PICA_CIC ? m0[d] + 1 : m0[d];
     }
-    const int itn = its[0] * its[1] * its[2];
-    const int dtn = dts[0] * dts[1] * dts[2];
+    const int isx = its[1] * its[2], isy = its[2];
+    const int dsx = dts[1] * dts[2], dsy = dts[2];
+    const int itn = its[0] * isx;
+    const int dtn = dts[0] * dsx;
+    // every LDS pointer is smem + a byte offset, so that the compiler keeps DS instructions
     float* itile = (float*)smem;
-    float* jtile = itile + 6 * itn;
-    uint64_t* mask = (uint64_t*)(((uintptr_t)(jtile + 3 * dtn) + 7) & ~(uintptr_t)7);
+    const int joff = (int)((sizeof(float) * 6 * itn + 7) & ~(size_t)7);
+    double* jtile = (double*)(smem + joff);
+    uint64_t* mask = (uint64_t*)(smem + joff + sizeof(double) * 3 * dtn);
     int* wtot = (int*)(mask + sp.mask_words);
     int* dircnt = wtot + WAVES;
     int* bad = dircnt + 27;
@@ -145,10 +191,10 @@ __global__ void __launch_bounds__(PK_BLOCK) k_particles_split(GridView<float> g,
     const int lane = tid & 63, wave = tid >> 6;
     for (int t = tid; t < 6 * itn; t += PK_BLOCK) {
         const int comp = t / itn, r = t % itn;
-        const int i = r / (its[1] * its[2]), j = (r / its[2]) % its[1], k = r % its[2];
+        const int i = r / isx, j = (r / isy) % its[1], k = r % isy;
         itile[t] = g.f[PK_EX + comp][((long)(ibase[0] + i) * g.s1 + (ibase[1] + j)) * g.s2 + (ibase[2] + k)];
     }
-    for (int t = tid; t < 3 * dtn; t += PK_BLOCK) jtile[t] = 0.0f;
+    for (int t = tid; t < 3 * dtn; t += PK_BLOCK) jtile[t] = 0.0;
     for (int t = tid; t < 27; t += PK_BLOCK) dircnt[t] = 0;
     if (tid == 0) bad[0] = 0;
     __syncthreads();
@@ -160,32 +206,149 @@ __global__ void __launch_bounds__(PK_BLOCK) k_particles_split(GridView<float> g,
     for (int c0 = 0; c0 < n; c0 += PK_BLOCK) {
         const int i = c0 + tid;
         const bool act = i < n;
-        int cell = 0, ncell_lin = 0, code = 13;
-        float o[3], u[3], w = 0.0f;
-        bool ok = true;
+        int cell = 0, code = 13;
+        int c[3] = {0, 0, 0}, l[3] = {0, 0, 0}, sh[3] = {0, 0, 0};
+        float o[3] = {0, 0, 0}, on[3] = {0, 0, 0}, u[3] = {0, 0, 0}, v[3] = {0, 0, 0}, w = 0.0f;
+        bool ok = act;
+        // ---- load, interpolate, push, move (lane-divergent)
         if (act) {
             const long p = base + i;
             cell = pm.cell[p];
             o[0] = pm.ox[p]; o[1] = pm.oy[p]; o[2] = pm.oz[p];
             u[0] = pm.ux[p]; u[1] = pm.uy[p]; u[2] = pm.uz[p];
             w = pm.w[p];
-            int c[3] = {cell / N1N2, (cell / eg.ncell[2]) % eg.ncell[1], cell % eg.ncell[2]};
-            int l[3];
+            c[0] = cell / N1N2; c[1] = (cell / eg.ncell[2]) % eg.ncell[1]; c[2] = cell % eg.ncell[2];
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 l[d] = c[d] - m0[d];
-                if (l[d] < 0 || l[d] >= eg.ncps[d] || !(o[d] >= 0.0f && o[d] < 1.0f)) ok = false;
+                if (l[d] < 0 || l[d] >= nn[d] || !(o[d] >= 0.0f && o[d] < 1.0f)) ok = false;
             }
             ok = ok && (unsigned)cell < (unsigned)(eg.ncell[0] * N1N2);
-            float e[3] = {0, 0, 0}, b[3] = {0, 0, 0};
+        }
+        if (ok) {
+            float e[3], b[3];
+            int bc[3], bs[3];
+            float wc[3][3], ws[3][3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) interp_stencil<SHAPE>(l[d], o[d], bc[d], wc[d], bs[d], ws[d]);
+#pragma unroll
+            for (int comp = 0; comp < 6; comp++) {
+                int bi[3];
+                float wt[3][3];
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    const bool col = colloc(comp, d);
+                    bi[d] = col ? bc[d] : bs[d];
+#pragma unroll
+                    for (int k = 0; k < 3; k++) wt[d][k] = col ? wc[d][k] : ws[d][k];
+                }
+                const float val = gather<SHAPE>(itile + comp * itn, isx, isy, bi, wt);
+                if (comp < 3) e[comp] = val; else b[comp - 3] = val;
+            }
+            // Boris push (BorisPusher.h:32-47) in FP32
+            float eM[3], um[3], t[3], up[3], sv[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) eM[d] = e[d] * sp.coeff;
+#pragma unroll
+            for (int d = 0; d < 3; d++) um[d] = u[d] + eM[d];
+            const float tk = sp.coeff * rsqrtf(1.0f + um[0] * um[0] + um[1] * um[1] + um[2] * um[2]);
+#pragma unroll
+            for (int d = 0; d < 3; d++) t[d] = b[d] * tk;
+            up[0] = um[0] + (um[1] * t[2] - um[2] * t[1]);
+            up[1] = um[1] + (um[2] * t[0] - um[0] * t[2]);
+            up[2] = um[2] + (um[0] * t[1] - um[1] * t[0]);
+            const float sk = 2.0f * __builtin_amdgcn_rcpf(1.0f + t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+#pragma unroll
+            for (int d = 0; d < 3; d++) sv[d] = t[d] * sk;
+            u[0] = um[0] + (up[1] * sv[2] - up[2] * sv[1]) + eM[0];
+            u[1] = um[1] + (up[2] * sv[0] - up[0] * sv[2]) + eM[1];
+            u[2] = um[2] + (up[0] * sv[1] - up[1] * sv[0]) + eM[2];
+            const float invGamma = rsqrtf(1.0f + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+#pragma unroll
+            for (int d = 0; d < 3; d++) v[d] = u[d] * invGamma;    // in units of c
+            ok = isfinite(u[0]) && isfinite(u[1]) && isfinite(u[2]);
+            // new offset and cell shift in {-1, 0, 1}
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                float x = o[d] + v[d] * sp.cdt_h[d];
+                int k = x < 0.0f ? -1 : (x >= 1.0f ? 1 : 0);
+                x -= (float)k;
+                if (x >= 1.0f) { x = 0.0f; k += 1; }     // a tiny negative offset rounded up to 1
+                if (x < 0.0f) x = 0.0f;
+                on[d] = x;
+                sh[d] = k;
+            }
+        }
+        // ---- deposition
+        const float qw = sp.q * w;
+        if (DEP == PK_ESIRKEPOV) {
+            // window per dimension: is a larger one needed by any lane of this wavefront?
+            bool big[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                bool need;
+                if (SHAPE == PK_TSC) {
+                    need = sh[d] != 0;
+                } else {
+                    const int lo0 = o[d] < 0.5f ? -1 : 0;
+                    const int lo1 = sh[d] + (on[d] < 0.5f ? -1 : 0);
+                    need = lo0 != lo1;
+                }
+                big[d] = __ballot(ok && need) != 0ull;
+            }
+            float S0[3][4], DS[3][4];
+            int wb[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const int nc0 = l[d] + 2, nc1 = l[d] + sh[d] + 2;
+                if (SHAPE == PK_TSC) {
+                    wb[d] = (big[d] ? (nc0 < nc1 ? nc0 : nc1) : nc0) - 1;
+                } else {
+                    const int lo0 = o[d] < 0.5f ? nc0 - 1 : nc0;
+                    const int lo1 = on[d] < 0.5f ? nc1 - 1 : nc1;
+                    wb[d] = big[d] ? (lo0 < lo1 ? lo0 : lo1) : lo0;
+                }
+                float S1[4];
+                rho_shape<SHAPE>(nc0, o[d], wb[d], S0[d]);
+                rho_shape<SHAPE>(nc1, on[d], wb[d], S1);
+#pragma unroll
+                for (int k = 0; k < 4; k++) DS[d][k] = S1[k] - S0[d][k];
+            }
+            const float kq[3] = {-qw * sp.hdt[0], -qw * sp.hdt[1], -qw * sp.hdt[2]};
+            constexpr int KS = SHAPE == PK_TSC ? 3 : 2;    // window without a node change
+            const int sel = (big[0] ? 4 : 0) | (big[1] ? 2 : 0) | (big[2] ? 1 : 0);
             if (ok) {
-                // ---- interpolation
-                int bc[3], bs[3];
-                float wc[3][3], ws[3][3];
+                switch (sel) {
+                case 0: esirkepov<KS, KS, KS>(jtile, dtn, dsx, dsy, wb, S0, DS, kq); break;
+                case 1: esirkepov<KS, KS, KS + 1>(jtile, dtn, dsx, dsy, wb, S0, DS, kq); break;
+                case 2: esirkepov<KS, KS + 1, KS>(jtile, dtn, dsx, dsy, wb, S0, DS, kq); break;
+                case 3: esirkepov<KS, KS + 1, KS + 1>(jtile, dtn, dsx, dsy, wb, S0, DS, kq); break;
+                case 4: esirkepov<KS + 1, KS, KS>(jtile, dtn, dsx, dsy, wb, S0, DS, kq); break;
+                case 5: esirkepov<KS + 1, KS, KS + 1>(jtile, dtn, dsx, dsy, wb, S0, DS, kq); break;
+                case 6: esirkepov<KS + 1, KS + 1, KS>(jtile, dtn, dsx, dsy, wb, S0, DS, kq); break;
+                default: esirkepov<KS + 1, KS + 1, KS + 1>(jtile, dtn, dsx, dsy, wb, S0, DS, kq); break;
+                }
+            }
+        } else if (ok) {
+            // direct deposition of q w v at the new position: the CIC tile is pica's (base m+1,
+            // the interpolation indices), the TSC tile has base m (indices + 1)
+            constexpr int KK = SHAPE == PK_TSC ? 3 : 2;
+            int bc[3], bs[3];
+            float wc[3][3], ws[3][3];
 #pragma unroll
-                for (int d = 0; d < 3; d++) interp_stencil<SHAPE>(l[d], o[d], bc[d], wc[d], bs[d], ws[d]);
+            for (int d = 0; d < 3; d++) {
+                interp_stencil<SHAPE>(l[d] + sh[d], on[d], bc[d], wc[d], bs[d], ws[d]);
+                if (SHAPE == PK_TSC) {
+                    bc[d] += 1;
+                    bs[d] += 1;
+                }
+                if (bc[d] < 0 || bc[d] + KK > dts[d] || bs[d] < 0 || bs[d] + KK > dts[d]) ok = false;
+            }
+            if (ok) {
 #pragma unroll
-                for (int comp = 0; comp < 6; comp++) {
+                for (int comp = 0; comp < 3; comp++) {
+                    const float val = v[comp] * (float)PK_C * qw;
+                    double* t = jtile + comp * dtn;
                     int bi[3];
                     float wt[3][3];
 #pragma unroll
@@ -195,181 +358,42 @@ __global__ void __launch_bounds__(PK_BLOCK) k_particles_split(GridView<float> g,
 #pragma unroll
                         for (int k = 0; k < 3; k++) wt[d][k] = col ? wc[d][k] : ws[d][k];
                     }
-                    const float v = gather<SHAPE>(itile + comp * itn, its, bi, wt);
-                    if (comp < 3) e[comp] = v; else b[comp - 3] = v;
+                    double* tb = t + bi[0] * dsx + bi[1] * dsy + bi[2];
+#pragma unroll
+                    for (int a = 0; a < KK; a++)
+#pragma unroll
+                        for (int bb = 0; bb < KK; bb++)
+#pragma unroll
+                            for (int cc = 0; cc < KK; cc++)
+                                atomicAdd(&tb[a * dsx + bb * dsy + cc], (double)(wt[0][a] * wt[1][bb] * wt[2][cc] * val));
                 }
             }
-            // ---- Boris push (BorisPusher.h:32-47) in FP32
-            float v[3] = {0, 0, 0};
-            if (ok) {
-                float eM[3], um[3], t[3], up[3], sv[3];
+        }
+        // ---- periodic wrap of the cell and direction class
+        int ncell_lin = cell;
+        if (ok) {
+            int cn[3];
+            code = 0;
 #pragma unroll
-                for (int d = 0; d < 3; d++) eM[d] = e[d] * sp.coeff;
-#pragma unroll
-                for (int d = 0; d < 3; d++) um[d] = u[d] + eM[d];
-                const float tk = sp.coeff * rsqrtf(1.0f + um[0] * um[0] + um[1] * um[1] + um[2] * um[2]);
-#pragma unroll
-                for (int d = 0; d < 3; d++) t[d] = b[d] * tk;
-                up[0] = um[0] + (um[1] * t[2] - um[2] * t[1]);
-                up[1] = um[1] + (um[2] * t[0] - um[0] * t[2]);
-                up[2] = um[2] + (um[0] * t[1] - um[1] * t[0]);
-                const float sk = 2.0f / (1.0f + t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
-#pragma unroll
-                for (int d = 0; d < 3; d++) sv[d] = t[d] * sk;
-                u[0] = um[0] + (up[1] * sv[2] - up[2] * sv[1]) + eM[0];
-                u[1] = um[1] + (up[2] * sv[0] - up[0] * sv[2]) + eM[1];
-                u[2] = um[2] + (up[0] * sv[1] - up[1] * sv[0]) + eM[2];
-                const float invGamma = rsqrtf(1.0f + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
-#pragma unroll
-                for (int d = 0; d < 3; d++) v[d] = u[d] * invGamma;    // in units of c
-                ok = isfinite(u[0]) && isfinite(u[1]) && isfinite(u[2]);
+            for (int d = 0; d < 3; d++) {
+                cn[d] = c[d] + sh[d];
+                if (cn[d] < 0) cn[d] += eg.ncell[d];
+                if (cn[d] >= eg.ncell[d]) cn[d] -= eg.ncell[d];
+                int dd = cn[d] / nn[d] - sidx[d];
+                if (dd > 1) dd -= eg.nsc[d];
+                if (dd < -1) dd += eg.nsc[d];
+                code = code * 3 + (dd + 1);
+                o[d] = on[d];
             }
-            // ---- move: new offset, cell shift in {-1, 0, 1}
-            float on[3];
-            int sh[3];
-            if (ok) {
-#pragma unroll
-                for (int d = 0; d < 3; d++) {
-                    float x = o[d] + v[d] * sp.cdt_h[d];
-                    int k = x < 0.0f ? -1 : (x >= 1.0f ? 1 : 0);
-                    x -= (float)k;
-                    if (x >= 1.0f) { x = 0.0f; k += 1; }     // a tiny negative offset rounded up to 1
-                    if (x < 0.0f) x = 0.0f;
-                    on[d] = x;
-                    sh[d] = k;
-                    if (k < -1 || k > 1) ok = false;
-                }
-            }
-            // ---- deposition
-            if (ok) {
-                const float qw = sp.q * w;
-                if (DEP == PK_ESIRKEPOV) {
-                    constexpr int K = SHAPE == PK_TSC ? 4 : 3;
-                    float S0[3][4], DS[3][4];
-                    int wb[3];
-#pragma unroll
-                    for (int d = 0; d < 3; d++) {
-                        const int nc0 = l[d] + 2, nc1 = l[d] + sh[d] + 2;
-                        if (SHAPE == PK_TSC) {
-                            // nearest rho node is the cell's own centre; window of 4 from min - 1
-                            wb[d] = (nc0 < nc1 ? nc0 : nc1) - 1;
-                        } else {
-                            // lower CIC node: the centre below the particle; window of 3 from the min
-                            const int lo0 = o[d] < 0.5f ? nc0 - 1 : nc0;
-                            const int lo1 = on[d] < 0.5f ? nc1 - 1 : nc1;
-                            wb[d] = lo0 < lo1 ? lo0 : lo1;
-                        }
-                        float S1[4];
-                        rho_shape<SHAPE>(nc0, o[d], wb[d], S0[d]);
-                        rho_shape<SHAPE>(nc1, on[d], wb[d], S1);
-#pragma unroll
-                        for (int k = 0; k < 4; k++) DS[d][k] = S1[k] - S0[d][k];
-                    }
-                    const float third = 1.0f / 3.0f;
-#pragma unroll
-                    for (int comp = 0; comp < 3; comp++) {
-                        const int d1 = (comp + 1) % 3, d2 = (comp + 2) % 3;
-                        const float kq = -qw * sp.hdt[comp];
-                        float* t = jtile + comp * dtn;
-                        int stride[3];
-                        stride[0] = dts[1] * dts[2];
-                        stride[1] = dts[2];
-                        stride[2] = 1;
-                        const int orig = wb[0] * stride[0] + wb[1] * stride[1] + wb[2] * stride[2] + stride[comp];
-#pragma unroll
-                        for (int p = 0; p < K; p++)
-#pragma unroll
-                            for (int r = 0; r < K; r++) {
-                                const float coef = kq * (S0[d1][p] * S0[d2][r] + 0.5f * DS[d1][p] * S0[d2][r] +
-                                                         0.5f * S0[d1][p] * DS[d2][r] + third * DS[d1][p] * DS[d2][r]);
-                                float cum = 0.0f;
-                                const int rowbase = orig + p * stride[d1] + r * stride[d2];
-#pragma unroll
-                                for (int a = 0; a < K - 1; a++) {
-                                    cum += DS[comp][a] * coef;
-                                    if (cum != 0.0f) atomicAdd(&t[rowbase + a * stride[comp]], cum);
-                                }
-                            }
-                    }
-                } else {
-                    // direct deposition of q w v at the new position: the CIC tile is pica's
-                    // (base m+1, the interpolation indices), the TSC tile has base m (indices + 1);
-                    // check the stencil fits before touching the tile
-                    int bc[3], bs[3];
-                    float wc[3][3], ws[3][3];
-#pragma unroll
-                    for (int d = 0; d < 3; d++) {
-                        interp_stencil<SHAPE>(l[d] + sh[d], on[d], bc[d], wc[d], bs[d], ws[d]);
-                        if (SHAPE == PK_TSC) {
-                            bc[d] += 1;
-                            bs[d] += 1;
-                        }
-                    }
-                    constexpr int KK = SHAPE == PK_TSC ? 3 : 2;
-#pragma unroll
-                    for (int d = 0; d < 3; d++)
-                        if (bc[d] < 0 || bc[d] + KK > dts[d] || bs[d] < 0 || bs[d] + KK > dts[d]) ok = false;
-                }
-                if (ok && DEP == PK_DIRECT) {
-                    constexpr int KK = SHAPE == PK_TSC ? 3 : 2;
-                    int bc[3], bs[3];
-                    float wc[3][3], ws[3][3];
-#pragma unroll
-                    for (int d = 0; d < 3; d++) {
-                        interp_stencil<SHAPE>(l[d] + sh[d], on[d], bc[d], wc[d], bs[d], ws[d]);
-                        if (SHAPE == PK_TSC) {
-                            bc[d] += 1;
-                            bs[d] += 1;
-                        }
-                    }
-#pragma unroll
-                    for (int comp = 0; comp < 3; comp++) {
-                        const float val = v[comp] * (float)PK_C * qw;
-                        float* t = jtile + comp * dtn;
-                        int bi[3];
-                        float wt[3][3];
-#pragma unroll
-                        for (int d = 0; d < 3; d++) {
-                            const bool col = colloc(comp, d);
-                            bi[d] = col ? bc[d] : bs[d];
-#pragma unroll
-                            for (int k = 0; k < 3; k++) wt[d][k] = col ? wc[d][k] : ws[d][k];
-                        }
-#pragma unroll
-                        for (int a = 0; a < KK; a++)
-#pragma unroll
-                            for (int bb = 0; bb < KK; bb++)
-#pragma unroll
-                                for (int cc = 0; cc < KK; cc++)
-                                    atomicAdd(&t[tix(dts, bi[0] + a, bi[1] + bb, bi[2] + cc)],
-                                              wt[0][a] * wt[1][bb] * wt[2][cc] * val);
-                    }
-                }
-                // ---- periodic wrap of the cell and direction class
-                int cn[3];
-                code = 0;
-#pragma unroll
-                for (int d = 0; d < 3; d++) {
-                    cn[d] = c[d] + sh[d];
-                    if (cn[d] < 0) cn[d] += eg.ncell[d];
-                    if (cn[d] >= eg.ncell[d]) cn[d] -= eg.ncell[d];
-                    int dd = cn[d] / eg.ncps[d] - sidx[d];
-                    if (dd > 1) dd -= eg.nsc[d];
-                    if (dd < -1) dd += eg.nsc[d];
-                    code = code * 3 + (dd + 1);
-                    o[d] = on[d];
-                }
-                ncell_lin = (cn[0] * eg.ncell[1] + cn[1]) * eg.ncell[2] + cn[2];
-            }
-            if (!ok) {
-                // leave the particle unchanged and report
-                cell = pm.cell[p];
-                ncell_lin = cell;
-                o[0] = pm.ox[p]; o[1] = pm.oy[p]; o[2] = pm.oz[p];
-                u[0] = pm.ux[p]; u[1] = pm.uy[p]; u[2] = pm.uz[p];
-                code = 13;
-                bad[0] = 1;
-            }
+            ncell_lin = (cn[0] * eg.ncell[1] + cn[1]) * eg.ncell[2] + cn[2];
+        } else if (act) {
+            // leave the particle unchanged and report
+            const long p = base + i;
+            ncell_lin = pm.cell[p];
+            o[0] = pm.ox[p]; o[1] = pm.oy[p]; o[2] = pm.oz[p];
+            u[0] = pm.ux[p]; u[1] = pm.uy[p]; u[2] = pm.uz[p];
+            code = 13;
+            bad[0] = 1;
         }
         const bool mover = act && code != 13;
         const uint64_t mb = __ballot(mover);
@@ -447,12 +471,12 @@ __global__ void __launch_bounds__(PK_BLOCK) k_particles_split(GridView<float> g,
     }
     __syncthreads();
     for (int t = tid; t < 3 * dtn; t += PK_BLOCK) {
-        const float v = jtile[t];
-        if (v == 0.0f) continue;
+        const double v = jtile[t];
+        if (v == 0.0) continue;
         const int comp = t / dtn, r = t % dtn;
-        const int i = r / (dts[1] * dts[2]), j = (r / dts[2]) % dts[1], k = r % dts[2];
+        const int i = r / dsx, j = (r / dsy) % dts[1], k = r % dsy;
         atomicAdd(&g.f[PK_JX + comp][((long)(dbase[0] + i) * g.s1 + (dbase[1] + j)) * g.s2 + (dbase[2] + k)],
-                  v * sp.norm);
+                  (float)(v * (double)sp.norm));
     }
     if (tid < 27) mvc[s * 27 + tid] = dircnt[tid];
     if (tid == 0) {
@@ -462,7 +486,7 @@ __global__ void __launch_bounds__(PK_BLOCK) k_particles_split(GridView<float> g,
     }
 }
 
-template <int SHAPE, int DEP>
+template <int SHAPE, int DEP, int NC>
 int launch(pk_ens* e, pk_grid* g, double dt)
 {
     EnsGeom eg = ens_geom(e);
@@ -483,25 +507,36 @@ int launch(pk_ens* e, pk_grid* g, double dt)
         its[d] = e->ncps[d] + (SHAPE == PK_TSC ? 3 : 2);
         dts[d] = e->ncps[d] + ((SHAPE == PK_CIC && DEP == PK_DIRECT) ? 3 : 4);
     }
-    const size_t lds = sizeof(float) * (6 * its[0] * its[1] * its[2] + 3 * dts[0] * dts[1] * dts[2]) + 8 +
-                       sizeof(uint64_t) * sp.mask_words + sizeof(int) * 32 + sizeof(long) * (sp.mask_words + 1) + 16;
+    const size_t joff = (sizeof(float) * 6 * its[0] * its[1] * its[2] + 7) & ~(size_t)7;
+    const size_t lds = joff + sizeof(double) * 3 * dts[0] * dts[1] * dts[2] + sizeof(uint64_t) * sp.mask_words +
+                       sizeof(int) * 32 + sizeof(long) * (sp.mask_words + 1) + 16;
     if (lds > 160 * 1024) return pk_fail(e->ctx, PK_E_CAPACITY, "particle step: supercell too large for LDS");
     const unsigned nblocks = (unsigned)(PK_XCD * (long)eg.xcd_chunk);
     pk_ktimer tm(e->ctx, "particles");
-    hipLaunchKernelGGL((k_particles_split<SHAPE, DEP>), dim3(nblocks), dim3(PK_BLOCK), lds, e->ctx->stream,
+    hipLaunchKernelGGL((k_particles_split<SHAPE, DEP, NC>), dim3(nblocks), dim3(PK_BLOCK), lds, e->ctx->stream,
                        grid_view<float>(g), eg, part_split(e, e->main), part_split(e, e->outbox), e->odir, e->off,
                        e->cnt, e->nstay, e->nmove, e->mvc, e->flags, sp);
     PK_HIP(e->ctx, hipGetLastError());
     return PK_OK;
 }
 
+template <int SHAPE, int DEP>
+int launch_nc(pk_ens* e, pk_grid* g, double dt)
+{
+    const bool cube = e->ncps[0] == e->ncps[1] && e->ncps[1] == e->ncps[2];
+    if (cube && e->ncps[0] == 4) return launch<SHAPE, DEP, 4>(e, g, dt);
+    if (cube && e->ncps[0] == 2) return launch<SHAPE, DEP, 2>(e, g, dt);
+    if (cube && e->ncps[0] == 8) return launch<SHAPE, DEP, 8>(e, g, dt);
+    return launch<SHAPE, DEP, 0>(e, g, dt);
+}
+
 }  // namespace
 
 int pk_particles_split(pk_ens* e, pk_grid* g, double dt, int shape, int dep)
 {
-    if (shape == PK_CIC && dep == PK_DIRECT) return launch<PK_CIC, PK_DIRECT>(e, g, dt);
-    if (shape == PK_CIC && dep == PK_ESIRKEPOV) return launch<PK_CIC, PK_ESIRKEPOV>(e, g, dt);
-    if (shape == PK_TSC && dep == PK_DIRECT) return launch<PK_TSC, PK_DIRECT>(e, g, dt);
-    if (shape == PK_TSC && dep == PK_ESIRKEPOV) return launch<PK_TSC, PK_ESIRKEPOV>(e, g, dt);
+    if (shape == PK_CIC && dep == PK_DIRECT) return launch_nc<PK_CIC, PK_DIRECT>(e, g, dt);
+    if (shape == PK_CIC && dep == PK_ESIRKEPOV) return launch_nc<PK_CIC, PK_ESIRKEPOV>(e, g, dt);
+    if (shape == PK_TSC && dep == PK_DIRECT) return launch_nc<PK_TSC, PK_DIRECT>(e, g, dt);
+    if (shape == PK_TSC && dep == PK_ESIRKEPOV) return launch_nc<PK_TSC, PK_ESIRKEPOV>(e, g, dt);
     return pk_fail(e->ctx, PK_E_ARG, "particle step: unknown shape/depositor");
 }
