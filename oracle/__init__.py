@@ -329,3 +329,123 @@ def ref_run_cic_direct(ncell, step, ncps, q, m, dt, nsteps, x, u, w, f=None, nth
     if rc:
         raise RuntimeError("ref_run_cic_direct failed: %d" % rc)
     return np.stack(xs, 1), np.stack(us, 1), ws, off, grid
+
+
+def interp_sc(grid, scmin, ncps, shape, pos):
+    """supercell interpolation (pica's FieldInterpolatorCICSupercell for CIC) of positions pos."""
+    n = pos.shape[0]
+    xs, xp = _xyz(pos)
+    e = [np.zeros(n) for _ in range(3)]
+    b = [np.zeros(n) for _ in range(3)]
+    r = lib().or_interp_sc(grid.c(), (ctypes.c_double * 3)(*scmin), (ctypes.c_int * 3)(*ncps), shape,
+                           ctypes.c_long(n), xp, (_dp * 3)(*[_ptr(v) for v in e]), (_dp * 3)(*[_ptr(v) for v in b]))
+    if r:
+        raise ValueError("or_interp_sc: position outside the supercell tile (%d)" % r)
+    return np.stack(e, 1), np.stack(b, 1)
+
+
+def deposit_direct_sc(grid, scmin, ncps, shape, pos, cur):
+    """supercell direct deposition (pica's CurrentDepositorCICSupercell for CIC), flushed x 1/V."""
+    n = pos.shape[0]
+    xs, xp = _xyz(pos)
+    cs, cp = _xyz(cur)
+    r = lib().or_deposit_direct_sc(grid.c(), (ctypes.c_double * 3)(*scmin), (ctypes.c_int * 3)(*ncps), shape,
+                                   ctypes.c_long(n), xp, cp)
+    if r:
+        raise ValueError("or_deposit_direct_sc: position outside the supercell tile (%d)" % r)
+
+
+def deposit_esirkepov_sc(grid, scmin, ncps, shape, x0, x1, qw, dt):
+    """Esirkepov deposition of particles moving x0 -> x1 with charges qw, flushed x 1/V."""
+    n = x0.shape[0]
+    a, ap = _xyz(x0)
+    b, bp = _xyz(x1)
+    q = np.ascontiguousarray(qw, np.float64)
+    r = lib().or_deposit_esirkepov_sc(grid.c(), (ctypes.c_double * 3)(*scmin), (ctypes.c_int * 3)(*ncps), shape,
+                                      ctypes.c_long(n), ap, bp, _ptr(q), ctypes.c_double(dt))
+    if r:
+        raise ValueError("or_deposit_esirkepov_sc: move outside the supercell tile (%d)" % r)
+
+
+class _Ref:
+    """thin access to pica's own entry points in oracle/_ref (see ref_harness.cpp)"""
+
+    @staticmethod
+    def yee(kind, grid, dt):
+        r = ref()
+        s = (ctypes.c_int * 3)(*grid.s)
+        if grid.dtype == np.float64:
+            fn = r.ref_yee_update_e if kind == "e" else r.ref_yee_update_b
+            fn(s, (ctypes.c_double * 3)(*grid.origin), (ctypes.c_double * 3)(*grid.step), grid.comps_ptrs(),
+               ctypes.c_double(dt))
+        else:
+            fn = r.ref_yee_update_e_f32 if kind == "e" else r.ref_yee_update_b_f32
+            fn(s, (ctypes.c_float * 3)(*grid.origin), (ctypes.c_float * 3)(*grid.step), grid.comps_ptrs(),
+               ctypes.c_float(dt))
+
+    @staticmethod
+    def interp_cic(grid, pos, scmin=None, ncps=None):
+        r = ref()
+        n = pos.shape[0]
+        xs = [np.ascontiguousarray(pos[:, d], np.float64) for d in range(3)]
+        e = np.zeros((n, 3))
+        b = np.zeros((n, 3))
+        args = [(ctypes.c_int * 3)(*grid.s), (ctypes.c_double * 3)(*grid.origin), (ctypes.c_double * 3)(*grid.step),
+                grid.comps_ptrs()]
+        if scmin is None:
+            r.ref_interp_cic(*args, ctypes.c_long(n), _ptr(xs[0]), _ptr(xs[1]), _ptr(xs[2]), _ptr(e), _ptr(b))
+        else:
+            r.ref_interp_cic_sc(*args, (ctypes.c_double * 3)(*scmin), (ctypes.c_int * 3)(*ncps), ctypes.c_long(n),
+                                _ptr(xs[0]), _ptr(xs[1]), _ptr(xs[2]), _ptr(e), _ptr(b))
+        return e, b
+
+    @staticmethod
+    def deposit_cic(grid, pos, cur, scmin=None, ncps=None):
+        r = ref()
+        n = pos.shape[0]
+        xs = [np.ascontiguousarray(pos[:, d], np.float64) for d in range(3)]
+        c = np.ascontiguousarray(cur, np.float64)
+        args = [(ctypes.c_int * 3)(*grid.s), (ctypes.c_double * 3)(*grid.origin), (ctypes.c_double * 3)(*grid.step),
+                grid.comps_ptrs()]
+        if scmin is None:
+            r.ref_deposit_cic(*args, ctypes.c_long(n), _ptr(xs[0]), _ptr(xs[1]), _ptr(xs[2]), _ptr(c))
+        else:
+            r.ref_deposit_cic_sc(*args, (ctypes.c_double * 3)(*scmin), (ctypes.c_int * 3)(*ncps), ctypes.c_long(n),
+                                 _ptr(xs[0]), _ptr(xs[1]), _ptr(xs[2]), _ptr(c))
+
+    @staticmethod
+    def boris(x, u, e, b, w, q, m, dt):
+        r = ref()
+        n = x.shape[0]
+        xx = [np.ascontiguousarray(x[:, d], np.float64).copy() for d in range(3)]
+        uu = [np.ascontiguousarray(u[:, d], np.float64).copy() for d in range(3)]
+        r.ref_boris_push(ctypes.c_long(n), ctypes.c_double(q), ctypes.c_double(m), ctypes.c_double(dt), _ptr(xx[0]),
+                         _ptr(xx[1]), _ptr(xx[2]), _ptr(uu[0]), _ptr(uu[1]), _ptr(uu[2]),
+                         _ptr(np.ascontiguousarray(w, np.float64)), _ptr(np.ascontiguousarray(e, np.float64)),
+                         _ptr(np.ascontiguousarray(b, np.float64)))
+        return np.stack(xx, 1), np.stack(uu, 1)
+
+    @staticmethod
+    def cell_index(grid, pos):
+        r = ref()
+        n = pos.shape[0]
+        xs = [np.ascontiguousarray(pos[:, d], np.float64) for d in range(3)]
+        out = np.zeros((n, 3), np.int32)
+        r.ref_cell_index((ctypes.c_int * 3)(*grid.s), (ctypes.c_double * 3)(*grid.origin),
+                         (ctypes.c_double * 3)(*grid.step), ctypes.c_long(n), _ptr(xs[0]), _ptr(xs[1]), _ptr(xs[2]),
+                         _ptr(out, ctypes.c_int))
+        return out
+
+    @staticmethod
+    def supercell_index(mn, mx, ncell, ncps, pos):
+        r = ref()
+        n = pos.shape[0]
+        xs = [np.ascontiguousarray(pos[:, d], np.float64) for d in range(3)]
+        out = np.zeros((n, 3), np.int32)
+        r.ref_supercell_index((ctypes.c_double * 3)(*mn), (ctypes.c_double * 3)(*mx), (ctypes.c_int * 3)(*ncell),
+                              (ctypes.c_int * 3)(*ncps), ctypes.c_long(n), _ptr(xs[0]), _ptr(xs[1]), _ptr(xs[2]),
+                              _ptr(out, ctypes.c_int))
+        return out
+
+
+REF = _Ref()

@@ -780,8 +780,10 @@ int or_deposit_esirkepov_sc(or_grid* g, const double scmin[3], const int ncps[3]
     return rc;
 }
 
-/* Particle stage over all supercells in 8 colours (2x2x2): tiles of supercells of one colour
- * never overlap, so the J sums are independent of thread scheduling. */
+/* Particle stage over all supercells in colours: supercells p apart in a dimension, with
+ * p = ceil((n + 4) / n) (2 for n >= 4, so 2x2x2 = 8 colours), have disjoint deposition tiles,
+ * so one colour runs in parallel and the J sums do not depend on thread scheduling.
+ * colour = ((sx % px) * py + sy % py) * pz + sz % pz, colours in increasing order. */
 int or_particle_stage(or_grid* g, or_ens* e, double dt, int shape, int dep, int nthreads)
 {
     const double coeff = or_boris_coeff(e->q, e->m, dt);
@@ -792,7 +794,10 @@ int or_particle_stage(or_grid* g, or_ens* e, double dt, int shape, int dep, int 
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
 #endif
-    for (int colour = 0; colour < 8; colour++) {
+    int pc[3];
+    for (int d = 0; d < 3; d++) pc[d] = (e->ncps[d] + 4 + e->ncps[d] - 1) / e->ncps[d];
+    const int ncolours = pc[0] * pc[1] * pc[2];
+    for (int colour = 0; colour < ncolours; colour++) {
         #pragma omp parallel
         {
             double* work = (double*)malloc(sizeof(double) * wn);
@@ -801,7 +806,7 @@ int or_particle_stage(or_grid* g, or_ens* e, double dt, int shape, int dep, int 
                 const int sx = (int)(s / ((long)e->nsc[1] * e->nsc[2]));
                 const int sy = (int)((s / e->nsc[2]) % e->nsc[1]);
                 const int sz = (int)(s % e->nsc[2]);
-                if (((sx & 1) * 4 + (sy & 1) * 2 + (sz & 1)) != colour) continue;
+                if (((sx % pc[0]) * pc[1] + sy % pc[1]) * pc[2] + sz % pc[2] != colour) continue;
                 const int r = or_process_supercell(g, e, sx, sy, sz, dt, shape, dep, coeff, work);
                 if (r) {
                     #pragma omp atomic write

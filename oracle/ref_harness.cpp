@@ -313,13 +313,16 @@ int ref_run_cic_direct(const int ncell[3], const double step[3], const int ncps[
         for (int i = 0; i < gs.x; i++)
             for (int j = 0; j < gs.y; j++)
                 for (int k = 0; k < gs.z; k++) g.jx(i, j, k) = g.jy(i, j, k) = g.jz(i, j, k) = 0.0;
-        for (int colour = 0; colour < 8; colour++) {
+        // colours as in or_particle_stage: supercells p = ceil((n+4)/n) apart never share tile cells
+        int pc[3];
+        for (int d = 0; d < 3; d++) pc[d] = (nps[d] + 4 + nps[d] - 1) / nps[d];
+        for (int colour = 0; colour < pc[0] * pc[1] * pc[2]; colour++) {
             #pragma omp parallel for schedule(dynamic, 4)
             for (long s = 0; s < nsct; s++) {
                 const int sx = (int)(s / ((long)nsc.y * nsc.z));
                 const int sy = (int)((s / nsc.z) % nsc.y);
                 const int sz = (int)(s % nsc.z);
-                if (((sx & 1) * 4 + (sy & 1) * 2 + (sz & 1)) != colour) continue;
+                if (((sx % pc[0]) * pc[1] + sy % pc[1]) * pc[2] + sz % pc[2] != colour) continue;
                 const FP3 scMin(sx * nps.x * h.x, sy * nps.y * h.y, sz * nps.z * h.z);
                 FieldInterpolatorCICSupercell<double> interp(g, scMin, nps);
                 CurrentDepositorCICSupercell<double> dep(g, scMin, nps);

@@ -1,0 +1,54 @@
+"""The C++ surface (include/pica_amd/pica.hpp, pica's names over the C ABI) driven by a plain
+g++ program, checked against the oracle: particles and fields of a short TSC/Esirkepov run."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+import pica_amd as P
+from helpers import courant_dt, plasma, random_particles, rel_err
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_cxx_surface_matches_oracle(tmp_path):
+    print("[case] c++ surface", flush=True)
+    exe = tmp_path / "surface_run"
+    libdir = os.path.dirname(P.LIBPATH)
+    subprocess.run(["g++", "-std=c++17", "-O2", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "cxx", "surface_run.cpp"), "-o", str(exe), "-L", libdir, "-lpica_amd",
+                    "-Wl,-rpath," + libdir], check=True)
+    rng = np.random.default_rng(17)
+    ncell, ncps, ppc = (8, 8, 8), (4, 4, 4), 4
+    dt = courant_dt()
+    n0, w0 = plasma(dt, ppc)
+    x, u = random_particles(rng, ncell, ppc, uth=0.05, drift=0.1)
+    w = np.full(x.shape[0], w0)
+    inp = tmp_path / "in.bin"
+    with open(inp, "wb") as f:
+        np.array(ncell, np.int32).tofile(f)
+        np.array(ncps, np.int32).tofile(f)
+        np.array([x.shape[0]], np.int64).tofile(f)
+        np.concatenate([x, u, w[:, None]], 1).astype(np.float64).tofile(f)
+    out = tmp_path / "out.bin"
+    r = subprocess.run([str(exe), str(inp), str(out), "5", "1", "1"], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    raw = np.fromfile(out, np.uint8)
+    n = int(raw[:8].view(np.int64)[0])
+    p = raw[8:8 + n * 56].view(np.float64).reshape(n, 7)
+    nsc = int(np.prod([a // b for a, b in zip(ncell, ncps)]))
+    off = raw[8 + n * 56:8 + n * 56 + (nsc + 1) * 8].view(np.int64)
+    fld = raw[8 + n * 56 + (nsc + 1) * 8:].view(np.float64).reshape((9,) + tuple(c + 4 for c in ncell))
+    og = O.Grid.periodic(ncell)
+    oe = O.Ensemble(ncell, ncps, (1.0, 1.0, 1.0), x, u, w)
+    for _ in range(5):
+        O.step(og, oe, dt, O.TSC, O.ESIRKEPOV)
+    assert np.array_equal(off, oe.off)
+    assert rel_err(p[:, :3], oe.positions()) < 1e-12
+    assert rel_err(p[:, 3:6], oe.momenta()) < 1e-12
+    for c in range(9):
+        assert rel_err(fld[c], og.f[c]) < 1e-12
