@@ -37,7 +37,22 @@ def alg_bytes(prec, ppc):
     return 60.0 + 36.0 / ppc          # read cell,off,u,w 32 B + write cell,off,u 28 B; FP32 grid
 
 
+def measured_traffic_per_particle(prec):
+    """HBM bytes per particle of one particle-kernel launch, from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs summarised in profiles/ (calibrated against a plain copy); None if absent"""
+    path = os.path.join(ROOT, "profiles", "particles_tsc_esirkepov_256cube.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_particle_%s:" % prec):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def run_gpu(args, prec):
+    """load the workload with the seeded device loader, run warmup + timed steps, return rates,
+    per-kernel event times and the final energies"""
     import pica_amd as P
     ctx = P.Context(0)
     n = args.n
@@ -53,6 +68,7 @@ def run_gpu(args, prec):
     npart = ens.size()
     shape = P.TSC if args.shape == "tsc" else P.CIC
     dep = P.ESIRKEPOV if args.dep == "esirkepov" else P.DIRECT
+    d0 = P.diag(ens, grid, shape)
     for _ in range(args.warmup):
         P.step(ens, grid, dt, shape, dep)
     ctx.sync()
@@ -61,7 +77,8 @@ def run_gpu(args, prec):
     t0 = time.perf_counter()
     for i in range(args.steps):
         P.step(ens, grid, dt, shape, dep)
-        print("step %d/%d done" % (i + 1, args.steps), file=sys.stderr, flush=True) if (i + 1) % 20 == 0 else None
+        if (i + 1) % 20 == 0:
+            print("%s step %d/%d done" % (prec, i + 1, args.steps), file=sys.stderr, flush=True)
     ctx.sync()
     t1 = time.perf_counter()
     ctx.timer_enable(False)
@@ -79,18 +96,53 @@ def run_gpu(args, prec):
         "particle_steps_per_s": npart * args.steps / (t1 - t0),
         "cell_updates_per_s": float(np.prod(ncell)) * args.steps / (t1 - t0),
         "kernels": kernels,
-        "energy": {"field": d[P.DIAG_FIELD_ENERGY], "kinetic": d[P.DIAG_KINETIC_ENERGY]},
+        "energy0": d0[P.DIAG_FIELD_ENERGY] + d0[P.DIAG_KINETIC_ENERGY],
+        "energy": d[P.DIAG_FIELD_ENERGY] + d[P.DIAG_KINETIC_ENERGY],
+        "field_energy": d[P.DIAG_FIELD_ENERGY],
+        "gauss_max_over_rho": d[P.DIAG_GAUSS_MAX] / max(d[P.DIAG_RHO_MAX], 1e-300),
     }
     if "particles" in kernels:
         mean_s = kernels["particles"]["ms_mean"] * 1e-3
         ach = alg_bytes(prec, args.ppc) * npart / mean_s
         out["particle_kernel_rate"] = npart / mean_s
+        tpp = measured_traffic_per_particle(prec) if (args.shape, args.dep, args.n, args.ppc, args.sc) == \
+            ("tsc", "esirkepov", 256, 16, 4) else None
         out["roofline"] = {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                           "frac": ach / HBM_PEAK, "traffic": None}
+                           "frac": ach / HBM_PEAK, "traffic": tpp * npart if tpp else None}
+    if "yee_e" in kernels and "yee_b" in kernels:
+        # FDTD cell updates per second of the B/2, E, B/2 kernels alone
+        fd = (kernels["yee_e"]["ms_mean"] + 2 * kernels["yee_b"]["ms_mean"]) * 1e-3
+        out["fdtd_cell_updates_per_s"] = float(np.prod(ncell)) / fd
     ens.close()
     grid.close()
     ctx.close()
     return out
+
+
+def cpu_reference(args):
+    """pica's own supercell CIC path (oracle/_ref: pica's headers compiled) on a bounded sample:
+    pica has no TSC or Esirkepov, so this is its CIC + direct deposition, FP64."""
+    import oracle as O
+    if O.ref() is None:
+        return None
+    rng = np.random.default_rng(8)
+    n = args.cpu_n
+    ppc = args.ppc
+    npart = n ** 3 * ppc
+    dt = 0.5 / O.C
+    x = rng.random((npart, 3)) * n
+    u = rng.standard_normal((npart, 3)) * 0.01
+    w = np.full(npart, 1e9)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    t0 = time.perf_counter()
+    O.ref_run_cic_direct((n, n, n), (1.0, 1.0, 1.0), (args.sc,) * 3, O.E_CHARGE, O.E_MASS, dt, args.cpu_steps, x, u,
+                         w, nthreads=threads)
+    t1 = time.perf_counter()
+    return {"value": npart * args.cpu_steps / (t1 - t0), "unit": "particle-steps/s", "cores": threads,
+            "kind": "reference",
+            "sample": "pica's own FieldInterpolatorCICSupercell + BorisPusher + CurrentDepositorCICSupercell + "
+                      "YeeSolver, %d^3 x %d ppc, %d steps incl. bucketing and re-sort glue, FP64 (pica has no "
+                      "TSC/Esirkepov)" % (n, ppc, args.cpu_steps)}
 
 
 def cpu_baseline(args):
@@ -137,7 +189,8 @@ def main():
     ap.add_argument("--sc", type=int, default=4)
     ap.add_argument("--shape", default="tsc", choices=["cic", "tsc"])
     ap.add_argument("--dep", default="esirkepov", choices=["direct", "esirkepov"])
-    ap.add_argument("--prec", default="f64", choices=["f64", "f32"])
+    ap.add_argument("--prec", default="both", choices=["f64", "f32", "both"],
+                    help="f32: the split-position FP32 form (the headline value); f64: the plain FP64 path")
     ap.add_argument("--cpu-n", type=int, default=96)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
@@ -147,8 +200,13 @@ def main():
         print("bench.py: this build runs on one GPU only (multi-GPU domain decomposition is design-only); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    res = run_gpu(args, args.prec)
-    print("gpu done: %.3g particle-steps/s" % res["particle_steps_per_s"], file=sys.stderr, flush=True)
+    head = "f64" if args.prec == "f64" else "f32"
+    res = run_gpu(args, head)
+    print("gpu %s done: %.3g particle-steps/s" % (head, res["particle_steps_per_s"]), file=sys.stderr, flush=True)
+    plain = None
+    if args.prec == "both":
+        plain = run_gpu(args, "f64")
+        print("gpu f64 done: %.3g particle-steps/s" % plain["particle_steps_per_s"], file=sys.stderr, flush=True)
     line = {
         "metric": METRIC,
         "value": res["particle_steps_per_s"],
@@ -160,21 +218,34 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.prec == "f64" else "f32",
+        "dtype": "f32" if head == "f32" else "f64",
         "data": "synthetic (seeded device loader: uniform positions, Maxwellian + two-stream momenta)",
         "config": {"workload": "3D periodic %d^3 Yee grid, %d ppc (%d particles), two-stream +-0.1c, %s, %s, "
                                "%d^3 supercells re-sorted every step, %s; whole PIC step timed"
                                % (args.n, args.ppc, res["particles"], args.shape.upper(), args.dep, args.sc,
-                                  args.prec.upper()),
+                                  "FP32 split positions (int32 cell + FP32 offset)" if head == "f32" else "FP64"),
                    "grid": [args.n] * 3, "ppc": args.ppc, "particles": res["particles"]},
         "roofline": res.get("roofline"),
         "particle_kernel_rate": res.get("particle_kernel_rate"),
-        "cell_updates_per_s": res["cell_updates_per_s"],
+        "fdtd_cell_updates_per_s": res.get("fdtd_cell_updates_per_s"),
         "kernels_ms_mean": {k: v["ms_mean"] for k, v in res["kernels"].items()},
     }
+    if plain is not None:
+        line["plain_fp64"] = {"value": plain["particle_steps_per_s"], "ms_per_step": plain["ms_per_step"],
+                              "particle_kernel_rate": plain.get("particle_kernel_rate"),
+                              "roofline": plain.get("roofline"),
+                              "fdtd_cell_updates_per_s": plain.get("fdtd_cell_updates_per_s"),
+                              "kernels_ms_mean": {k: v["ms_mean"] for k, v in plain["kernels"].items()}}
+        line["fp32_speedup_vs_fp64"] = res["particle_steps_per_s"] / plain["particle_steps_per_s"]
+        # identical seeded inputs, same number of steps: total-energy difference of the FP32 form
+        line["fp32_energy_rel_err_vs_fp64"] = abs(res["energy"] - plain["energy"]) / abs(plain["energy"])
+        line["energy_drift_fp64"] = abs(plain["energy"] - plain["energy0"]) / abs(plain["energy0"])
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args)
         line["gpu_vs_cpu"] = line["value"] / line["cpu_baseline"]["value"]
+        ref = cpu_reference(argparse.Namespace(cpu_n=64, ppc=args.ppc, sc=args.sc, cpu_steps=2))
+        if ref:
+            line["cpu_baseline_reference"] = ref
     print(json.dumps(line), flush=True)
 
 
