@@ -55,6 +55,8 @@ def run_gpu(args, prec):
     per-kernel event times and the final energies"""
     import pica_amd as P
     ctx = P.Context(0)
+    ctx.set_option(P.OPT_PARTICLE_KERNEL, {"auto": P.KERNEL_AUTO, "per_particle": P.KERNEL_PER_PARTICLE,
+                                           "cell_sorted": P.KERNEL_CELL_SORTED}[args.kernel])
     n = args.n
     ncell = (n, n, n)
     ncps = (args.sc, args.sc, args.sc)
@@ -195,6 +197,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-events", action="store_true")
+    ap.add_argument("--kernel", default="auto", choices=["auto", "per_particle", "cell_sorted"],
+                    help="FP32 Esirkepov particle kernel (auto: cell-sorted when it fits)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (multi-GPU domain decomposition is design-only); "
@@ -224,7 +228,8 @@ def main():
                                "%d^3 supercells re-sorted every step, %s; whole PIC step timed"
                                % (args.n, args.ppc, res["particles"], args.shape.upper(), args.dep, args.sc,
                                   "FP32 split positions (int32 cell + FP32 offset)" if head == "f32" else "FP64"),
-                   "grid": [args.n] * 3, "ppc": args.ppc, "particles": res["particles"]},
+                   "grid": [args.n] * 3, "ppc": args.ppc, "particles": res["particles"],
+                   "particle_kernel": args.kernel},
         "roofline": res.get("roofline"),
         "particle_kernel_rate": res.get("particle_kernel_rate"),
         "fdtd_cell_updates_per_s": res.get("fdtd_cell_updates_per_s"),

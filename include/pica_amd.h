@@ -71,6 +71,17 @@ void* pk_ctx_stream(pk_ctx* ctx);
 int pk_timer_enable(pk_ctx* ctx, int on);
 int pk_timer_get(pk_ctx* ctx, const char* kernel, double* total_ms, long* launches);
 int pk_timer_reset(pk_ctx* ctx);
+/* tuning options of a context (no counterpart in pica, whose loops are the caller's):
+ * PK_OPT_PARTICLE_KERNEL selects the FP32 Esirkepov particle kernel: PK_KERNEL_AUTO (the
+ * per-particle kernel), PK_KERNEL_PER_PARTICLE (one lane per particle, every contribution an LDS
+ * atomic) or PK_KERNEL_CELL_SORTED (particles sorted by cell in LDS, per-cell register sums;
+ * 2^3 or 4^3 supercells).
+ * PK_OPT_CHANGE_QUEUE caps the cell-sorted kernel's LDS queue of cell-changing particles
+ * (0 = as many entries as the LDS budget allows; beyond the queue a lane deposits at once). */
+enum { PK_OPT_PARTICLE_KERNEL = 0, PK_OPT_CHANGE_QUEUE = 1, PK_OPT_N };
+enum { PK_KERNEL_AUTO = 0, PK_KERNEL_PER_PARTICLE = 1, PK_KERNEL_CELL_SORTED = 2 };
+int pk_ctx_set_option(pk_ctx* ctx, int option, int value);
+int pk_ctx_get_option(pk_ctx* ctx, int option, int* value);
 
 /* ParticleTypes table (Particle.h:21-31): species masses and charges (CGS). */
 int pk_species_set(pk_ctx* ctx, int ntypes, const double* mass, const double* charge);

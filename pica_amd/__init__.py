@@ -32,6 +32,8 @@ H2D, D2H = 0, 1
 FILL_E, FILL_B, FILL_J, FOLD_J, FOLD_RHO = 1, 2, 4, 8, 16
 DIAG_FIELD_ENERGY, DIAG_KINETIC_ENERGY, DIAG_CHARGE, DIAG_GAUSS_MAX, DIAG_GAUSS_DRIFT, DIAG_RHO_MAX = range(6)
 DIAG_N = 6
+OPT_PARTICLE_KERNEL, OPT_CHANGE_QUEUE = 0, 1
+KERNEL_AUTO, KERNEL_PER_PARTICLE, KERNEL_CELL_SORTED = 0, 1, 2
 
 # pica's CGS constants (Constants.h:10-13)
 PI = 3.14159265358
@@ -42,7 +44,8 @@ ELECTRON_MASS = 9.10938215e-28
 # every symbol include/pica_amd.h declares
 ABI_SYMBOLS = (
     "pk_ctx_create", "pk_ctx_destroy", "pk_last_error", "pk_sync", "pk_ctx_stream", "pk_timer_enable",
-    "pk_timer_get", "pk_timer_reset", "pk_species_set", "pk_grid_create", "pk_grid_destroy", "pk_grid_copy",
+    "pk_timer_get", "pk_timer_reset", "pk_ctx_set_option", "pk_ctx_get_option", "pk_species_set",
+    "pk_grid_create", "pk_grid_destroy", "pk_grid_copy",
     "pk_grid_zero", "pk_yee_update_e", "pk_yee_update_b", "pk_ghost", "pk_ens_create", "pk_ens_destroy",
     "pk_ens_set_slack", "pk_ens_upload", "pk_ens_download", "pk_ens_size", "pk_ens_num_supercells",
     "pk_ens_cells", "pk_ens_load_thermal", "pk_particle_step", "pk_ens_resort", "pk_step", "pk_run",
@@ -91,6 +94,8 @@ def lib():
         for name in ("pk_sync", "pk_timer_reset"):
             getattr(L, name).argtypes = [vp]
         L.pk_timer_enable.argtypes = [vp, ctypes.c_int]
+        L.pk_ctx_set_option.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.pk_ctx_get_option.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         L.pk_timer_get.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long)]
         L.pk_yee_update_e.argtypes = [vp, ctypes.c_double]
         L.pk_yee_update_b.argtypes = [vp, ctypes.c_double]
@@ -162,6 +167,15 @@ class Context:
 
     def timer_reset(self):
         self.check(lib().pk_timer_reset(self.h))
+
+    def set_option(self, option, value):
+        """pk_ctx_set_option, e.g. (OPT_PARTICLE_KERNEL, KERNEL_CELL_SORTED)"""
+        self.check(lib().pk_ctx_set_option(self.h, int(option), int(value)))
+
+    def get_option(self, option):
+        v = ctypes.c_int()
+        self.check(lib().pk_ctx_get_option(self.h, int(option), ctypes.byref(v)))
+        return v.value
 
     def timer(self, kernel):
         ms, n = ctypes.c_double(), ctypes.c_long()

@@ -132,6 +132,31 @@ int pk_timer_enable(pk_ctx* c, int on)
     return PK_OK;
 }
 
+int pk_ctx_set_option(pk_ctx* c, int option, int value)
+{
+    if (!c) return PK_E_ARG;
+    if (option == PK_OPT_PARTICLE_KERNEL) {
+        if (value < PK_KERNEL_AUTO || value > PK_KERNEL_CELL_SORTED)
+            return pk_fail(c, PK_E_ARG, "pk_ctx_set_option: unknown particle kernel");
+        c->opt[option] = value;
+        return PK_OK;
+    }
+    if (option == PK_OPT_CHANGE_QUEUE) {
+        if (value < 0) return pk_fail(c, PK_E_ARG, "pk_ctx_set_option: negative queue size");
+        c->opt[option] = value;
+        return PK_OK;
+    }
+    return pk_fail(c, PK_E_ARG, "pk_ctx_set_option: unknown option");
+}
+
+int pk_ctx_get_option(pk_ctx* c, int option, int* value)
+{
+    if (!c || !value) return PK_E_ARG;
+    if (option < 0 || option >= PK_OPT_N) return pk_fail(c, PK_E_ARG, "pk_ctx_get_option: unknown option");
+    *value = c->opt[option];
+    return PK_OK;
+}
+
 static int timer_collect(pk_ctx* c)
 {
     if (c->pend.empty()) return PK_OK;

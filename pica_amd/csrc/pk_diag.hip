@@ -36,7 +36,8 @@ __global__ void __launch_bounds__(256) k_rho(EnsGeom eg, PartF64 pf, PartSplit p
         double xi[3], w;
         if (SPLIT) {
             const int c = ps.cell[p];
-            const int cc[3] = {c / (eg.ncell[1] * eg.ncell[2]), (c / eg.ncell[2]) % eg.ncell[1], c % eg.ncell[2]};
+            int cc[3];
+            cell_coords(c, eg, cc);
             const float o[3] = {ps.ox[p], ps.oy[p], ps.oz[p]};
             for (int d = 0; d < 3; d++) xi[d] = (double)cc[d] + (double)o[d] + 1.5;   // node a at (a - 1.5) h
             w = ps.w[p];

@@ -364,10 +364,11 @@ __global__ void __launch_bounds__(256) k_compact_split(EnsGeom eg, PartSplit pm,
     for (int l = threadIdx.x; l < n; l += blockDim.x) {
         const long p = off[s] + l, o = runstart[s] + l;
         const int c = pm.cell[p];
-        const int c0 = c / (eg.ncell[1] * eg.ncell[2]), c1 = (c / eg.ncell[2]) % eg.ncell[1], c2 = c % eg.ncell[2];
-        out.f[0][o] = ((double)c0 + (double)pm.ox[p]) * eg.step[0];
-        out.f[1][o] = ((double)c1 + (double)pm.oy[p]) * eg.step[1];
-        out.f[2][o] = ((double)c2 + (double)pm.oz[p]) * eg.step[2];
+        int cc[3];
+        cell_coords(c, eg, cc);
+        out.f[0][o] = ((double)cc[0] + (double)pm.ox[p]) * eg.step[0];
+        out.f[1][o] = ((double)cc[1] + (double)pm.oy[p]) * eg.step[1];
+        out.f[2][o] = ((double)cc[2] + (double)pm.oz[p]) * eg.step[2];
         out.f[3][o] = pm.ux[p]; out.f[4][o] = pm.uy[p]; out.f[5][o] = pm.uz[p];
         out.f[6][o] = pm.w[p];
     }
@@ -452,6 +453,8 @@ EnsGeom ens_geom(const pk_ens* e)
     }
     g.nsct = e->nsct;
     g.xcd_chunk = (int)((e->nsct + PK_XCD - 1) / PK_XCD);
+    g.div12 = make_fastdiv((uint32_t)(e->ncell[1] * e->ncell[2]));
+    g.div2 = make_fastdiv((uint32_t)e->ncell[2]);
     return g;
 }
 

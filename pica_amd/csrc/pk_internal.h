@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/pica_amd.h"
+#include "pk_fastdiv.h"
 
 // pica's constants (Constants.h:10-13), pi deliberately the truncated literal pica uses.
 #define PK_PI 3.14159265358
@@ -38,6 +39,7 @@ struct pk_ctx {
     std::vector<hipEvent_t> evpool;
     std::map<std::string, pk_timer_rec> timers;
     int32_t* flags_host = nullptr;     // pinned, PK_NFLAGS
+    int opt[PK_OPT_N] = {PK_KERNEL_AUTO, 0};
 };
 
 enum { PK_FLAG_ERR = 0, PK_FLAG_OVERFLOW = 1, PK_NFLAGS = 4 };
@@ -160,7 +162,17 @@ struct EnsGeom {
     long nsct;
     double step[3];
     int xcd_chunk;   // ceil(nsct / PK_XCD)
+    FastDiv div12, div2;   // by ncell[1] * ncell[2] and by ncell[2]
 };
+
+// 3D cell of a linear physical cell index 0 <= c < 2^31 (z fastest)
+__device__ __forceinline__ void cell_coords(int c, const EnsGeom& eg, int cc[3])
+{
+    const int q12 = (int)fastdiv((uint32_t)c, eg.div12), q2 = (int)fastdiv((uint32_t)c, eg.div2);
+    cc[0] = q12;
+    cc[1] = q2 - q12 * eg.ncell[1];
+    cc[2] = c - q2 * eg.ncell[2];
+}
 EnsGeom ens_geom(const pk_ens* e);
 
 // XCD-aware supercell of a workgroup: the workgroups one XCD receives (b % 8 == x) get one

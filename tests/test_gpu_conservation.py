@@ -45,12 +45,17 @@ def _continuity(ctx, prec, n, ppc, shape, dep):
     return float(np.abs(res).max() / scale)
 
 
-@pytest.mark.parametrize("prec,tol", [(P.F64, 1e-11), (P.F32, 2e-4)])
+@pytest.mark.parametrize("prec,tol,kernel", [(P.F64, 1e-11, P.KERNEL_AUTO), (P.F32, 2e-4, P.KERNEL_PER_PARTICLE),
+                                             (P.F32, 2e-4, P.KERNEL_CELL_SORTED)])
 @pytest.mark.parametrize("shape", [P.CIC, P.TSC])
-def test_esirkepov_continuity(ctx, prec, tol, shape):
-    _say("continuity prec=%d shape=%d" % (prec, shape))
-    r = _continuity(ctx, prec, 32, 8, shape, P.ESIRKEPOV)
-    print("continuity residual (prec=%d shape=%d): %.3g" % (prec, shape, r), flush=True)
+def test_esirkepov_continuity(ctx, prec, tol, kernel, shape):
+    _say("continuity prec=%d shape=%d kernel=%d" % (prec, shape, kernel))
+    ctx.set_option(P.OPT_PARTICLE_KERNEL, kernel)
+    try:
+        r = _continuity(ctx, prec, 32, 8, shape, P.ESIRKEPOV)
+    finally:
+        ctx.set_option(P.OPT_PARTICLE_KERNEL, P.KERNEL_AUTO)
+    print("continuity residual (prec=%d shape=%d kernel=%d): %.3g" % (prec, shape, kernel, r), flush=True)
     assert r < tol
 
 

@@ -4,6 +4,8 @@ The two runs start from identical seeded inputs.  Particles are matched through 
 weights (w_k = w0 (1 + k 2^-20), exact in FP32).  Tolerances are stated per quantity; they are
 FP32 round-off bounds, not parity in the bit sense.
 """
+import contextlib
+
 import numpy as np
 import pytest
 
@@ -35,12 +37,20 @@ def _inputs(seed, ncell, ppc, uth=0.05, drift=0.1):
     return dt, x, u, w
 
 
-@pytest.mark.parametrize("shape,dep", [(P.CIC, P.DIRECT), (P.CIC, P.ESIRKEPOV), (P.TSC, P.DIRECT),
-                                       (P.TSC, P.ESIRKEPOV)])
-def test_split_matches_fp64_oracle(ctx, shape, dep):
-    _say("split vs fp64 shape=%d dep=%d" % (shape, dep))
-    ncell, ncps, ppc, nsteps = (16, 16, 16), (4, 4, 4), 8, 10
-    dt, x, u, w = _inputs(41 + 2 * shape + dep, ncell, ppc)
+@contextlib.contextmanager
+def _kernel(ctx, kernel, queue=0):
+    """run with the given FP32 Esirkepov kernel and change-queue cap, restoring the defaults"""
+    ctx.set_option(P.OPT_PARTICLE_KERNEL, kernel)
+    ctx.set_option(P.OPT_CHANGE_QUEUE, queue)
+    try:
+        yield
+    finally:
+        ctx.set_option(P.OPT_PARTICLE_KERNEL, P.KERNEL_AUTO)
+        ctx.set_option(P.OPT_CHANGE_QUEUE, 0)
+
+
+def _compare(ctx, shape, dep, ncell, ncps, ppc, nsteps, seed, uth=0.05, drift=0.1, kernel=P.KERNEL_AUTO, queue=0):
+    dt, x, u, w = _inputs(seed, ncell, ppc, uth, drift)
     og = O.Grid.periodic(ncell)
     oe = O.Ensemble(ncell, ncps, (1.0, 1.0, 1.0), x, u, w)
     for _ in range(nsteps):
@@ -49,7 +59,8 @@ def test_split_matches_fp64_oracle(ctx, shape, dep):
     g = P.YeeGrid.periodic(ctx, ncell, prec=P.F32)
     ens = P.EnsembleSupercells(ctx, ncell, (1.0, 1.0, 1.0), ncps, prec=P.F32)
     ens.add(x, u, w)
-    P.run(ens, g, dt, nsteps, shape, dep)
+    with _kernel(ctx, kernel, queue):
+        P.run(ens, g, dt, nsteps, shape, dep)
     gx, gu, gw, goff = ens.particles()
     assert gx.shape[0] == oe.np
     d = P.diag(ens, g, shape)
@@ -67,9 +78,43 @@ def test_split_matches_fp64_oracle(ctx, shape, dep):
         "J": max(rel_err(g.get(c), og.f[c]) for c in (P.JX, P.JY, P.JZ)),
         "energy": abs(e1 - e0) / abs(e0),
     }
-    print("split errors vs FP64 oracle (shape=%d dep=%d): %s" % (shape, dep, errs), flush=True)
+    print("split errors vs FP64 oracle (shape=%d dep=%d kernel=%d): %s" % (shape, dep, kernel, errs), flush=True)
     for k, v in errs.items():
         assert v < TOL[k], "%s error %.3g above %.3g" % (k, v, TOL[k])
+
+
+@pytest.mark.parametrize("shape,dep,kernel", [(P.CIC, P.DIRECT, P.KERNEL_AUTO),
+                                              (P.CIC, P.ESIRKEPOV, P.KERNEL_PER_PARTICLE),
+                                              (P.CIC, P.ESIRKEPOV, P.KERNEL_CELL_SORTED),
+                                              (P.TSC, P.DIRECT, P.KERNEL_AUTO),
+                                              (P.TSC, P.ESIRKEPOV, P.KERNEL_PER_PARTICLE),
+                                              (P.TSC, P.ESIRKEPOV, P.KERNEL_CELL_SORTED)])
+def test_split_matches_fp64_oracle(ctx, shape, dep, kernel):
+    _say("split vs fp64 shape=%d dep=%d kernel=%d" % (shape, dep, kernel))
+    _compare(ctx, shape, dep, (16, 16, 16), (4, 4, 4), 8, 10, 41 + 2 * shape + dep, kernel=kernel)
+
+
+@pytest.mark.parametrize("kernel", [P.KERNEL_PER_PARTICLE, P.KERNEL_CELL_SORTED])
+def test_split_one_and_two_supercells_per_dimension(ctx, kernel):
+    """8 x 4 x 12 cells in 4^3 supercells: 2, 1 and 3 supercells per dimension, so leavers wrap
+    onto the neighbour from both sides (2) or back onto their own supercell (1)"""
+    _say("split nsc=(2,1,3) kernel=%d" % kernel)
+    _compare(ctx, P.TSC, P.ESIRKEPOV, (8, 4, 12), (4, 4, 4), 8, 10, 5, uth=0.1, kernel=kernel)
+
+
+@pytest.mark.parametrize("shape", [P.CIC, P.TSC])
+def test_split_sorted_small_supercells(ctx, shape):
+    """the cell-sorted kernel on 2^3 supercells"""
+    _say("split sorted 2^3 shape=%d" % shape)
+    _compare(ctx, shape, P.ESIRKEPOV, (8, 8, 6), (2, 2, 2), 12, 10, 9, kernel=P.KERNEL_CELL_SORTED)
+
+
+def test_split_sorted_queue_overflow(ctx):
+    """a hot plasma with the change queue capped at 8 entries: most cell changers deposit
+    straight from the lane (largest window) instead of from the queue"""
+    _say("split sorted queue cap 8")
+    _compare(ctx, P.TSC, P.ESIRKEPOV, (12, 8, 8), (4, 4, 4), 10, 10, 13, uth=0.2, drift=0.0,
+             kernel=P.KERNEL_CELL_SORTED, queue=8)
 
 
 def test_split_cells_and_counts(ctx):
