@@ -13,6 +13,10 @@ roofline: the fused particle kernel, algorithmic bytes per particle-step (FP64:
 104 + 72/ppc) x particles / its mean launch time from HIP events on the library's stream.
 cpu_baseline: the CPU oracle (OpenMP restatement of pica's path) on a bounded sample of the
 same workload, on this host's cores.
+batched: 64 independent periodic plasmas with edges drawn from {32,48,64,96,128} and ppc from
+{4,8,16,32} (4.8e8 particles, FP32 split positions, TSC, Esirkepov), and 512 small ones (edges
+{16,24,32}, ppc {4,8}), each stepped one by one (pk_step per plasma) and as one batch
+(pk_batch_run after pk_batch_autotune).
 """
 import argparse
 import json
@@ -121,6 +125,64 @@ def run_gpu(args, prec):
     return out
 
 
+def batch_specs(n=64, seed=3, edges=(32, 48, 64, 96, 128), ppcs=(4, 8, 16, 32)):
+    """a batched workload: (edge, ppc, seed) of each plasma"""
+    rng = np.random.default_rng(seed)
+    e = rng.choice(list(edges), n)
+    p = rng.choice(list(ppcs), n)
+    return [(int(a), int(b), 1000 + i) for i, (a, b) in enumerate(zip(e, p))]
+
+
+def run_batched(args, specs):
+    """one-by-one vs batched stepping of a batched workload; aggregate particle-steps/s"""
+    import pica_amd as P
+    ctx = P.Context(0)
+    dt = 0.5 / P.C
+    shape = P.TSC if args.shape == "tsc" else P.CIC
+    dep = P.ESIRKEPOV if args.dep == "esirkepov" else P.DIRECT
+    pairs = []
+    for edge, ppc, seed in specs:
+        n0, w0 = P.plasma_parameters(dt, ppc)
+        g = P.YeeGrid.periodic(ctx, (edge,) * 3, prec=P.F32)
+        e = P.EnsembleSupercells(ctx, (edge,) * 3, (1.0, 1.0, 1.0), (args.sc,) * 3, prec=P.F32)
+        e.load_thermal(ppc, 0.01, 0.1 / math.sqrt(0.99), w0, seed)
+        pairs.append((e, g))
+    npart = sum(e.size() for e, _ in pairs)
+    out = {"instances": len(pairs), "particles": npart, "steps": args.batch_steps}
+    for e, g in pairs:
+        P.step(e, g, dt, shape, dep)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.batch_steps):
+        for e, g in pairs:
+            P.step(e, g, dt, shape, dep)
+    ctx.sync()
+    t1 = time.perf_counter()
+    out["one_by_one"] = {"value": npart * args.batch_steps / (t1 - t0), "ms_per_step": 1e3 * (t1 - t0) / args.batch_steps}
+    print("batched workload one by one: %.3g particle-steps/s" % out["one_by_one"]["value"], file=sys.stderr, flush=True)
+    b = P.Batch(ctx, pairs)
+    tuned_ms = b.autotune(dt, shape, dep, trial_steps=2)
+    knobs = {k: b.get(v) for k, v in (("wg_per_cu", P.BATCH_WG_PER_CU), ("order", P.BATCH_ORDER),
+                                         ("graph", P.BATCH_GRAPH))}
+    b.run(dt, 1, shape, dep)
+    ctx.sync()
+    t0 = time.perf_counter()
+    b.run(dt, args.batch_steps, shape, dep)
+    ctx.sync()
+    t1 = time.perf_counter()
+    out["batched"] = {"value": npart * args.batch_steps / (t1 - t0), "ms_per_step": 1e3 * (t1 - t0) / args.batch_steps,
+                      "autotuned_knobs": knobs, "autotune_trial_ms_per_step": tuned_ms}
+    out["speedup"] = out["batched"]["value"] / out["one_by_one"]["value"]
+    print("batched workload as one batch: %.3g particle-steps/s" % out["batched"]["value"], file=sys.stderr,
+          flush=True)
+    b.close()
+    for e, g in pairs:
+        e.close()
+        g.close()
+    ctx.close()
+    return out
+
+
 def cpu_reference(args):
     """pica's own supercell CIC path (oracle/_ref: pica's headers compiled) on a bounded sample:
     pica has no TSC or Esirkepov, so this is its CIC + direct deposition, FP64."""
@@ -197,6 +259,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-events", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 64-plasma batched workload")
+    ap.add_argument("--batch-n", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--kernel", default="auto", choices=["auto", "per_particle", "cell_sorted"],
                     help="FP32 Esirkepov particle kernel (auto: cell-sorted when it fits)")
     args = ap.parse_args()
@@ -245,6 +310,10 @@ def main():
         # identical seeded inputs, same number of steps: total-energy difference of the FP32 form
         line["fp32_energy_rel_err_vs_fp64"] = abs(res["energy"] - plain["energy"]) / abs(plain["energy"])
         line["energy_drift_fp64"] = abs(plain["energy"] - plain["energy0"]) / abs(plain["energy0"])
+    if not args.no_batched:
+        line["batched_workload"] = run_batched(args, batch_specs(args.batch_n))
+        # many small plasmas, where one launch per stage for all of them matters
+        line["batched_small_workload"] = run_batched(args, batch_specs(512, 5, (16, 24, 32), (4, 8)))
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args)
         line["gpu_vs_cpu"] = line["value"] / line["cpu_baseline"]["value"]

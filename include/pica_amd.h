@@ -128,6 +128,29 @@ int pk_ens_cells(pk_ens* e, int32_t* cell_counts, int32_t* cell_of_particle);
  * u ~ N(0, uth) per component, plus +drift_x on even and -drift_x on odd particle ids. */
 int pk_ens_load_thermal(pk_ens* e, int ppc, double uth, double drift_x, double weight, uint64_t seed);
 
+/* ---- many ensembles stepped together (no counterpart in pica, whose loops are the caller's) ----
+ * A batch steps n (ensemble, grid) pairs of one precision and one supercell shape, with uneven
+ * grid sizes and particle counts, doing each stage of pk_step for all of them in one launch
+ * (particles and re-sort over a list of (ensemble, supercell) items), optionally captured once
+ * into a HIP graph and replayed every step.  The ensembles and grids stay owned by the caller
+ * and must outlive the batch; they can still be used one by one between batch calls. */
+typedef struct pk_batch pk_batch;
+enum {
+    PK_BATCH_WG_PER_CU = 0,   /* particle kernel: 0 = one workgroup per item, k = persistent, k per CU */
+    PK_BATCH_ORDER = 1,       /* items: 0 = ensemble by ensemble, 1 = most particles first */
+    PK_BATCH_GRAPH = 2,       /* 1 = replay a captured HIP graph per step, 0 = launch eagerly */
+    PK_BATCH_NKNOB
+};
+int pk_batch_create(pk_ctx* ctx, int n, pk_ens* const* ens, pk_grid* const* grids, pk_batch** out);
+void pk_batch_destroy(pk_batch* b);
+int pk_batch_set(pk_batch* b, int knob, int value);
+int pk_batch_get(pk_batch* b, int knob, int* value);
+/* nsteps periodic PIC steps (pk_step's sequence) of every ensemble of the batch */
+int pk_batch_run(pk_batch* b, double dt, int shape, int dep, int nsteps);
+/* autotuner: times trial_steps steps of every knob setting from the current state, restoring the
+ * state after each trial, and keeps the fastest setting (its ms per step in *best_ms) */
+int pk_batch_autotune(pk_batch* b, double dt, int shape, int dep, int trial_steps, double* best_ms);
+
 /* ---- the step ----
  * Fused interpolation + Boris push + deposition for every particle (one workgroup per
  * supercell: FieldInterpolatorCICSupercell, BorisPusher::push, CurrentDepositorCICSupercell,

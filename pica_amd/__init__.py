@@ -33,6 +33,7 @@ FILL_E, FILL_B, FILL_J, FOLD_J, FOLD_RHO = 1, 2, 4, 8, 16
 DIAG_FIELD_ENERGY, DIAG_KINETIC_ENERGY, DIAG_CHARGE, DIAG_GAUSS_MAX, DIAG_GAUSS_DRIFT, DIAG_RHO_MAX = range(6)
 DIAG_N = 6
 OPT_PARTICLE_KERNEL, OPT_CHANGE_QUEUE = 0, 1
+BATCH_WG_PER_CU, BATCH_ORDER, BATCH_GRAPH = 0, 1, 2
 KERNEL_AUTO, KERNEL_PER_PARTICLE, KERNEL_CELL_SORTED = 0, 1, 2
 
 # pica's CGS constants (Constants.h:10-13)
@@ -49,7 +50,8 @@ ABI_SYMBOLS = (
     "pk_grid_zero", "pk_yee_update_e", "pk_yee_update_b", "pk_ghost", "pk_ens_create", "pk_ens_destroy",
     "pk_ens_set_slack", "pk_ens_upload", "pk_ens_download", "pk_ens_size", "pk_ens_num_supercells",
     "pk_ens_cells", "pk_ens_load_thermal", "pk_particle_step", "pk_ens_resort", "pk_step", "pk_run",
-    "pk_deposit_rho", "pk_diag", "pk_diag_set_reference",
+    "pk_deposit_rho", "pk_diag", "pk_diag_set_reference", "pk_batch_create", "pk_batch_destroy", "pk_batch_set",
+    "pk_batch_get", "pk_batch_run", "pk_batch_autotune",
 )
 
 _LIB = None
@@ -115,6 +117,13 @@ def lib():
         L.pk_deposit_rho.argtypes = [vp, vp, ctypes.c_int]
         L.pk_diag.argtypes = [vp, vp, ctypes.c_int, vp]
         L.pk_diag_set_reference.argtypes = [vp, vp, ctypes.c_int]
+        L.pk_batch_create.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.POINTER(vp)]
+        L.pk_batch_destroy.argtypes = [vp]
+        L.pk_batch_set.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.pk_batch_get.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.pk_batch_run.argtypes = [vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.pk_batch_autotune.argtypes = [vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_double)]
         _LIB = L
     return _LIB
 
@@ -329,6 +338,53 @@ class EnsembleSupercells:
         if getattr(self, "h", None):
             if self.ctx.h:
                 lib().pk_ens_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many (ensemble, grid) pairs of one precision and supercell shape stepped together
+    (pk_batch_*): one launch per stage for all of them, optionally a captured HIP graph per step.
+    The ensembles and grids must outlive the batch."""
+
+    def __init__(self, ctx, pairs):
+        self.ctx = ctx
+        self.pairs = list(pairs)
+        n = len(self.pairs)
+        ens = (ctypes.c_void_p * n)(*[e.h.value for e, _ in self.pairs])
+        grids = (ctypes.c_void_p * n)(*[g.h.value for _, g in self.pairs])
+        h = ctypes.c_void_p()
+        ctx.check(lib().pk_batch_create(ctx.h, n, ens, grids, ctypes.byref(h)))
+        self.h = h
+        ctx._adopt(self)
+
+    def set(self, knob, value):
+        self.ctx.check(lib().pk_batch_set(self.h, int(knob), int(value)))
+
+    def get(self, knob):
+        v = ctypes.c_int()
+        self.ctx.check(lib().pk_batch_get(self.h, int(knob), ctypes.byref(v)))
+        return v.value
+
+    def run(self, dt, nsteps, shape=CIC, dep=DIRECT):
+        self.ctx.check(lib().pk_batch_run(self.h, float(dt), shape, dep, int(nsteps)))
+
+    def autotune(self, dt, shape=CIC, dep=DIRECT, trial_steps=3):
+        """times every knob setting from the current state (restored after each trial), keeps the
+        fastest; returns its ms per step"""
+        ms = ctypes.c_double()
+        self.ctx.check(lib().pk_batch_autotune(self.h, float(dt), shape, dep, int(trial_steps), ctypes.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            if self.ctx.h:
+                lib().pk_batch_destroy(self.h)
             self.h = None
 
     def __del__(self):

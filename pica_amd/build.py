@@ -21,6 +21,7 @@ SOURCES = [
     ("pk_particles_split.hip", ["-ffp-contract=fast"]),
     ("pk_ensemble.hip", ["-ffp-contract=off"]),
     ("pk_diag.hip", ["-ffp-contract=off"]),
+    ("pk_batch.hip", ["-ffp-contract=off"]),
 ]
 
 COMMON = [

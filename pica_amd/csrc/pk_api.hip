@@ -41,7 +41,9 @@ __global__ void k_fill_flags(int32_t* f, int n)
     if ((int)threadIdx.x < n) f[threadIdx.x] = 0;
 }
 
-int check_pair(pk_ens* e, pk_grid* g)
+}  // namespace
+
+int pk_check_pair(pk_ens* e, pk_grid* g)
 {
     if (!e || !g) return PK_E_ARG;
     if (e->ctx != g->ctx) return pk_fail(e->ctx, PK_E_ARG, "ensemble and grid belong to different contexts");
@@ -57,7 +59,7 @@ int check_pair(pk_ens* e, pk_grid* g)
     return PK_OK;
 }
 
-int flags_read(pk_ens* e, int32_t out[PK_NFLAGS])
+int pk_flags_read(pk_ens* e, int32_t out[PK_NFLAGS])
 {
     pk_ctx* c = e->ctx;
     PK_HIP(c, hipMemcpyAsync(c->flags_host, e->flags, sizeof(int32_t) * PK_NFLAGS, hipMemcpyDeviceToHost, c->stream));
@@ -66,14 +68,12 @@ int flags_read(pk_ens* e, int32_t out[PK_NFLAGS])
     return PK_OK;
 }
 
-int flags_clear(pk_ens* e)
+int pk_flags_clear(pk_ens* e)
 {
     hipLaunchKernelGGL(k_fill_flags, dim3(1), dim3(64), 0, e->ctx->stream, e->flags, PK_NFLAGS);
     PK_HIP(e->ctx, hipGetLastError());
     return PK_OK;
 }
-
-}  // namespace
 
 extern "C" {
 
@@ -380,14 +380,14 @@ int pk_ens_resort(pk_ens* e)
     int r = pk_ens_insert_launch(e);
     if (r) return r;
     int32_t fl[PK_NFLAGS];
-    r = flags_read(e, fl);
+    r = pk_flags_read(e, fl);
     if (r) return r;
     e->pending_movers = false;
     if (fl[PK_FLAG_OVERFLOW]) {
         r = pk_ens_rebuild(e, 0);
         if (r) return r;
     }
-    r = flags_clear(e);
+    r = pk_flags_clear(e);
     if (r) return r;
     if (fl[PK_FLAG_ERR])
         return pk_fail(e->ctx, PK_E_STATE,
@@ -432,7 +432,7 @@ int pk_ens_load_thermal(pk_ens* e, int ppc, double uth, double drift_x, double w
 // ---------------------------------------------------------------- step
 int pk_particle_step(pk_ens* e, pk_grid* g, double dt, int shape, int dep)
 {
-    int r = check_pair(e, g);
+    int r = pk_check_pair(e, g);
     if (r) return r;
     if ((shape != PK_CIC && shape != PK_TSC) || (dep != PK_DIRECT && dep != PK_ESIRKEPOV))
         return pk_fail(e->ctx, PK_E_ARG, "pk_particle_step: unknown shape or depositor");
@@ -449,7 +449,7 @@ int pk_particle_step(pk_ens* e, pk_grid* g, double dt, int shape, int dep)
 
 int pk_step(pk_ens* e, pk_grid* g, double dt, int shape, int dep)
 {
-    int r = check_pair(e, g);
+    int r = pk_check_pair(e, g);
     if (r) return r;
     for (int comp = PK_JX; comp <= PK_JZ; comp++)
         if ((r = pk_field_zero(g, comp))) return r;
@@ -477,7 +477,7 @@ int pk_run(pk_ens* e, pk_grid* g, double dt, int shape, int dep, int nsteps)
 // ---------------------------------------------------------------- diagnostics
 int pk_deposit_rho(pk_ens* e, pk_grid* g, int shape)
 {
-    int r = check_pair(e, g);
+    int r = pk_check_pair(e, g);
     if (r) return r;
     if ((r = pk_ens_resort(e))) return r;
     return pk_rho_impl(e, g, shape);
@@ -485,7 +485,7 @@ int pk_deposit_rho(pk_ens* e, pk_grid* g, int shape)
 
 int pk_diag(pk_ens* e, pk_grid* g, int shape, double* out)
 {
-    int r = check_pair(e, g);
+    int r = pk_check_pair(e, g);
     if (r) return r;
     if (!out) return PK_E_ARG;
     if ((r = pk_ens_resort(e))) return r;
@@ -494,7 +494,7 @@ int pk_diag(pk_ens* e, pk_grid* g, int shape, double* out)
 
 int pk_diag_set_reference(pk_ens* e, pk_grid* g, int shape)
 {
-    int r = check_pair(e, g);
+    int r = pk_check_pair(e, g);
     if (r) return r;
     if ((r = pk_ens_resort(e))) return r;
     double out[PK_DIAG_N];

@@ -1,0 +1,418 @@
+// pk_batch.hip -- many (ensemble, grid) pairs stepped together (no counterpart in pica).
+//
+// One launch per stage covers every ensemble of the batch: the particle and insert kernels walk a
+// list of (ensemble, supercell) items, the field kernels find their grid from a block prefix.  A
+// step is the single-ensemble pk_step sequence (zero J, particles, fold + fill J, B/2, E, B/2 with
+// ghost fills, insert), 14 launches for the whole batch instead of 14 per ensemble; with
+// PK_BATCH_GRAPH it is captured once into a HIP graph and replayed.
+//
+// Slack overflow without a host round trip: an insert that finds a supercell's slack exhausted
+// leaves that supercell as it is and raises PK_HALT_PENDING; the last kernel of the step turns
+// that into PK_HALT_STOP, which every kernel of the following steps checks first.  The
+// host, after the queued steps, rebuilds the overflowed ensembles (their stayers are still in
+// place and their leavers in the outbox), re-prepares and replays the remaining steps.
+#include <algorithm>
+
+#include "pk_internal.h"
+
+namespace {
+
+__global__ void k_batch_step_done(int* halt)
+{
+    if (threadIdx.x != 0 || halt[PK_HALT_STOP]) return;
+    if (halt[PK_HALT_PENDING]) halt[PK_HALT_STOP] = 1;
+    else halt[PK_HALT_DONE] += 1;
+}
+
+__global__ void k_batch_clear(int* halt)
+{
+    if (threadIdx.x < PK_HALT_N) halt[threadIdx.x] = 0;
+}
+
+constexpr unsigned MASK_E = (1u << PK_EX) | (1u << PK_EY) | (1u << PK_EZ);
+constexpr unsigned MASK_B = (1u << PK_BX) | (1u << PK_BY) | (1u << PK_BZ);
+constexpr unsigned MASK_J = (1u << PK_JX) | (1u << PK_JY) | (1u << PK_JZ);
+
+// (ensemble, supercell) items: ensemble by ensemble in the XCD-aware supercell order of the
+// single-ensemble kernels, or all items by decreasing particle count (largest first)
+int build_items(pk_batch* b)
+{
+    std::vector<int2> items;
+    std::vector<int32_t> key;
+    for (int i = 0; i < b->n(); i++) {
+        const pk_ens* e = b->ens[i];
+        std::vector<int32_t> cnt((size_t)e->nsct);
+        PK_HIP(b->ctx, hipMemcpy(cnt.data(), e->cnt, sizeof(int32_t) * e->nsct, hipMemcpyDeviceToHost));
+        const long chunk = (e->nsct + PK_XCD - 1) / PK_XCD;
+        for (long w = 0; w < PK_XCD * chunk; w++) {
+            const long s = (w % PK_XCD) * chunk + w / PK_XCD;
+            if (s >= e->nsct) continue;
+            items.push_back(make_int2(i, (int)s));
+            key.push_back(cnt[(size_t)s]);
+        }
+    }
+    if (b->knob[PK_BATCH_ORDER] == 1) {
+        std::vector<size_t> idx(items.size());
+        for (size_t k = 0; k < idx.size(); k++) idx[k] = k;
+        std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t c) { return key[a] > key[c]; });
+        std::vector<int2> sorted(items.size());
+        for (size_t k = 0; k < idx.size(); k++) sorted[k] = items[idx[k]];
+        items.swap(sorted);
+    }
+    b->nitems = (long)items.size();
+    return pk_batch_upload(b, (void**)&b->items, items.data(), sizeof(int2) * items.size());
+}
+
+// enqueue one step of every ensemble
+int enqueue_step(pk_batch* b)
+{
+    int r;
+    if ((r = pk_batch_fields_launch(b, PK_BOP_ZERO_J, 0, 0))) return r;
+    r = b->prec == PK_F64 ? pk_batch_f64_launch(b, b->shape, b->dep) : pk_batch_split_launch(b, b->shape, b->dep);
+    if (r) return r;
+    for (int d = 0; d < 3; d++)
+        if ((r = pk_batch_fields_launch(b, PK_BOP_FOLD, MASK_J, d))) return r;
+    if ((r = pk_batch_fields_launch(b, PK_BOP_FILL, MASK_J, 0))) return r;
+    if ((r = pk_batch_fields_launch(b, PK_BOP_YEE_B, 0, 0))) return r;
+    if ((r = pk_batch_fields_launch(b, PK_BOP_FILL, MASK_B, 0))) return r;
+    if ((r = pk_batch_fields_launch(b, PK_BOP_YEE_E, 0, 0))) return r;
+    if ((r = pk_batch_fields_launch(b, PK_BOP_FILL, MASK_E, 0))) return r;
+    if ((r = pk_batch_fields_launch(b, PK_BOP_YEE_B, 0, 0))) return r;
+    if ((r = pk_batch_fields_launch(b, PK_BOP_FILL, MASK_B, 0))) return r;
+    if ((r = pk_batch_insert_launch(b))) return r;
+    hipLaunchKernelGGL(k_batch_step_done, dim3(1), dim3(64), 0, b->ctx->stream, b->halt);
+    PK_HIP(b->ctx, hipGetLastError());
+    return PK_OK;
+}
+
+void drop_graph(pk_batch* b)
+{
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+}
+
+// work lists, per-ensemble arguments and (with PK_BATCH_GRAPH) the captured step
+int prepare(pk_batch* b, double dt, int shape, int dep)
+{
+    pk_ctx* c = b->ctx;
+    PK_HIP(c, hipStreamSynchronize(c->stream));
+    drop_graph(b);
+    int r;
+    if ((r = build_items(b))) return r;
+    if ((r = pk_batch_fields_prepare(b, dt))) return r;
+    r = b->prec == PK_F64 ? pk_batch_f64_prepare(b, dt, shape, dep) : pk_batch_split_prepare(b, dt, shape, dep);
+    if (r) return r;
+    if ((r = pk_batch_insert_prepare(b))) return r;
+    b->dt = dt;
+    b->shape = shape;
+    b->dep = dep;
+    if (b->knob[PK_BATCH_GRAPH]) {
+        const bool timing = c->timing;
+        c->timing = false;      // no event records inside the capture
+        PK_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        r = enqueue_step(b);
+        hipGraph_t graph = nullptr;
+        const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+        c->timing = timing;
+        if (r) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return r;
+        }
+        if (ec != hipSuccess) return pk_fail(c, PK_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+        const hipError_t ei = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) {
+            b->exec = nullptr;
+            return pk_fail(c, PK_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+        }
+    }
+    b->dirty = false;
+    return PK_OK;
+}
+
+int check_args(pk_batch* b, double dt, int shape, int dep)
+{
+    if ((shape != PK_CIC && shape != PK_TSC) || (dep != PK_DIRECT && dep != PK_ESIRKEPOV))
+        return pk_fail(b->ctx, PK_E_ARG, "pk_batch_run: unknown shape or depositor");
+    if (!(dt > 0.0)) return pk_fail(b->ctx, PK_E_ARG, "pk_batch_run: dt must be positive");
+    return PK_OK;
+}
+
+int run(pk_batch* b, double dt, int shape, int dep, int nsteps)
+{
+    pk_ctx* c = b->ctx;
+    int r;
+    for (pk_ens* e : b->ens)
+        if ((r = pk_ens_resort(e))) return r;
+    if (b->dirty || dt != b->dt || shape != b->shape || dep != b->dep)
+        if ((r = prepare(b, dt, shape, dep))) return r;
+    hipLaunchKernelGGL(k_batch_clear, dim3(1), dim3(64), 0, c->stream, b->halt);
+    PK_HIP(c, hipGetLastError());
+    bool state_err = false;
+    int done = 0;
+    while (done < nsteps) {
+        {
+            pk_ktimer tm(c, "batch_steps");
+            for (int k = done; k < nsteps; k++) {
+                if (b->exec) {
+                    PK_HIP(c, hipGraphLaunch(b->exec, c->stream));
+                } else if ((r = enqueue_step(b))) {
+                    return r;
+                }
+            }
+        }
+        int32_t h[PK_HALT_N];
+        PK_HIP(c, hipMemcpyAsync(h, b->halt, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+        PK_HIP(c, hipStreamSynchronize(c->stream));
+        if (!h[PK_HALT_STOP]) break;
+        // step done + h[PK_HALT_DONE] overflowed in its insert: finish its re-sort by rebuilds
+        done += h[PK_HALT_DONE] + 1;
+        for (pk_ens* e : b->ens) {
+            int32_t fl[PK_NFLAGS];
+            if ((r = pk_flags_read(e, fl))) return r;
+            if (fl[PK_FLAG_ERR]) state_err = true;
+            if (fl[PK_FLAG_OVERFLOW] && (r = pk_ens_rebuild(e, 0))) return r;
+            if ((r = pk_flags_clear(e))) return r;
+        }
+        b->dirty = true;
+        if (done < nsteps && (r = prepare(b, dt, shape, dep))) return r;
+        hipLaunchKernelGGL(k_batch_clear, dim3(1), dim3(64), 0, c->stream, b->halt);
+        PK_HIP(c, hipGetLastError());
+    }
+    for (pk_ens* e : b->ens) {
+        int32_t fl[PK_NFLAGS];
+        if ((r = pk_flags_read(e, fl))) return r;
+        if (fl[PK_FLAG_ERR]) state_err = true;
+        if ((r = pk_flags_clear(e))) return r;
+    }
+    if (state_err)
+        return pk_fail(c, PK_E_STATE,
+                       "batch step: a particle was not finite or moved further than its tile allows "
+                       "(it was left unchanged)");
+    return PK_OK;
+}
+
+// device copy of every ensemble's particles and every grid, for the autotuner's trials
+struct Snapshot {
+    struct Ens {
+        void* main = nullptr;
+        int64_t* off = nullptr;
+        int32_t* cnt = nullptr;
+        size_t capacity = 0;
+        int maxcap = 0;
+        int64_t np = 0;
+    };
+    std::vector<Ens> ens;
+    std::vector<void*> grids;
+    ~Snapshot()
+    {
+        for (auto& s : ens) {
+            if (s.main) (void)hipFree(s.main);
+            if (s.off) (void)hipFree(s.off);
+            if (s.cnt) (void)hipFree(s.cnt);
+        }
+        for (void* g : grids)
+            if (g) (void)hipFree(g);
+    }
+};
+
+size_t grid_bytes(const pk_grid* g) { return g->pitch * PK_NCOMP * g->esize(); }
+
+int snapshot_take(pk_batch* b, Snapshot& S)
+{
+    pk_ctx* c = b->ctx;
+    S.ens.resize(b->n());
+    S.grids.assign(b->n(), nullptr);
+    for (int i = 0; i < b->n(); i++) {
+        pk_ens* e = b->ens[i];
+        Snapshot::Ens& s = S.ens[i];
+        const size_t mb = e->capacity * e->nfield() * e->esize();
+        if (hipMalloc(&s.main, mb ? mb : 16) != hipSuccess || hipMalloc((void**)&s.off, sizeof(int64_t) * (e->nsct + 1)) != hipSuccess ||
+            hipMalloc((void**)&s.cnt, sizeof(int32_t) * e->nsct) != hipSuccess ||
+            hipMalloc(&S.grids[i], grid_bytes(b->grids[i])) != hipSuccess) {
+            (void)hipGetLastError();
+            return pk_fail(c, PK_E_ALLOC, "pk_batch_autotune: no device memory for the state copy");
+        }
+        PK_HIP(c, hipMemcpyAsync(s.main, e->main, mb, hipMemcpyDeviceToDevice, c->stream));
+        PK_HIP(c, hipMemcpyAsync(s.off, e->off, sizeof(int64_t) * (e->nsct + 1), hipMemcpyDeviceToDevice, c->stream));
+        PK_HIP(c, hipMemcpyAsync(s.cnt, e->cnt, sizeof(int32_t) * e->nsct, hipMemcpyDeviceToDevice, c->stream));
+        PK_HIP(c, hipMemcpyAsync(S.grids[i], b->grids[i]->data, grid_bytes(b->grids[i]), hipMemcpyDeviceToDevice,
+                                 c->stream));
+        s.capacity = e->capacity;
+        s.maxcap = e->maxcap;
+        s.np = e->np;
+    }
+    PK_HIP(c, hipStreamSynchronize(c->stream));
+    return PK_OK;
+}
+
+int snapshot_restore(pk_batch* b, const Snapshot& S)
+{
+    pk_ctx* c = b->ctx;
+    PK_HIP(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < b->n(); i++) {
+        pk_ens* e = b->ens[i];
+        const Snapshot::Ens& s = S.ens[i];
+        int r;
+        if (e->capacity != s.capacity && (r = pk_ens_alloc(e, s.capacity))) return r;
+        const size_t mb = e->capacity * e->nfield() * e->esize();
+        PK_HIP(c, hipMemcpyAsync(e->main, s.main, mb, hipMemcpyDeviceToDevice, c->stream));
+        PK_HIP(c, hipMemcpyAsync(e->off, s.off, sizeof(int64_t) * (e->nsct + 1), hipMemcpyDeviceToDevice, c->stream));
+        PK_HIP(c, hipMemcpyAsync(e->cnt, s.cnt, sizeof(int32_t) * e->nsct, hipMemcpyDeviceToDevice, c->stream));
+        PK_HIP(c, hipMemcpyAsync(b->grids[i]->data, S.grids[i], grid_bytes(b->grids[i]), hipMemcpyDeviceToDevice,
+                                 c->stream));
+        e->maxcap = s.maxcap;
+        e->np = s.np;
+        e->pending_movers = false;
+        if ((r = pk_flags_clear(e))) return r;
+    }
+    PK_HIP(c, hipStreamSynchronize(c->stream));
+    b->dirty = true;
+    return PK_OK;
+}
+
+}  // namespace
+
+int pk_batch_upload(pk_batch* b, void** dst, const void* src, size_t bytes)
+{
+    if (*dst) (void)hipFree(*dst);
+    *dst = nullptr;
+    if (bytes == 0) return PK_OK;
+    if (hipMalloc(dst, bytes) != hipSuccess) {
+        *dst = nullptr;
+        (void)hipGetLastError();
+        return pk_fail(b->ctx, PK_E_ALLOC, "batch: hipMalloc of a work list failed");
+    }
+    PK_HIP(b->ctx, hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+    return PK_OK;
+}
+
+extern "C" {
+
+int pk_batch_create(pk_ctx* c, int n, pk_ens* const* ens, pk_grid* const* grids, pk_batch** out)
+{
+    if (!c || !out || n <= 0 || !ens || !grids) return pk_fail(c, PK_E_ARG, "pk_batch_create: bad arguments");
+    *out = nullptr;
+    for (int i = 0; i < n; i++) {
+        if (!ens[i] || !grids[i] || ens[i]->ctx != c || grids[i]->ctx != c)
+            return pk_fail(c, PK_E_ARG, "pk_batch_create: every ensemble and grid must belong to this context");
+        int r = pk_check_pair(ens[i], grids[i]);
+        if (r) return r;
+        if (ens[i]->prec != ens[0]->prec) return pk_fail(c, PK_E_ARG, "pk_batch_create: mixed precisions");
+        for (int d = 0; d < 3; d++)
+            if (ens[i]->ncps[d] != ens[0]->ncps[d])
+                return pk_fail(c, PK_E_ARG, "pk_batch_create: all ensembles need the same supercell shape");
+        for (int j = 0; j < i; j++)
+            if (ens[j] == ens[i] || grids[j] == grids[i])
+                return pk_fail(c, PK_E_ARG, "pk_batch_create: an ensemble or grid appears twice");
+    }
+    pk_batch* b = new pk_batch();
+    b->ctx = c;
+    b->prec = ens[0]->prec;
+    b->ens.assign(ens, ens + n);
+    b->grids.assign(grids, grids + n);
+    if (hipMalloc((void**)&b->halt, sizeof(int) * PK_HALT_N) != hipSuccess) {
+        (void)hipGetLastError();
+        delete b;
+        return pk_fail(c, PK_E_ALLOC, "pk_batch_create: hipMalloc failed");
+    }
+    *out = b;
+    return PK_OK;
+}
+
+void pk_batch_destroy(pk_batch* b)
+{
+    if (!b) return;
+    (void)hipStreamSynchronize(b->ctx->stream);
+    drop_graph(b);
+    for (void* p : {(void*)b->items, b->pjobs, b->ijobs, b->fjobs, (void*)b->fstart, (void*)b->halt})
+        if (p) (void)hipFree(p);
+    delete b;
+}
+
+int pk_batch_set(pk_batch* b, int knob, int value)
+{
+    if (!b) return PK_E_ARG;
+    if (knob < 0 || knob >= PK_BATCH_NKNOB) return pk_fail(b->ctx, PK_E_ARG, "pk_batch_set: unknown knob");
+    const int hi = knob == PK_BATCH_WG_PER_CU ? 64 : 1;
+    if (value < 0 || value > hi) return pk_fail(b->ctx, PK_E_ARG, "pk_batch_set: value out of range");
+    if (b->knob[knob] != value) b->dirty = true;
+    b->knob[knob] = value;
+    return PK_OK;
+}
+
+int pk_batch_get(pk_batch* b, int knob, int* value)
+{
+    if (!b || !value) return PK_E_ARG;
+    if (knob < 0 || knob >= PK_BATCH_NKNOB) return pk_fail(b->ctx, PK_E_ARG, "pk_batch_get: unknown knob");
+    *value = b->knob[knob];
+    return PK_OK;
+}
+
+int pk_batch_run(pk_batch* b, double dt, int shape, int dep, int nsteps)
+{
+    if (!b) return PK_E_ARG;
+    int r = check_args(b, dt, shape, dep);
+    if (r) return r;
+    if (nsteps < 0) return pk_fail(b->ctx, PK_E_ARG, "pk_batch_run: negative step count");
+    if (nsteps == 0) return PK_OK;
+    return run(b, dt, shape, dep, nsteps);
+}
+
+int pk_batch_autotune(pk_batch* b, double dt, int shape, int dep, int trial_steps, double* best_ms)
+{
+    if (!b) return PK_E_ARG;
+    int r = check_args(b, dt, shape, dep);
+    if (r) return r;
+    if (trial_steps < 1) return pk_fail(b->ctx, PK_E_ARG, "pk_batch_autotune: trial_steps must be >= 1");
+    pk_ctx* c = b->ctx;
+    for (pk_ens* e : b->ens)
+        if ((r = pk_ens_resort(e))) return r;
+    Snapshot S;
+    if ((r = snapshot_take(b, S))) return r;
+    hipEvent_t t0, t1;
+    PK_HIP(c, hipEventCreate(&t0));
+    if (hipEventCreate(&t1) != hipSuccess) {
+        (void)hipEventDestroy(t0);
+        return pk_fail(c, PK_E_HIP, "pk_batch_autotune: hipEventCreate failed");
+    }
+    const int saved[PK_BATCH_NKNOB] = {b->knob[0], b->knob[1], b->knob[2]};
+    int best[PK_BATCH_NKNOB] = {saved[0], saved[1], saved[2]};
+    double best_t = 1e300;
+    const int wgs[] = {0, 1, 2, 4, 8};
+    for (int graph = 0; graph <= 1 && !r; graph++)
+        for (int order = 0; order <= 1 && !r; order++)
+            for (int wg : wgs) {
+                if ((r = snapshot_restore(b, S))) break;
+                b->knob[PK_BATCH_WG_PER_CU] = wg;
+                b->knob[PK_BATCH_ORDER] = order;
+                b->knob[PK_BATCH_GRAPH] = graph;
+                b->dirty = true;
+                if ((r = run(b, dt, shape, dep, 1))) break;     // warm-up, prepares and captures
+                if (hipEventRecord(t0, c->stream) != hipSuccess) { r = PK_E_HIP; break; }
+                if ((r = run(b, dt, shape, dep, trial_steps))) break;
+                if (hipEventRecord(t1, c->stream) != hipSuccess || hipEventSynchronize(t1) != hipSuccess) {
+                    r = PK_E_HIP;
+                    break;
+                }
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, t0, t1);
+                if (ms < best_t) {
+                    best_t = ms;
+                    best[0] = wg;
+                    best[1] = order;
+                    best[2] = graph;
+                }
+            }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    const int rr = snapshot_restore(b, S);
+    for (int k = 0; k < PK_BATCH_NKNOB; k++) b->knob[k] = r ? saved[k] : best[k];
+    b->dirty = true;
+    if (r) return r == PK_E_HIP ? pk_fail(c, PK_E_HIP, "pk_batch_autotune: event timing failed") : r;
+    if (rr) return rr;
+    if (best_ms) *best_ms = best_t / trial_steps;
+    return PK_OK;
+}
+
+}  // extern "C"

@@ -48,50 +48,97 @@ __device__ __forceinline__ long sc_neighbour(const EnsGeom& eg, long s, int code
     return ((long)c[0] * eg.nsc[1] + c[1]) * eg.nsc[2] + c[2];
 }
 
-// one wavefront per supercell: append incoming movers, class by class
+template <class W>
+struct InsertArgs {
+    EnsGeom eg;
+    Fields<W> pm, po;
+    const uint8_t* odir;
+    const int64_t* off;
+    int32_t* cnt;
+    const int32_t *nstay, *nmove, *mvc;
+    int32_t* flags;
+};
+
+// one wavefront appends the incoming movers of supercell s, class by class; an overflow of the
+// slack leaves the supercell as it is, raises the ensemble's overflow flag (and *pending)
 template <class W, int NF>
-__global__ void __launch_bounds__(64) k_insert(EnsGeom eg, Fields<W> pm, Fields<W> po, const uint8_t* __restrict__ odir,
-                                               const int64_t* __restrict__ off, int32_t* __restrict__ cnt,
-                                               const int32_t* __restrict__ nstay, const int32_t* __restrict__ nmove,
-                                               const int32_t* __restrict__ mvc, int32_t* __restrict__ flags)
+__device__ __forceinline__ void insert_sc(const InsertArgs<W>& A, long s, int* pending)
 {
-    const long s = pk_xcd_supercell(blockIdx.x, eg.xcd_chunk);
-    if (s >= eg.nsct) return;
+    const EnsGeom& eg = A.eg;
     const int lane = threadIdx.x;
-    const int nfinal = nstay[s];
-    const long cap = off[s + 1] - off[s];
+    const int nfinal = A.nstay[s];
+    const long cap = A.off[s + 1] - A.off[s];
     int term = 0;
-    if (lane < 27 && lane != 13) term = mvc[sc_neighbour(eg, s, lane, -1) * 27 + lane];
+    if (lane < 27 && lane != 13) term = A.mvc[sc_neighbour(eg, s, lane, -1) * 27 + lane];
     int nin = term;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) nin += __shfl_xor(nin, o);
     if ((long)nfinal + nin > cap) {
-        if (lane == 0) atomicOr(&flags[PK_FLAG_OVERFLOW], 1);
+        if (lane == 0) {
+            atomicOr(&A.flags[PK_FLAG_OVERFLOW], 1);
+            if (pending) atomicOr(pending, 1);
+        }
         return;
     }
-    if (lane == 0) cnt[s] = nfinal + nin;
-    long p = off[s] + nfinal;
+    if (lane == 0) A.cnt[s] = nfinal + nin;
+    long p = A.off[s] + nfinal;
     for (int d = 0; d < 27; d++) {
         if (d == 13) continue;
         const long src = sc_neighbour(eg, s, d, -1);
-        const int m = mvc[src * 27 + d];
+        const int m = A.mvc[src * 27 + d];
         if (m == 0) continue;
-        const int nm = nmove[src];
-        const long ob = off[src];
+        const int nm = A.nmove[src];
+        const long ob = A.off[src];
         int found = 0;
         for (int k0 = 0; k0 < nm && found < m; k0 += 64) {
             const int k = k0 + lane;
-            const bool f = k < nm && odir[ob + k] == d;
-            const uint64_t b = __ballot(f);
+            const bool f = k < nm && A.odir[ob + k] == d;
+            const uint64_t bal = __ballot(f);
             if (f) {
-                const long dst = p + found + __popcll(b & ((1ull << lane) - 1ull));
+                const long dst = p + found + __popcll(bal & ((1ull << lane) - 1ull));
 #pragma unroll
-                for (int q = 0; q < NF; q++) pm.f[q][dst] = po.f[q][ob + k];
+                for (int q = 0; q < NF; q++) A.pm.f[q][dst] = A.po.f[q][ob + k];
             }
-            found += __popcll(b);
+            found += __popcll(bal);
         }
         p += m;
     }
+}
+
+template <class W, int NF>
+__global__ void __launch_bounds__(64) k_insert(InsertArgs<W> A)
+{
+    const long s = pk_xcd_supercell(blockIdx.x, A.eg.xcd_chunk);
+    if (s >= A.eg.nsct) return;
+    insert_sc<W, NF>(A, s, nullptr);
+}
+
+template <class W, int NF>
+__global__ void __launch_bounds__(64) k_insert_batch(const InsertArgs<W>* __restrict__ jobs,
+                                                     const int2* __restrict__ items, long nitems, int* halt)
+{
+    if (halt[PK_HALT_STOP]) return;
+    for (long w = blockIdx.x; w < nitems; w += gridDim.x) {
+        const int2 it = items[w];
+        insert_sc<W, NF>(jobs[it.x], it.y, halt + PK_HALT_PENDING);
+    }
+}
+
+template <class W>
+InsertArgs<W> insert_args(const pk_ens* e)
+{
+    InsertArgs<W> A;
+    A.eg = ens_geom(e);
+    A.pm = fields_of<W>(e, e->main);
+    A.po = fields_of<W>(e, e->outbox);
+    A.odir = e->odir;
+    A.off = e->off;
+    A.cnt = e->cnt;
+    A.nstay = e->nstay;
+    A.nmove = e->nmove;
+    A.mvc = e->mvc;
+    A.flags = e->flags;
+    return A;
 }
 
 // ---- rebuild ----
@@ -488,14 +535,37 @@ int pk_ens_insert_launch(pk_ens* e)
     const unsigned nb = (unsigned)(PK_XCD * (long)eg.xcd_chunk);
     pk_ktimer tm(e->ctx, "insert");
     if (e->prec == PK_F64)
-        hipLaunchKernelGGL((k_insert<uint64_t, 7>), dim3(nb), dim3(64), 0, e->ctx->stream, eg,
-                           fields_of<uint64_t>(e, e->main), fields_of<uint64_t>(e, e->outbox), e->odir, e->off, e->cnt,
-                           e->nstay, e->nmove, e->mvc, e->flags);
+        hipLaunchKernelGGL((k_insert<uint64_t, 7>), dim3(nb), dim3(64), 0, e->ctx->stream, insert_args<uint64_t>(e));
     else
-        hipLaunchKernelGGL((k_insert<uint32_t, 8>), dim3(nb), dim3(64), 0, e->ctx->stream, eg,
-                           fields_of<uint32_t>(e, e->main), fields_of<uint32_t>(e, e->outbox), e->odir, e->off, e->cnt,
-                           e->nstay, e->nmove, e->mvc, e->flags);
+        hipLaunchKernelGGL((k_insert<uint32_t, 8>), dim3(nb), dim3(64), 0, e->ctx->stream, insert_args<uint32_t>(e));
     PK_HIP(e->ctx, hipGetLastError());
+    return PK_OK;
+}
+
+int pk_batch_insert_prepare(pk_batch* b)
+{
+    const int n = b->n();
+    if (b->prec == PK_F64) {
+        std::vector<InsertArgs<uint64_t>> jobs(n);
+        for (int i = 0; i < n; i++) jobs[i] = insert_args<uint64_t>(b->ens[i]);
+        return pk_batch_upload(b, &b->ijobs, jobs.data(), sizeof(jobs[0]) * n);
+    }
+    std::vector<InsertArgs<uint32_t>> jobs(n);
+    for (int i = 0; i < n; i++) jobs[i] = insert_args<uint32_t>(b->ens[i]);
+    return pk_batch_upload(b, &b->ijobs, jobs.data(), sizeof(jobs[0]) * n);
+}
+
+int pk_batch_insert_launch(pk_batch* b)
+{
+    if (b->nitems == 0) return PK_OK;
+    const unsigned nb = (unsigned)b->nitems;
+    if (b->prec == PK_F64)
+        hipLaunchKernelGGL((k_insert_batch<uint64_t, 7>), dim3(nb), dim3(64), 0, b->ctx->stream,
+                           (const InsertArgs<uint64_t>*)b->ijobs, b->items, b->nitems, b->halt);
+    else
+        hipLaunchKernelGGL((k_insert_batch<uint32_t, 8>), dim3(nb), dim3(64), 0, b->ctx->stream,
+                           (const InsertArgs<uint32_t>*)b->ijobs, b->items, b->nitems, b->halt);
+    PK_HIP(b->ctx, hipGetLastError());
     return PK_OK;
 }
 

@@ -16,10 +16,8 @@ struct YeeCoef {
 };
 
 template <class T>
-__global__ void __launch_bounds__(256) k_yee_e(GridView<T> g, YeeCoef<T> k, long n)
+__device__ __forceinline__ void yee_e_at(const GridView<T>& g, const YeeCoef<T>& k, long c)
 {
-    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n) return;
     const int s2 = g.s2;
     const long s12 = (long)g.s1 * g.s2;
     const int kk = (int)(c % s2);
@@ -40,10 +38,8 @@ __global__ void __launch_bounds__(256) k_yee_e(GridView<T> g, YeeCoef<T> k, long
 }
 
 template <class T>
-__global__ void __launch_bounds__(256) k_yee_b(GridView<T> g, YeeCoef<T> k, long n)
+__device__ __forceinline__ void yee_b_at(const GridView<T>& g, const YeeCoef<T>& k, long c)
 {
-    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n) return;
     const int s2 = g.s2;
     const long s12 = (long)g.s1 * g.s2;
     const int kk = (int)(c % s2);
@@ -60,13 +56,26 @@ __global__ void __launch_bounds__(256) k_yee_b(GridView<T> g, YeeCoef<T> k, long
         g.f[PK_BZ][c] += k.ky * (ex[c] - ex[c - s2]) - k.kx * (ey[c] - ey[c - s12]);
 }
 
+template <class T>
+__global__ void __launch_bounds__(256) k_yee_e(GridView<T> g, YeeCoef<T> k, long n)
+{
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n) yee_e_at(g, k, c);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_yee_b(GridView<T> g, YeeCoef<T> k, long n)
+{
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n) yee_b_at(g, k, c);
+}
+
 // Periodic fill: every ghost cell takes the value of its wrapped physical image.  The ghost
 // cells are enumerated as x slabs (full y,z), then y slabs (physical x, full z), then z slabs
 // (physical x,y); the closed-form image equals the dimension-by-dimension copy.
 template <class T>
-__global__ void __launch_bounds__(256) k_ghost_fill(GridView<T> g, unsigned mask, long nx_, long ny_, long nz_)
+__device__ __forceinline__ void ghost_fill_at(const GridView<T>& g, unsigned mask, long nx_, long ny_, long nz_, long t)
 {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int G = g.G;
     const int N0 = g.s0 - 2 * G, N1 = g.s1 - 2 * G, N2 = g.s2 - 2 * G;
     int i, j, k;
@@ -99,14 +108,19 @@ __global__ void __launch_bounds__(256) k_ghost_fill(GridView<T> g, unsigned mask
         if (mask & (1u << c)) g.f[c][dst] = g.f[c][src];
 }
 
+template <class T>
+__global__ void __launch_bounds__(256) k_ghost_fill(GridView<T> g, unsigned mask, long nx_, long ny_, long nz_)
+{
+    ghost_fill_at(g, mask, nx_, ny_, nz_, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // Periodic fold along dimension d: physical += ghost image, low ghosts first then high ones,
 // over the full extent of the other two dimensions (the oracle's order exactly).
 template <class T>
-__global__ void __launch_bounds__(256) k_ghost_fold(GridView<T> g, unsigned mask, int d)
+__device__ __forceinline__ void ghost_fold_at(const GridView<T>& g, unsigned mask, int d, long t)
 {
     const int s[3] = {g.s0, g.s1, g.s2};
     const int o1 = (d + 1) % 3, o2 = (d + 2) % 3;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)s[o1] * s[o2]) return;
     const int p = (int)(t / s[o2]), q = (int)(t % s[o2]);
     const int G = g.G;
@@ -125,6 +139,12 @@ __global__ void __launch_bounds__(256) k_ghost_fold(GridView<T> g, unsigned mask
 }
 
 template <class T>
+__global__ void __launch_bounds__(256) k_ghost_fold(GridView<T> g, unsigned mask, int d)
+{
+    ghost_fold_at(g, mask, d, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+template <class T>
 __global__ void __launch_bounds__(256) k_fill_zero(T* p, long n)
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -134,35 +154,36 @@ __global__ void __launch_bounds__(256) k_fill_zero(T* p, long n)
 inline unsigned nblk(long n, int b) { return (unsigned)((n + b - 1) / b); }
 
 template <class T>
-int yee_e(pk_grid* g, double dt_in)
+YeeCoef<T> yee_coef(const pk_grid* g, double dt_in, bool e_update)
 {
     const T dt = (T)dt_in;
     YeeCoef<T> k;
-    k.cc = -(T)4 * (T)PK_PI * dt;                 // YeeSolver.h:135
-    const T cdt = (T)PK_C * dt;                   // :137
-    k.kx = cdt / (T)g->step[0];                   // :138
+    k.cc = e_update ? -(T)4 * (T)PK_PI * dt : (T)0;    // YeeSolver.h:135
+    const T cdt = (T)PK_C * dt;                        // :137, :181-182
+    k.kx = cdt / (T)g->step[0];
     k.ky = cdt / (T)g->step[1];
     k.kz = cdt / (T)g->step[2];
+    return k;
+}
+
+template <class T>
+int yee_e(pk_grid* g, double dt)
+{
     const long n = (long)g->n;
     pk_ktimer tm(g->ctx, "yee_e");
-    hipLaunchKernelGGL(k_yee_e<T>, dim3(nblk(n, 256)), dim3(256), 0, g->ctx->stream, grid_view<T>(g), k, n);
+    hipLaunchKernelGGL(k_yee_e<T>, dim3(nblk(n, 256)), dim3(256), 0, g->ctx->stream, grid_view<T>(g),
+                       yee_coef<T>(g, dt, true), n);
     PK_HIP(g->ctx, hipGetLastError());
     return PK_OK;
 }
 
 template <class T>
-int yee_b(pk_grid* g, double dt_in)
+int yee_b(pk_grid* g, double dt)
 {
-    const T dt = (T)dt_in;
-    YeeCoef<T> k;
-    k.cc = 0;
-    const T cdt = (T)PK_C * dt;                   // YeeSolver.h:181-182
-    k.kx = cdt / (T)g->step[0];
-    k.ky = cdt / (T)g->step[1];
-    k.kz = cdt / (T)g->step[2];
     const long n = (long)g->n;
     pk_ktimer tm(g->ctx, "yee_b");
-    hipLaunchKernelGGL(k_yee_b<T>, dim3(nblk(n, 256)), dim3(256), 0, g->ctx->stream, grid_view<T>(g), k, n);
+    hipLaunchKernelGGL(k_yee_b<T>, dim3(nblk(n, 256)), dim3(256), 0, g->ctx->stream, grid_view<T>(g),
+                       yee_coef<T>(g, dt, false), n);
     PK_HIP(g->ctx, hipGetLastError());
     return PK_OK;
 }
@@ -202,6 +223,106 @@ int ghost(pk_grid* g, int ops)
     return PK_OK;
 }
 
+// ---- many grids per launch ----
+template <class T>
+struct FieldJob {
+    GridView<T> g;
+    YeeCoef<T> ke, kb;      // updateE(dt), updateB(dt / 2)
+    long ncell, nx, ny, nz;
+};
+
+// ensemble of block b: the last i with start[i] <= b (empty ranges are skipped)
+__device__ __forceinline__ int job_of_block(const long* start, int n, long b)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= b) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <class T, int OP>
+__global__ void __launch_bounds__(256) k_batch_field(const FieldJob<T>* __restrict__ jobs, const long* __restrict__ start,
+                                                     int n, unsigned mask, int d, const int* __restrict__ halt)
+{
+    if (halt[PK_HALT_STOP]) return;
+    const int i = job_of_block(start, n, blockIdx.x);
+    const FieldJob<T>& J = jobs[i];
+    const long t = (blockIdx.x - start[i]) * 256 + threadIdx.x;
+    if (OP == PK_BOP_ZERO_J) {
+        if (t < J.ncell)
+            for (int c = PK_JX; c <= PK_JZ; c++) J.g.f[c][t] = T(0);
+    } else if (OP == PK_BOP_YEE_E) {
+        if (t < J.ncell) yee_e_at(J.g, J.ke, t);
+    } else if (OP == PK_BOP_YEE_B) {
+        if (t < J.ncell) yee_b_at(J.g, J.kb, t);
+    } else if (OP == PK_BOP_FILL) {
+        ghost_fill_at(J.g, mask, J.nx, J.ny, J.nz, t);
+    } else {
+        ghost_fold_at(J.g, mask, d, t);
+    }
+}
+
+template <class T>
+int batch_prepare(pk_batch* b, double dt)
+{
+    const int n = b->n();
+    std::vector<FieldJob<T>> jobs(n);
+    std::vector<long> start((size_t)PK_BSTAGES * (n + 1), 0);
+    for (int i = 0; i < n; i++) {
+        const pk_grid* g = b->grids[i];
+        FieldJob<T>& J = jobs[i];
+        J.g = grid_view<T>(g);
+        J.ke = yee_coef<T>(g, dt, true);
+        J.kb = yee_coef<T>(g, dt * 0.5, false);
+        const int G = g->ghost;
+        J.ncell = (long)g->n;
+        J.nx = (long)2 * G * g->s[1] * g->s[2];
+        J.ny = (long)(g->s[0] - 2 * G) * 2 * G * g->s[2];
+        J.nz = (long)(g->s[0] - 2 * G) * (g->s[1] - 2 * G) * 2 * G;
+        const long work[PK_BSTAGES] = {J.ncell, J.nx + J.ny + J.nz, (long)g->s[1] * g->s[2], (long)g->s[2] * g->s[0],
+                                       (long)g->s[0] * g->s[1]};
+        for (int st = 0; st < PK_BSTAGES; st++)
+            start[(size_t)st * (n + 1) + i + 1] = start[(size_t)st * (n + 1) + i] + (long)nblk(work[st], 256);
+    }
+    for (int st = 0; st < PK_BSTAGES; st++) b->fblocks[st] = start[(size_t)st * (n + 1) + n];
+    int r = pk_batch_upload(b, &b->fjobs, jobs.data(), sizeof(FieldJob<T>) * n);
+    if (r) return r;
+    return pk_batch_upload(b, (void**)&b->fstart, start.data(), sizeof(long) * start.size());
+}
+
+template <class T>
+int batch_launch(pk_batch* b, int op, unsigned mask, int d)
+{
+    const int n = b->n();
+    const int st = op == PK_BOP_FILL ? PK_BSTAGE_FILL : (op == PK_BOP_FOLD ? PK_BSTAGE_FOLD + d : PK_BSTAGE_CELLS);
+    const long nb = b->fblocks[st];
+    if (nb == 0) return PK_OK;
+    const FieldJob<T>* jobs = (const FieldJob<T>*)b->fjobs;
+    const long* start = b->fstart + (size_t)st * (n + 1);
+    hipStream_t s = b->ctx->stream;
+    switch (op) {
+    case PK_BOP_ZERO_J:
+        hipLaunchKernelGGL((k_batch_field<T, PK_BOP_ZERO_J>), dim3(nb), dim3(256), 0, s, jobs, start, n, mask, d, b->halt);
+        break;
+    case PK_BOP_YEE_E:
+        hipLaunchKernelGGL((k_batch_field<T, PK_BOP_YEE_E>), dim3(nb), dim3(256), 0, s, jobs, start, n, mask, d, b->halt);
+        break;
+    case PK_BOP_YEE_B:
+        hipLaunchKernelGGL((k_batch_field<T, PK_BOP_YEE_B>), dim3(nb), dim3(256), 0, s, jobs, start, n, mask, d, b->halt);
+        break;
+    case PK_BOP_FILL:
+        hipLaunchKernelGGL((k_batch_field<T, PK_BOP_FILL>), dim3(nb), dim3(256), 0, s, jobs, start, n, mask, d, b->halt);
+        break;
+    default:
+        hipLaunchKernelGGL((k_batch_field<T, PK_BOP_FOLD>), dim3(nb), dim3(256), 0, s, jobs, start, n, mask, d, b->halt);
+        break;
+    }
+    PK_HIP(b->ctx, hipGetLastError());
+    return PK_OK;
+}
+
 }  // namespace
 
 int pk_field_zero(pk_grid* g, int comp)
@@ -221,3 +342,13 @@ int pk_field_zero(pk_grid* g, int comp)
 int pk_field_yee_e(pk_grid* g, double dt) { return g->prec == PK_F64 ? yee_e<double>(g, dt) : yee_e<float>(g, dt); }
 int pk_field_yee_b(pk_grid* g, double dt) { return g->prec == PK_F64 ? yee_b<double>(g, dt) : yee_b<float>(g, dt); }
 int pk_field_ghost(pk_grid* g, int ops) { return g->prec == PK_F64 ? ghost<double>(g, ops) : ghost<float>(g, ops); }
+
+int pk_batch_fields_prepare(pk_batch* b, double dt)
+{
+    return b->prec == PK_F64 ? batch_prepare<double>(b, dt) : batch_prepare<float>(b, dt);
+}
+
+int pk_batch_fields_launch(pk_batch* b, int op, unsigned mask, int d)
+{
+    return b->prec == PK_F64 ? batch_launch<double>(b, op, mask, d) : batch_launch<float>(b, op, mask, d);
+}

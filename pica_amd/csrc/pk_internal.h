@@ -182,6 +182,51 @@ __device__ __forceinline__ long pk_xcd_supercell(long b, int chunk)
     return (b % PK_XCD) * (long)chunk + b / PK_XCD;
 }
 
+// ---- many ensembles stepped together (pk_batch_*) ----
+// Work of one stage for every ensemble goes into one launch: the particle and insert kernels
+// walk a list of (ensemble, supercell) items, the field kernels a per-ensemble block prefix.
+enum { PK_BSTAGE_CELLS = 0, PK_BSTAGE_FILL = 1, PK_BSTAGE_FOLD = 2, PK_BSTAGES = 5 };   // FOLD + d
+enum { PK_BOP_ZERO_J = 0, PK_BOP_YEE_E, PK_BOP_YEE_B, PK_BOP_FILL, PK_BOP_FOLD };
+// device halt words of a batch: every batched kernel returns at once while STOP is set; an insert
+// overflow sets PENDING, which the step's last kernel turns into STOP (else it counts the step in DONE)
+enum { PK_HALT_STOP = 0, PK_HALT_DONE = 1, PK_HALT_PENDING = 2, PK_HALT_N = 4 };
+
+struct pk_batch {
+    pk_ctx* ctx = nullptr;
+    int prec = PK_F64;
+    std::vector<pk_ens*> ens;
+    std::vector<pk_grid*> grids;
+    int knob[PK_BATCH_NKNOB] = {0, 1, 1};
+    // prepared launch state, valid while !dirty and (dt, shape, dep) are unchanged
+    bool dirty = true;
+    double dt = 0.0;
+    int shape = -1, dep = -1;
+    int2* items = nullptr;     // (ensemble, supercell) in launch order
+    long nitems = 0;
+    void* pjobs = nullptr;     // particle-kernel arguments, one entry per ensemble
+    void* ijobs = nullptr;     // insert-kernel arguments
+    void* fjobs = nullptr;     // field-kernel arguments
+    long* fstart = nullptr;    // PK_BSTAGES x (n + 1) block prefixes of the field stages
+    long fblocks[PK_BSTAGES] = {0, 0, 0, 0, 0};
+    size_t plds = 0;           // dynamic LDS of the particle kernel (largest ensemble)
+    int* halt = nullptr;       // device halt words (PK_HALT_*)
+    hipGraphExec_t exec = nullptr;
+    int n() const { return (int)ens.size(); }
+};
+// replace a device buffer by a copy of host data (outside any capture)
+int pk_batch_upload(pk_batch* b, void** dst, const void* src, size_t bytes);
+int pk_batch_fields_prepare(pk_batch* b, double dt);
+int pk_batch_fields_launch(pk_batch* b, int op, unsigned mask, int d);
+int pk_batch_split_prepare(pk_batch* b, double dt, int shape, int dep);
+int pk_batch_split_launch(pk_batch* b, int shape, int dep);
+int pk_batch_f64_prepare(pk_batch* b, double dt, int shape, int dep);
+int pk_batch_f64_launch(pk_batch* b, int shape, int dep);
+int pk_batch_insert_prepare(pk_batch* b);
+int pk_batch_insert_launch(pk_batch* b);
+int pk_check_pair(pk_ens* e, pk_grid* g);   // the periodic step's grid/ensemble requirements
+int pk_flags_read(pk_ens* e, int32_t out[PK_NFLAGS]);
+int pk_flags_clear(pk_ens* e);
+
 // ---- launchers implemented in the .hip files ----
 int pk_field_zero(pk_grid* g, int comp);
 int pk_field_yee_e(pk_grid* g, double dt);

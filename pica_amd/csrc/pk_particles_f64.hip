@@ -400,16 +400,35 @@ struct StepParams {
     int mask_words;
 };
 
+// all arguments of one ensemble's particle step (one kernel argument, or one entry of a batch)
+struct F64Args {
+    GridView<double> g;
+    EnsGeom eg;
+    PartF64 pm, po;
+    uint8_t* odir;
+    const int64_t* off;
+    const int32_t* cnt;
+    int32_t *nstay, *nmove, *mvc, *flags;
+    StepParams sp;
+};
+
+// one supercell s of one ensemble
 template <int SHAPE, int DEP>
-__global__ void __launch_bounds__(PK_BLOCK) k_particles(GridView<double> g, EnsGeom eg, PartF64 pm, PartF64 po,
-                                                        uint8_t* __restrict__ odir, const int64_t* __restrict__ off,
-                                                        const int32_t* __restrict__ cnt, int32_t* __restrict__ nstay,
-                                                        int32_t* __restrict__ nmove, int32_t* __restrict__ mvc,
-                                                        int32_t* __restrict__ flags, StepParams sp)
+__device__ __forceinline__ void particles_f64_sc(const F64Args& A, long s)
 {
+    const GridView<double>& g = A.g;
+    const EnsGeom& eg = A.eg;
+    const PartF64& pm = A.pm;
+    const PartF64& po = A.po;
+    uint8_t* __restrict__ odir = A.odir;
+    const int64_t* __restrict__ off = A.off;
+    const int32_t* __restrict__ cnt = A.cnt;
+    int32_t* __restrict__ nstay = A.nstay;
+    int32_t* __restrict__ nmove = A.nmove;
+    int32_t* __restrict__ mvc = A.mvc;
+    int32_t* __restrict__ flags = A.flags;
+    const StepParams& sp = A.sp;
     extern __shared__ __align__(16) unsigned char smem[];
-    const long s = pk_xcd_supercell(blockIdx.x, eg.xcd_chunk);
-    if (s >= eg.nsct) return;
     const int sidx[3] = {(int)(s / ((long)eg.nsc[1] * eg.nsc[2])), (int)((s / eg.nsc[2]) % eg.nsc[1]),
                          (int)(s % eg.nsc[2])};
     ScGeom q;
@@ -587,9 +606,29 @@ __global__ void __launch_bounds__(PK_BLOCK) k_particles(GridView<double> g, EnsG
 }
 
 template <int SHAPE, int DEP>
-int launch(pk_ens* e, pk_grid* g, double dt)
+__global__ void __launch_bounds__(PK_BLOCK) k_particles(F64Args A)
 {
-    EnsGeom eg = ens_geom(e);
+    const long s = pk_xcd_supercell(blockIdx.x, A.eg.xcd_chunk);
+    if (s >= A.eg.nsct) return;
+    particles_f64_sc<SHAPE, DEP>(A, s);
+}
+
+// many ensembles: a list of (ensemble, supercell) items; gridDim.x < nitems makes it persistent
+template <int SHAPE, int DEP>
+__global__ void __launch_bounds__(PK_BLOCK) k_particles_batch(const F64Args* __restrict__ jobs,
+                                                              const int2* __restrict__ items, long nitems,
+                                                              const int* __restrict__ halt)
+{
+    if (halt[PK_HALT_STOP]) return;
+    for (long w = blockIdx.x; w < nitems; w += gridDim.x) {
+        __syncthreads();
+        const int2 it = items[w];
+        particles_f64_sc<SHAPE, DEP>(jobs[it.x], it.y);     // arguments read in place (a copy would live in scratch)
+    }
+}
+
+StepParams step_params(const pk_ens* e, double dt)
+{
     StepParams sp;
     sp.dt = dt;
     const double q = e->ctx->charge[e->type], m = e->ctx->mass[e->type];
@@ -601,24 +640,79 @@ int launch(pk_ens* e, pk_grid* g, double dt)
         sp.scinv[d] = 1.0 / ((double)e->ncps[d] * cellSize);
     }
     sp.mask_words = (e->maxcap + PK_BLOCK + 63) / 64 + 1;
+    return sp;
+}
+
+// dynamic LDS of k_particles: field and J tiles, mover mask, counters, mask prefix
+size_t f64_lds(const pk_ens* e, int shape, int dep, int mask_words)
+{
     int its[3], dts[3];
     for (int d = 0; d < 3; d++) {
-        its[d] = e->ncps[d] + (SHAPE == PK_CIC ? 2 : 3);
-        dts[d] = e->ncps[d] + ((SHAPE == PK_CIC && DEP == PK_DIRECT) ? 3 : 4);
+        its[d] = e->ncps[d] + (shape == PK_CIC ? 2 : 3);
+        dts[d] = e->ncps[d] + ((shape == PK_CIC && dep == PK_DIRECT) ? 3 : 4);
     }
-    const size_t lds = sizeof(double) * (6 * its[0] * its[1] * its[2] + 3 * dts[0] * dts[1] * dts[2]) +
-                       sizeof(uint64_t) * sp.mask_words + sizeof(int) * 32 +
-                       sizeof(long) * (sp.mask_words + 1) + 16;
+    return sizeof(double) * (6 * its[0] * its[1] * its[2] + 3 * dts[0] * dts[1] * dts[2]) +
+           sizeof(uint64_t) * mask_words + sizeof(int) * 32 + sizeof(long) * (mask_words + 1) + 16;
+}
+
+F64Args f64_args(pk_ens* e, pk_grid* g, const EnsGeom& eg, const StepParams& sp)
+{
+    F64Args A;
+    A.g = grid_view<double>(g);
+    A.eg = eg;
+    A.pm = part_f64(e, e->main);
+    A.po = part_f64(e, e->outbox);
+    A.odir = e->odir;
+    A.off = e->off;
+    A.cnt = e->cnt;
+    A.nstay = e->nstay;
+    A.nmove = e->nmove;
+    A.mvc = e->mvc;
+    A.flags = e->flags;
+    A.sp = sp;
+    return A;
+}
+
+template <int SHAPE, int DEP>
+int launch(pk_ens* e, pk_grid* g, double dt)
+{
+    EnsGeom eg = ens_geom(e);
+    const StepParams sp = step_params(e, dt);
+    const size_t lds = f64_lds(e, SHAPE, DEP, sp.mask_words);
     if (lds > 160 * 1024)
         return pk_fail(e->ctx, PK_E_CAPACITY, "particle step: supercell too large for LDS");
     const unsigned nblocks = (unsigned)(PK_XCD * (long)eg.xcd_chunk);
-    PartF64 pm = part_f64(e, e->main), po = part_f64(e, e->outbox);
-    GridView<double> gv = grid_view<double>(g);
+    const F64Args A = f64_args(e, g, eg, sp);
     pk_ktimer tm(e->ctx, "particles");
-    hipLaunchKernelGGL((k_particles<SHAPE, DEP>), dim3(nblocks), dim3(PK_BLOCK), lds, e->ctx->stream, gv, eg, pm, po,
-                       e->odir, e->off, e->cnt, e->nstay, e->nmove, e->mvc, e->flags, sp);
+    hipLaunchKernelGGL((k_particles<SHAPE, DEP>), dim3(nblocks), dim3(PK_BLOCK), lds, e->ctx->stream, A);
     PK_HIP(e->ctx, hipGetLastError());
     return PK_OK;
+}
+
+template <int SHAPE, int DEP>
+int batch_go(pk_batch* b, bool prepare)
+{
+    const void* fn = (const void*)k_particles_batch<SHAPE, DEP>;
+    if (prepare) {
+        PK_HIP(b->ctx, hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->plds));
+        return PK_OK;
+    }
+    if (b->nitems == 0) return PK_OK;
+    const long wg = b->knob[PK_BATCH_WG_PER_CU];
+    const long nb = wg > 0 && wg * 256 < b->nitems ? wg * 256 : b->nitems;
+    hipLaunchKernelGGL((k_particles_batch<SHAPE, DEP>), dim3((unsigned)nb), dim3(PK_BLOCK), b->plds, b->ctx->stream,
+                       (const F64Args*)b->pjobs, b->items, b->nitems, b->halt);
+    PK_HIP(b->ctx, hipGetLastError());
+    return PK_OK;
+}
+
+int batch_dispatch(pk_batch* b, int shape, int dep, bool prepare)
+{
+    if (shape == PK_CIC && dep == PK_DIRECT) return batch_go<PK_CIC, PK_DIRECT>(b, prepare);
+    if (shape == PK_CIC && dep == PK_ESIRKEPOV) return batch_go<PK_CIC, PK_ESIRKEPOV>(b, prepare);
+    if (shape == PK_TSC && dep == PK_DIRECT) return batch_go<PK_TSC, PK_DIRECT>(b, prepare);
+    if (shape == PK_TSC && dep == PK_ESIRKEPOV) return batch_go<PK_TSC, PK_ESIRKEPOV>(b, prepare);
+    return pk_fail(b->ctx, PK_E_ARG, "particle step: unknown shape/depositor");
 }
 
 }  // namespace
@@ -631,3 +725,23 @@ int pk_particles_f64(pk_ens* e, pk_grid* g, double dt, int shape, int dep)
     if (shape == PK_TSC && dep == PK_ESIRKEPOV) return launch<PK_TSC, PK_ESIRKEPOV>(e, g, dt);
     return pk_fail(e->ctx, PK_E_ARG, "particle step: unknown shape/depositor");
 }
+
+int pk_batch_f64_prepare(pk_batch* b, double dt, int shape, int dep)
+{
+    const int n = b->n();
+    std::vector<F64Args> jobs(n);
+    size_t lds = 0;
+    for (int i = 0; i < n; i++) {
+        pk_ens* e = b->ens[i];
+        const StepParams sp = step_params(e, dt);
+        jobs[i] = f64_args(e, b->grids[i], ens_geom(e), sp);
+        const size_t l = f64_lds(e, shape, dep, sp.mask_words);
+        if (l > lds) lds = l;
+    }
+    if (lds > 160 * 1024) return pk_fail(b->ctx, PK_E_CAPACITY, "particle step: supercell too large for LDS");
+    b->plds = lds;
+    if (int r = pk_batch_upload(b, &b->pjobs, jobs.data(), sizeof(F64Args) * n)) return r;
+    return batch_dispatch(b, shape, dep, true);
+}
+
+int pk_batch_f64_launch(pk_batch* b, int shape, int dep) { return batch_dispatch(b, shape, dep, false); }

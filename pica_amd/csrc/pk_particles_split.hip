@@ -266,19 +266,36 @@ struct SplitParams {
     int mask_words;
 };
 
-// NC > 0: cubic supercells of NC cells (compile-time tile strides); NC == 0: any shape
+// all arguments of one ensemble's particle step (one kernel argument, or one entry of a batch)
+struct SplitArgs {
+    GridView<float> g;
+    EnsGeom eg;
+    PartSplit pm, po;
+    uint8_t* odir;
+    const int64_t* off;
+    const int32_t* cnt;
+    int32_t *nstay, *nmove, *mvc, *flags;
+    SplitParams sp;
+};
+
+// One supercell s of one ensemble.  NC > 0: cubic supercells of NC cells (compile-time tile
+// strides); NC == 0: any shape.
 template <int SHAPE, int DEP, int NC>
-__global__ void __launch_bounds__(PK_BLOCK) k_particles_split(GridView<float> g, EnsGeom eg, PartSplit pm, PartSplit po,
-                                                              uint8_t* __restrict__ odir,
-                                                              const int64_t* __restrict__ off,
-                                                              const int32_t* __restrict__ cnt,
-                                                              int32_t* __restrict__ nstay, int32_t* __restrict__ nmove,
-                                                              int32_t* __restrict__ mvc, int32_t* __restrict__ flags,
-                                                              SplitParams sp)
+__device__ __forceinline__ void particles_split_sc(const SplitArgs& A, long s)
 {
+    const GridView<float>& g = A.g;
+    const EnsGeom& eg = A.eg;
+    const PartSplit& pm = A.pm;
+    const PartSplit& po = A.po;
+    uint8_t* __restrict__ odir = A.odir;
+    const int64_t* __restrict__ off = A.off;
+    const int32_t* __restrict__ cnt = A.cnt;
+    int32_t* __restrict__ nstay = A.nstay;
+    int32_t* __restrict__ nmove = A.nmove;
+    int32_t* __restrict__ mvc = A.mvc;
+    int32_t* __restrict__ flags = A.flags;
+    const SplitParams& sp = A.sp;
     extern __shared__ __align__(16) unsigned char smem[];
-    const long s = pk_xcd_supercell(blockIdx.x, eg.xcd_chunk);
-    if (s >= eg.nsct) return;
     constexpr bool PICA_CIC = SHAPE == PK_CIC && DEP == PK_DIRECT;
     const int nn[3] = {NC ? NC : eg.ncps[0], NC ? NC : eg.ncps[1], NC ? NC : eg.ncps[2]};
     const int sidx[3] = {(int)(s / ((long)eg.nsc[1] * eg.nsc[2])), (int)((s / eg.nsc[2]) % eg.nsc[1]),
@@ -515,6 +532,28 @@ __global__ void __launch_bounds__(PK_BLOCK) k_particles_split(GridView<float> g,
         nstay[s] = nfinal;
         nmove[s] = nmv;
         if (bad[0]) atomicOr(&flags[PK_FLAG_ERR], 1);
+    }
+}
+
+template <int SHAPE, int DEP, int NC>
+__global__ void __launch_bounds__(PK_BLOCK) k_particles_split(SplitArgs A)
+{
+    const long s = pk_xcd_supercell(blockIdx.x, A.eg.xcd_chunk);
+    if (s >= A.eg.nsct) return;
+    particles_split_sc<SHAPE, DEP, NC>(A, s);
+}
+
+// many ensembles: a list of (ensemble, supercell) items; gridDim.x < nitems makes it persistent
+template <int SHAPE, int DEP, int NC>
+__global__ void __launch_bounds__(PK_BLOCK) k_particles_split_batch(const SplitArgs* __restrict__ jobs,
+                                                                    const int2* __restrict__ items, long nitems,
+                                                                    const int* __restrict__ halt)
+{
+    if (halt[PK_HALT_STOP]) return;
+    for (long w = blockIdx.x; w < nitems; w += gridDim.x) {
+        __syncthreads();
+        const int2 it = items[w];
+        particles_split_sc<SHAPE, DEP, NC>(jobs[it.x], it.y);     // arguments read in place (a copy would live in scratch)
     }
 }
 
@@ -954,6 +993,37 @@ __global__ void __launch_bounds__(PK_BLOCK) k_esk_sorted(GridView<float> g, EnsG
     }
 }
 
+// dynamic LDS of k_particles_split: field tile, FP64 J tile, mover mask, counters, mask prefix
+size_t split_lds(const pk_ens* e, int shape, int dep, int mask_words)
+{
+    int its[3], dts[3];
+    for (int d = 0; d < 3; d++) {
+        its[d] = e->ncps[d] + (shape == PK_TSC ? 3 : 2);
+        dts[d] = e->ncps[d] + ((shape == PK_CIC && dep == PK_DIRECT) ? 3 : 4);
+    }
+    const size_t joff = (sizeof(float) * 6 * its[0] * its[1] * its[2] + 7) & ~(size_t)7;
+    return joff + sizeof(double) * 3 * dts[0] * dts[1] * dts[2] + sizeof(uint64_t) * mask_words + sizeof(int) * 32 +
+           sizeof(long) * (mask_words + 1) + 16;
+}
+
+SplitArgs split_args(pk_ens* e, pk_grid* g, const EnsGeom& eg, const SplitParams& sp)
+{
+    SplitArgs A;
+    A.g = grid_view<float>(g);
+    A.eg = eg;
+    A.pm = part_split(e, e->main);
+    A.po = part_split(e, e->outbox);
+    A.odir = e->odir;
+    A.off = e->off;
+    A.cnt = e->cnt;
+    A.nstay = e->nstay;
+    A.nmove = e->nmove;
+    A.mvc = e->mvc;
+    A.flags = e->flags;
+    A.sp = sp;
+    return A;
+}
+
 int split_params(pk_ens* e, double dt, SplitParams& sp)
 {
     const double q = e->ctx->charge[e->type], m = e->ctx->mass[e->type];
@@ -976,20 +1046,12 @@ int launch(pk_ens* e, pk_grid* g, double dt)
     EnsGeom eg = ens_geom(e);
     SplitParams sp;
     if (int r = split_params(e, dt, sp)) return r;
-    int its[3], dts[3];
-    for (int d = 0; d < 3; d++) {
-        its[d] = e->ncps[d] + (SHAPE == PK_TSC ? 3 : 2);
-        dts[d] = e->ncps[d] + ((SHAPE == PK_CIC && DEP == PK_DIRECT) ? 3 : 4);
-    }
-    const size_t joff = (sizeof(float) * 6 * its[0] * its[1] * its[2] + 7) & ~(size_t)7;
-    const size_t lds = joff + sizeof(double) * 3 * dts[0] * dts[1] * dts[2] + sizeof(uint64_t) * sp.mask_words +
-                       sizeof(int) * 32 + sizeof(long) * (sp.mask_words + 1) + 16;
+    const size_t lds = split_lds(e, SHAPE, DEP, sp.mask_words);
     if (lds > 160 * 1024) return pk_fail(e->ctx, PK_E_CAPACITY, "particle step: supercell too large for LDS");
     const unsigned nblocks = (unsigned)(PK_XCD * (long)eg.xcd_chunk);
     pk_ktimer tm(e->ctx, "particles");
-    hipLaunchKernelGGL((k_particles_split<SHAPE, DEP, NC>), dim3(nblocks), dim3(PK_BLOCK), lds, e->ctx->stream,
-                       grid_view<float>(g), eg, part_split(e, e->main), part_split(e, e->outbox), e->odir, e->off,
-                       e->cnt, e->nstay, e->nmove, e->mvc, e->flags, sp);
+    const SplitArgs A = split_args(e, g, eg, sp);
+    hipLaunchKernelGGL((k_particles_split<SHAPE, DEP, NC>), dim3(nblocks), dim3(PK_BLOCK), lds, e->ctx->stream, A);
     PK_HIP(e->ctx, hipGetLastError());
     return PK_OK;
 }
@@ -1065,6 +1127,43 @@ int launch_nc(pk_ens* e, pk_grid* g, double dt)
     return launch<SHAPE, DEP, 0>(e, g, dt);
 }
 
+template <int SHAPE, int DEP, int NC>
+int batch_go(pk_batch* b, bool prepare)
+{
+    const void* fn = (const void*)k_particles_split_batch<SHAPE, DEP, NC>;
+    if (prepare) {
+        PK_HIP(b->ctx, hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->plds));
+        return PK_OK;
+    }
+    if (b->nitems == 0) return PK_OK;
+    const long wg = b->knob[PK_BATCH_WG_PER_CU];
+    const long nb = wg > 0 && wg * 256 < b->nitems ? wg * 256 : b->nitems;
+    hipLaunchKernelGGL((k_particles_split_batch<SHAPE, DEP, NC>), dim3((unsigned)nb), dim3(PK_BLOCK), b->plds,
+                       b->ctx->stream, (const SplitArgs*)b->pjobs, b->items, b->nitems, b->halt);
+    PK_HIP(b->ctx, hipGetLastError());
+    return PK_OK;
+}
+
+template <int SHAPE, int DEP>
+int batch_go_nc(pk_batch* b, bool prepare)
+{
+    const pk_ens* e = b->ens[0];
+    const bool cube = e->ncps[0] == e->ncps[1] && e->ncps[1] == e->ncps[2];
+    if (cube && e->ncps[0] == 4) return batch_go<SHAPE, DEP, 4>(b, prepare);
+    if (cube && e->ncps[0] == 2) return batch_go<SHAPE, DEP, 2>(b, prepare);
+    if (cube && e->ncps[0] == 8) return batch_go<SHAPE, DEP, 8>(b, prepare);
+    return batch_go<SHAPE, DEP, 0>(b, prepare);
+}
+
+int batch_dispatch(pk_batch* b, int shape, int dep, bool prepare)
+{
+    if (shape == PK_CIC && dep == PK_DIRECT) return batch_go_nc<PK_CIC, PK_DIRECT>(b, prepare);
+    if (shape == PK_CIC && dep == PK_ESIRKEPOV) return batch_go_nc<PK_CIC, PK_ESIRKEPOV>(b, prepare);
+    if (shape == PK_TSC && dep == PK_DIRECT) return batch_go_nc<PK_TSC, PK_DIRECT>(b, prepare);
+    if (shape == PK_TSC && dep == PK_ESIRKEPOV) return batch_go_nc<PK_TSC, PK_ESIRKEPOV>(b, prepare);
+    return pk_fail(b->ctx, PK_E_ARG, "particle step: unknown shape/depositor");
+}
+
 }  // namespace
 
 int pk_particles_split(pk_ens* e, pk_grid* g, double dt, int shape, int dep)
@@ -1075,3 +1174,24 @@ int pk_particles_split(pk_ens* e, pk_grid* g, double dt, int shape, int dep)
     if (shape == PK_TSC && dep == PK_ESIRKEPOV) return launch_nc<PK_TSC, PK_ESIRKEPOV>(e, g, dt);
     return pk_fail(e->ctx, PK_E_ARG, "particle step: unknown shape/depositor");
 }
+
+int pk_batch_split_prepare(pk_batch* b, double dt, int shape, int dep)
+{
+    const int n = b->n();
+    std::vector<SplitArgs> jobs(n);
+    size_t lds = 0;
+    for (int i = 0; i < n; i++) {
+        pk_ens* e = b->ens[i];
+        SplitParams sp;
+        if (int r = split_params(e, dt, sp)) return r;
+        jobs[i] = split_args(e, b->grids[i], ens_geom(e), sp);
+        const size_t l = split_lds(e, shape, dep, sp.mask_words);
+        if (l > lds) lds = l;
+    }
+    if (lds > 160 * 1024) return pk_fail(b->ctx, PK_E_CAPACITY, "particle step: supercell too large for LDS");
+    b->plds = lds;
+    if (int r = pk_batch_upload(b, &b->pjobs, jobs.data(), sizeof(SplitArgs) * n)) return r;
+    return batch_dispatch(b, shape, dep, true);
+}
+
+int pk_batch_split_launch(pk_batch* b, int shape, int dep) { return batch_dispatch(b, shape, dep, false); }
