@@ -127,6 +127,10 @@ int pk_ens_cells(pk_ens* e, int32_t* cell_counts, int32_t* cell_of_particle);
 /* seeded thermal loader on the device: ppc particles per cell at uniform random positions,
  * u ~ N(0, uth) per component, plus +drift_x on even and -drift_x on odd particle ids. */
 int pk_ens_load_thermal(pk_ens* e, int ppc, double uth, double drift_x, double weight, uint64_t seed);
+/* the same plus nmodes velocity modes, modes[7 m .. 7 m + 6] = (kx, ky, kz, ax, ay, az, phase):
+ * u += (ax, ay, az) sin(2 pi (kx x / Lx + ky y / Ly + kz z / Lz) + phase), L = ncell * step */
+int pk_ens_load_modes(pk_ens* e, int ppc, double uth, double drift_x, double weight, uint64_t seed, int nmodes,
+                      const double* modes);
 
 /* ---- many ensembles stepped together (no counterpart in pica, whose loops are the caller's) ----
  * A batch steps n (ensemble, grid) pairs of one precision and one supercell shape, with uneven
@@ -150,6 +154,45 @@ int pk_batch_run(pk_batch* b, double dt, int shape, int dep, int nsteps);
 /* autotuner: times trial_steps steps of every knob setting from the current state, restoring the
  * state after each trial, and keeps the fastest setting (its ms per step in *best_ms) */
 int pk_batch_autotune(pk_batch* b, double dt, int shape, int dep, int trial_steps, double* best_ms);
+
+/* ---- learned form (no counterpart in pica) ----
+ * A current denoiser: periodic conv3d 3 -> 16 -> 16 -> 3 with 3x3x3 taps and ReLU, BF16 MFMA
+ * with FP32 accumulation, trained with Adam on pairs of (low-ppc J, high-ppc J) taken from plain
+ * runs; applied to the deposited J of a low-ppc run, followed by a projection that restores
+ * discrete charge continuity against the particles' own charge density.
+ * Parameters (FP32, pk_dn_num_params() = 9539 values, in this order): W1[27][3][16], b1[16],
+ * W2[27][16][16], b2[16], W3[27][16][3], b3[3]; tap t = (dx+1)*9 + (dy+1)*3 + (dz+1) reads the
+ * input at (x+dx, y+dy, z+dz).  Host volumes are [3][n0][n1][n2] FP32, z fastest, periodic, in
+ * network units (J times the denoiser's scale). */
+typedef struct pk_dn pk_dn;
+int pk_dn_create(pk_ctx* ctx, pk_dn** out);
+void pk_dn_destroy(pk_dn* d);
+int pk_dn_num_params(void);
+int pk_dn_set_params(pk_dn* d, const float* params);   /* also resets the Adam state */
+int pk_dn_get_params(pk_dn* d, float* params);
+int pk_dn_set_scale(pk_dn* d, double scale);            /* network units = scale x J (CGS) */
+/* one forward pass on a host volume; y = network output [3][n] */
+int pk_dn_forward_host(pk_dn* d, const int n[3], const float* x, float* y);
+/* one convolution layer (1..3) on host input [cin][n]: pre-activation output [cout][n] */
+int pk_dn_layer_host(pk_dn* d, int layer, const int n[3], const float* in, float* out);
+/* gradient of the mean squared error of forward(x) against y; grad has pk_dn_num_params() values */
+int pk_dn_grad_host(pk_dn* d, const int n[3], const float* x, const float* y, float* grad, double* loss);
+/* training pairs: the physical J of two grids of equal size (low-ppc input, high-ppc target) */
+int pk_dn_add_pair(pk_dn* d, pk_grid* low, pk_grid* high);
+int pk_dn_num_pairs(pk_dn* d);
+int pk_dn_clear_pairs(pk_dn* d);
+/* iters Adam steps on mini-batches of `batch` pairs drawn with `seed`; per-step mean loss to losses */
+int pk_dn_train(pk_dn* d, int iters, int batch, double lr, uint64_t seed, float* losses);
+/* replace the physical J of g by the denoised J (ghosts refilled) */
+int pk_dn_apply(pk_dn* d, pk_grid* g);
+/* charge continuity: pk_rho_save keeps the charge density of the ensemble's particles now (FP64);
+ * pk_project_continuity deposits it again and subtracts grad(phi) from J, lap(phi) = div J +
+ * (rho_now - rho_saved)/dt (7-point periodic Laplacian, FFT; power-of-two physical edges), so that
+ * (rho_now - rho_saved)/dt + div J = 0; rho_now becomes the saved density. */
+int pk_rho_save(pk_ens* e, pk_grid* g, int shape);
+int pk_project_continuity(pk_ens* e, pk_grid* g, int shape, double dt);
+/* pk_step with the denoiser and the projection applied to J before the field update */
+int pk_step_denoised(pk_ens* e, pk_grid* g, pk_dn* d, double dt, int shape, int dep);
 
 /* ---- the step ----
  * Fused interpolation + Boris push + deposition for every particle (one workgroup per

@@ -51,7 +51,10 @@ ABI_SYMBOLS = (
     "pk_ens_set_slack", "pk_ens_upload", "pk_ens_download", "pk_ens_size", "pk_ens_num_supercells",
     "pk_ens_cells", "pk_ens_load_thermal", "pk_particle_step", "pk_ens_resort", "pk_step", "pk_run",
     "pk_deposit_rho", "pk_diag", "pk_diag_set_reference", "pk_batch_create", "pk_batch_destroy", "pk_batch_set",
-    "pk_batch_get", "pk_batch_run", "pk_batch_autotune",
+    "pk_batch_get", "pk_batch_run", "pk_batch_autotune", "pk_ens_load_modes", "pk_dn_create", "pk_dn_destroy",
+    "pk_dn_num_params", "pk_dn_set_params", "pk_dn_get_params", "pk_dn_set_scale", "pk_dn_forward_host",
+    "pk_dn_layer_host", "pk_dn_grad_host", "pk_dn_add_pair", "pk_dn_num_pairs", "pk_dn_clear_pairs", "pk_dn_train",
+    "pk_dn_apply", "pk_rho_save", "pk_project_continuity", "pk_step_denoised",
 )
 
 _LIB = None
@@ -117,6 +120,25 @@ def lib():
         L.pk_deposit_rho.argtypes = [vp, vp, ctypes.c_int]
         L.pk_diag.argtypes = [vp, vp, ctypes.c_int, vp]
         L.pk_diag_set_reference.argtypes = [vp, vp, ctypes.c_int]
+        L.pk_ens_load_modes.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_uint64, ctypes.c_int, vp]
+        L.pk_dn_create.argtypes = [vp, ctypes.POINTER(vp)]
+        L.pk_dn_destroy.argtypes = [vp]
+        L.pk_dn_num_params.argtypes = []
+        for name in ("pk_dn_set_params", "pk_dn_get_params"):
+            getattr(L, name).argtypes = [vp, vp]
+        L.pk_dn_set_scale.argtypes = [vp, ctypes.c_double]
+        L.pk_dn_forward_host.argtypes = [vp, vp, vp, vp]
+        L.pk_dn_layer_host.argtypes = [vp, ctypes.c_int, vp, vp, vp]
+        L.pk_dn_grad_host.argtypes = [vp, vp, vp, vp, vp, ctypes.POINTER(ctypes.c_double)]
+        L.pk_dn_add_pair.argtypes = [vp, vp, vp]
+        L.pk_dn_num_pairs.argtypes = [vp]
+        L.pk_dn_clear_pairs.argtypes = [vp]
+        L.pk_dn_train.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_uint64, vp]
+        L.pk_dn_apply.argtypes = [vp, vp]
+        L.pk_rho_save.argtypes = [vp, vp, ctypes.c_int]
+        L.pk_project_continuity.argtypes = [vp, vp, ctypes.c_int, ctypes.c_double]
+        L.pk_step_denoised.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.c_int, ctypes.c_int]
         L.pk_batch_create.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.POINTER(vp)]
         L.pk_batch_destroy.argtypes = [vp]
         L.pk_batch_set.argtypes = [vp, ctypes.c_int, ctypes.c_int]
@@ -306,6 +328,13 @@ class EnsembleSupercells:
         self.ctx.check(lib().pk_ens_load_thermal(self.h, int(ppc), float(uth), float(drift_x), float(weight),
                                                  int(seed) & 0xFFFFFFFFFFFFFFFF))
 
+    def load_modes(self, ppc, uth, drift_x, weight, seed, modes):
+        """load_thermal plus velocity modes: rows (kx, ky, kz, ax, ay, az, phase),
+        u += a sin(2 pi k . x / L + phase)"""
+        m = np.ascontiguousarray(modes, np.float64).reshape(-1, 7)
+        self.ctx.check(lib().pk_ens_load_modes(self.h, int(ppc), float(uth), float(drift_x), float(weight),
+                                               int(seed) & 0xFFFFFFFFFFFFFFFF, m.shape[0], _p(m)))
+
     def size(self):
         n = lib().pk_ens_size(self.h)
         if n < 0:
@@ -392,6 +421,99 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+class Denoiser:
+    """The learned current denoiser (pk_dn_*): periodic conv3d 3 -> 16 -> 16 -> 3, BF16 MFMA.
+    Host volumes are float32 [3][n0][n1][n2] in network units (scale x J)."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        ctx.check(lib().pk_dn_create(ctx.h, ctypes.byref(h)))
+        self.h = h
+        ctx._adopt(self)
+
+    @staticmethod
+    def num_params():
+        return int(lib().pk_dn_num_params())
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.num_params()
+        self.ctx.check(lib().pk_dn_set_params(self.h, _p(p)))
+
+    def params(self):
+        p = np.empty(self.num_params(), np.float32)
+        self.ctx.check(lib().pk_dn_get_params(self.h, _p(p)))
+        return p
+
+    def set_scale(self, scale):
+        self.ctx.check(lib().pk_dn_set_scale(self.h, float(scale)))
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.empty_like(x)
+        self.ctx.check(lib().pk_dn_forward_host(self.h, _i3(x.shape[1:]), _p(x), _p(y)))
+        return y
+
+    def layer(self, k, x):
+        """one convolution layer (1..3) of host input [cin][...]: pre-activation [cout][...]"""
+        x = np.ascontiguousarray(x, np.float32)
+        cout = 3 if k == 3 else 16
+        y = np.empty((cout,) + x.shape[1:], np.float32)
+        self.ctx.check(lib().pk_dn_layer_host(self.h, int(k), _i3(x.shape[1:]), _p(x), _p(y)))
+        return y
+
+    def grad(self, x, y):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        g = np.empty(self.num_params(), np.float32)
+        loss = ctypes.c_double()
+        self.ctx.check(lib().pk_dn_grad_host(self.h, _i3(x.shape[1:]), _p(x), _p(y), _p(g), ctypes.byref(loss)))
+        return loss.value, g
+
+    def add_pair(self, low, high):
+        self.ctx.check(lib().pk_dn_add_pair(self.h, low.h, high.h))
+
+    def num_pairs(self):
+        return int(lib().pk_dn_num_pairs(self.h))
+
+    def clear_pairs(self):
+        self.ctx.check(lib().pk_dn_clear_pairs(self.h))
+
+    def train(self, iters, batch, lr, seed=0):
+        losses = np.empty(max(int(iters), 1), np.float32)
+        self.ctx.check(lib().pk_dn_train(self.h, int(iters), int(batch), float(lr), int(seed) & 0xFFFFFFFFFFFFFFFF,
+                                         _p(losses)))
+        return losses[:int(iters)]
+
+    def apply(self, grid):
+        self.ctx.check(lib().pk_dn_apply(self.h, grid.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            if self.ctx.h:
+                lib().pk_dn_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def rho_save(ens, grid, shape=CIC):
+    ens.ctx.check(lib().pk_rho_save(ens.h, grid.h, shape))
+
+
+def project_continuity(ens, grid, dt, shape=CIC):
+    ens.ctx.check(lib().pk_project_continuity(ens.h, grid.h, shape, float(dt)))
+
+
+def step_denoised(ens, grid, denoiser, dt, shape=CIC, dep=DIRECT):
+    ens.ctx.check(lib().pk_step_denoised(ens.h, grid.h, denoiser.h, float(dt), shape, dep))
 
 
 def particle_step(ens, grid, dt, shape=CIC, dep=DIRECT):

@@ -22,6 +22,8 @@ SOURCES = [
     ("pk_ensemble.hip", ["-ffp-contract=off"]),
     ("pk_diag.hip", ["-ffp-contract=off"]),
     ("pk_batch.hip", ["-ffp-contract=off"]),
+    ("pk_denoise.hip", ["-ffp-contract=fast"]),
+    ("pk_project.hip", ["-ffp-contract=off"]),
 ]
 
 COMMON = [

@@ -248,6 +248,8 @@ void pk_grid_destroy(pk_grid* g)
     if (g->data) (void)hipFree(g->data);
     if (g->scratch) (void)hipFree(g->scratch);
     if (g->gauss_ref) (void)hipFree(g->gauss_ref);
+    if (g->rho_prev) (void)hipFree(g->rho_prev);
+    if (g->proj) (void)hipFree(g->proj);
     delete g;
 }
 
@@ -426,7 +428,14 @@ int pk_ens_cells(pk_ens* e, int32_t* counts, int32_t* cells)
 int pk_ens_load_thermal(pk_ens* e, int ppc, double uth, double drift_x, double weight, uint64_t seed)
 {
     if (!e || ppc <= 0 || !(uth >= 0.0) || !(weight > 0.0)) return PK_E_ARG;
-    return pk_ens_load_impl(e, ppc, uth, drift_x, weight, seed);
+    return pk_ens_load_impl(e, ppc, uth, drift_x, weight, seed, 0, nullptr);
+}
+
+int pk_ens_load_modes(pk_ens* e, int ppc, double uth, double drift_x, double weight, uint64_t seed, int nmodes,
+                      const double* modes)
+{
+    if (!e || ppc <= 0 || !(uth >= 0.0) || !(weight > 0.0) || nmodes < 0 || (nmodes > 0 && !modes)) return PK_E_ARG;
+    return pk_ens_load_impl(e, ppc, uth, drift_x, weight, seed, nmodes, modes);
 }
 
 // ---------------------------------------------------------------- step

@@ -1,0 +1,700 @@
+// pk_denoise.hip -- learned current denoiser: 3 -> 16 -> 16 -> 3 periodic conv3d (3x3x3 taps,
+// ReLU) on BF16 MFMA with FP32 accumulation, its backward pass, Adam, and the C ABI (pk_dn_*).
+// No counterpart in pica.
+//
+// Each convolution is an implicit GEMM: M = voxels, N = output channels, K = taps x input
+// channels.  A wavefront owns 16 consecutive voxels and chains v_mfma_f32_16x16x16_bf16 over K:
+//   A (16 voxels x 16 k): lane l holds A[l % 16][4 (l / 16) .. +3]  -- 4 BF16 of one voxel's
+//                         neighbour (one 8-byte load from the channel-last activation)
+//   B (16 k x 16 cout):   lane l holds B[4 (l / 16) .. +3][l % 16]  -- pre-packed weights,
+//                         held in registers for the whole kernel
+//   D (16 x 16, FP32):    lane l holds D[4 (l / 16) + j][l % 16], j = 0..3
+// With 16 input channels a K-chunk is one tap; with 3 (padded to 4) it is 4 taps x 4 channels
+// (7 chunks, the 28th tap zero).  The input gradient is the same forward kernel with the weights
+// flipped (tap -> 26 - tap) and transposed; the weight gradient is a GEMM over voxels per tap.
+//
+// Tap t = (dx+1)*9 + (dy+1)*3 + (dz+1) reads the input at (x+dx, y+dy, z+dz), periodic.
+// Activations are channel-last BF16 ([voxel][16] or [voxel][4]); the network output and the
+// training pairs are channel-first FP32 ([3][voxel]).
+#include "pk_internal.h"
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int HID = 16;
+constexpr int NPARAM = 27 * 3 * HID + HID + 27 * HID * HID + HID + 27 * HID * 3 + 3;   // 9539
+constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + 27 * 3 * HID, OFF_W2 = OFF_B1 + HID, OFF_B2 = OFF_W2 + 27 * HID * HID,
+              OFF_W3 = OFF_B2 + HID, OFF_B3 = OFF_W3 + 27 * HID * 3;
+
+struct pk_dn_pair {
+    float* low = nullptr;    // [3][nv] normalised input current
+    float* high = nullptr;   // [3][nv] normalised target current
+    int n[3] = {0, 0, 0};
+};
+
+struct pk_dn {
+    pk_ctx* ctx = nullptr;
+    double scale = 1.0;      // J (CGS) -> network units
+    float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;   // parameters, gradient, Adam moments
+    long adam_t = 0;
+    uint16_t* wf[3] = {nullptr, nullptr, nullptr};   // packed forward weights of layers 1..3
+    uint16_t* wt[2] = {nullptr, nullptr};            // packed flipped-transposed W2, W3 (input gradients)
+    // work buffers sized for cap voxels
+    long cap = 0;
+    uint16_t *xb = nullptr, *a1 = nullptr, *a2 = nullptr, *g3 = nullptr, *dz2 = nullptr, *dz1 = nullptr;
+    float *out = nullptr, *tgt = nullptr;
+    double* loss = nullptr;   // device accumulator
+    std::vector<pk_dn_pair> pairs;
+};
+
+namespace {
+
+struct Vol {
+    int n0, n1, n2;
+    long nv;
+};
+
+inline unsigned nblk(long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+__host__ __device__ __forceinline__ uint16_t to_bf16(float f)
+{
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__device__ __forceinline__ void voxel_xyz(const Vol& V, long v, int& x, int& y, int& z)
+{
+    const long n12 = (long)V.n1 * V.n2;
+    x = (int)(v / n12);
+    const int r = (int)(v - (long)x * n12);
+    y = r / V.n2;
+    z = r - y * V.n2;
+}
+
+__device__ __forceinline__ long neighbour(const Vol& V, int x, int y, int z, int tap)
+{
+    int a = x + tap / 9 - 1, b = y + (tap / 3) % 3 - 1, c = z + tap % 3 - 1;
+    a += a < 0 ? V.n0 : 0;
+    a -= a >= V.n0 ? V.n0 : 0;
+    b += b < 0 ? V.n1 : 0;
+    b -= b >= V.n1 ? V.n1 : 0;
+    c += c < 0 ? V.n2 : 0;
+    c -= c >= V.n2 ? V.n2 : 0;
+    return ((long)a * V.n1 + b) * V.n2 + c;
+}
+
+enum { OUT_RELU_BF16 = 0, OUT_F32 = 1, OUT_MASKED_BF16 = 2 };
+
+// one convolution layer; CIN = 16 (one tap per K-chunk) or 4 (4 taps x 4 channels per chunk)
+template <int CIN, int MODE>
+__global__ void __launch_bounds__(256) k_conv(Vol V, const uint16_t* __restrict__ in, const uint16_t* __restrict__ wf,
+                                              const float* __restrict__ bias, int cout, uint16_t* __restrict__ out_b,
+                                              float* __restrict__ out_f, const uint16_t* __restrict__ mask_act)
+{
+    constexpr int KC = CIN == 16 ? 27 : 7;
+    const int lane = threadIdx.x & 63;
+    const int m = lane & 15, q = lane >> 4;
+    bf16x4 B[KC];
+#pragma unroll
+    for (int kc = 0; kc < KC; kc++) B[kc] = *(const bf16x4*)(wf + ((size_t)kc * 64 + lane) * 4);
+    const float bco = (bias && m < cout) ? bias[m] : 0.0f;
+    const long ntiles = (V.nv + 15) / 16;
+    const long wave0 = (long)blockIdx.x * 4 + (threadIdx.x >> 6), nwaves = (long)gridDim.x * 4;
+    for (long t = wave0; t < ntiles; t += nwaves) {
+        long v = t * 16 + m;
+        if (v >= V.nv) v = V.nv - 1;          // padding rows read a valid voxel, never stored
+        int x, y, z;
+        voxel_xyz(V, v, x, y, z);
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int kc = 0; kc < KC; kc++) {
+            bf16x4 a;
+            if (CIN == 16) {
+                a = *(const bf16x4*)(in + neighbour(V, x, y, z, kc) * 16 + 4 * q);
+            } else {
+                const int tap = 4 * kc + q;
+                if (tap < 27) a = *(const bf16x4*)(in + neighbour(V, x, y, z, tap) * 4);
+                else a = bf16x4{0, 0, 0, 0};
+            }
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, B[kc], acc, 0, 0, 0);
+        }
+        const int co = m;
+        if (co >= cout) continue;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const long vo = t * 16 + 4 * q + j;
+            if (vo >= V.nv) continue;
+            const float zv = acc[j] + bco;
+            if (MODE == OUT_RELU_BF16) out_b[vo * 16 + co] = to_bf16(zv > 0.0f ? zv : 0.0f);
+            else if (MODE == OUT_F32) out_f[(long)co * V.nv + vo] = zv;
+            else out_b[vo * 16 + co] = to_bf16(from_bf16(mask_act[vo * 16 + co]) > 0.0f ? zv : 0.0f);
+        }
+    }
+}
+
+// dW[t][ci][co] += sum_v in[v + off(t)][ci] g[v][co].  Rows of the 16 x 16 tile: (tap, ci) --
+// one tap with CIN = 16, four taps x 4 channels with CIN = 4; columns: co.  Waves split voxels.
+template <int CIN>
+__global__ void __launch_bounds__(256) k_wgrad(Vol V, const uint16_t* __restrict__ in, const uint16_t* __restrict__ g,
+                                               int gpad, int cin, int cout, long chunk, float* __restrict__ dW)
+{
+    const int lane = threadIdx.x & 63;
+    const int m = lane & 15, q = lane >> 4;
+    const int group = blockIdx.y;                         // tap (CIN 16) or tap quad (CIN 4)
+    const int tap_m = CIN == 16 ? group : 4 * group + m / 4;
+    const int ci_m = CIN == 16 ? m : m % 4;
+    const bool row_ok = tap_m < 27 && ci_m < cin;
+    const long v_begin = ((long)blockIdx.x * 4 + (threadIdx.x >> 6)) * chunk;
+    const long v_end = v_begin + chunk < V.nv ? v_begin + chunk : V.nv;
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (long v0 = v_begin; v0 < v_end; v0 += 16) {
+        bf16x4 a, b;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const long v = v0 + 4 * q + j;
+            short av = 0, bv = 0;
+            if (v < v_end) {
+                if (row_ok) {
+                    int x, y, z;
+                    voxel_xyz(V, v, x, y, z);
+                    av = (short)in[neighbour(V, x, y, z, tap_m) * CIN + ci_m];
+                }
+                if (m < cout) bv = (short)g[v * gpad + m];
+            }
+            a[j] = av;
+            b[j] = bv;
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    }
+    const int co = m;
+    if (co >= cout) return;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int row = 4 * q + j;
+        const int tap = CIN == 16 ? group : 4 * group + row / 4;
+        const int ci = CIN == 16 ? row : row % 4;
+        if (tap < 27 && ci < cin && acc[j] != 0.0f) atomicAdd(&dW[((long)tap * cin + ci) * cout + co], acc[j]);
+    }
+}
+
+// db[co] += sum_v g[v][co]
+__global__ void __launch_bounds__(256) k_bgrad(long nv, const uint16_t* __restrict__ g, int gpad, int cout,
+                                               float* __restrict__ db)
+{
+    float s[16];
+#pragma unroll
+    for (int c = 0; c < 16; c++) s[c] = 0.0f;
+    for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += (long)gridDim.x * blockDim.x)
+#pragma unroll
+        for (int c = 0; c < 16; c++)
+            if (c < cout) s[c] += from_bf16(g[v * gpad + c]);
+#pragma unroll
+    for (int c = 0; c < 16; c++) {
+        float t = s[c];
+        for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
+        if ((threadIdx.x & 63) == 0 && c < cout && t != 0.0f) atomicAdd(&db[c], t);
+    }
+}
+
+// weights -> MFMA B fragments.  Effective layer: cin_e x cout_e with W_e(t, a, b) = W(t, a, b), or
+// for the input gradient (flip) W(26 - t, b, a); CIN_e = 16 or 4 sets the K-chunk layout.
+__global__ void k_pack(const float* __restrict__ W, int cin, int cout, int flip, int cin_pad, uint16_t* __restrict__ wf)
+{
+    const int kcs = cin_pad == 16 ? 27 : 7;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= kcs * 64 * 4) return;
+    const int j = i & 3, lane = (i >> 2) & 63, kc = i >> 8;
+    const int n = lane & 15, q = lane >> 4;
+    int tap, a;
+    if (cin_pad == 16) {
+        tap = kc;
+        a = 4 * q + j;
+    } else {
+        tap = 4 * kc + q;
+        a = j;
+    }
+    const int cin_e = flip ? cout : cin, cout_e = flip ? cin : cout;
+    float w = 0.0f;
+    if (tap < 27 && a < cin_e && n < cout_e)
+        w = flip ? W[((long)(26 - tap) * cin + n) * cout + a] : W[((long)tap * cin + a) * cout + n];
+    wf[i] = to_bf16(w);
+}
+
+// network input: x_b[v][0..3] = bf16(scale * src[c][v]) (channel 3 zero); src channel-first FP32
+__global__ void k_input_f32(long nv, const float* __restrict__ src, double scale, uint16_t* __restrict__ xb)
+{
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+#pragma unroll
+    for (int c = 0; c < 3; c++) xb[v * 4 + c] = to_bf16((float)(scale * src[(long)c * nv + v]));
+    xb[v * 4 + 3] = 0;
+}
+
+// physical J of a grid -> channel-first FP32 scaled (the network's units)
+template <class T>
+__global__ void k_grid_to_f32(GridView<T> g, Vol V, double scale, float* __restrict__ dst)
+{
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V.nv) return;
+    int x, y, z;
+    voxel_xyz(V, v, x, y, z);
+    const long gi = ((long)(x + g.G) * g.s1 + (y + g.G)) * g.s2 + (z + g.G);
+#pragma unroll
+    for (int c = 0; c < 3; c++) dst[(long)c * V.nv + v] = (float)(scale * (double)g.f[PK_JX + c][gi]);
+}
+
+template <class T>
+__global__ void k_f32_to_grid(GridView<T> g, Vol V, double inv_scale, const float* __restrict__ src)
+{
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V.nv) return;
+    int x, y, z;
+    voxel_xyz(V, v, x, y, z);
+    const long gi = ((long)(x + g.G) * g.s1 + (y + g.G)) * g.s2 + (z + g.G);
+#pragma unroll
+    for (int c = 0; c < 3; c++) g.f[PK_JX + c][gi] = (T)(inv_scale * (double)src[(long)c * V.nv + v]);
+}
+
+// MSE loss and its gradient: g3[v][c] = bf16(2 (out - y) / count), loss += sum (out - y)^2 / count
+__global__ void __launch_bounds__(256) k_loss(long nv, const float* __restrict__ out, const float* __restrict__ y,
+                                              double inv_count, uint16_t* __restrict__ g3, double* __restrict__ loss)
+{
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double s = 0.0;
+    if (v < nv) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float d = out[(long)c * nv + v] - y[(long)c * nv + v];
+            g3[v * 4 + c] = to_bf16((float)(2.0 * inv_count) * d);
+            s += (double)d * d;
+        }
+        g3[v * 4 + 3] = 0;
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if ((threadIdx.x & 63) == 0 && s != 0.0) atomicAdd(loss, s * inv_count);
+}
+
+__global__ void k_adam(int n, float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, float lr, float b1, float b2, float eps, float c1, float c2)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+__global__ void k_zero_f32(long n, float* p)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0f;
+}
+
+__global__ void k_zero_f64(double* p) { p[0] = 0.0; }
+
+// ---------------------------------------------------------------- host side
+int dalloc(pk_ctx* c, void** p, size_t bytes)
+{
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    if (hipMalloc(p, bytes ? bytes : 16) != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return pk_fail(c, PK_E_ALLOC, "denoiser: hipMalloc failed");
+    }
+    return PK_OK;
+}
+
+int ensure_cap(pk_dn* d, long nv)
+{
+    if (nv <= d->cap) return PK_OK;
+    pk_ctx* c = d->ctx;
+    PK_HIP(c, hipStreamSynchronize(c->stream));
+    int r;
+    if ((r = dalloc(c, (void**)&d->xb, sizeof(uint16_t) * 4 * nv))) return r;
+    if ((r = dalloc(c, (void**)&d->a1, sizeof(uint16_t) * 16 * nv))) return r;
+    if ((r = dalloc(c, (void**)&d->a2, sizeof(uint16_t) * 16 * nv))) return r;
+    if ((r = dalloc(c, (void**)&d->g3, sizeof(uint16_t) * 4 * nv))) return r;
+    if ((r = dalloc(c, (void**)&d->dz2, sizeof(uint16_t) * 16 * nv))) return r;
+    if ((r = dalloc(c, (void**)&d->dz1, sizeof(uint16_t) * 16 * nv))) return r;
+    if ((r = dalloc(c, (void**)&d->out, sizeof(float) * 3 * nv))) return r;
+    if ((r = dalloc(c, (void**)&d->tgt, sizeof(float) * 3 * nv))) return r;
+    d->cap = nv;
+    return PK_OK;
+}
+
+// pack all weight layouts from the FP32 parameters
+int pack_weights(pk_dn* d)
+{
+    hipStream_t s = d->ctx->stream;
+    hipLaunchKernelGGL(k_pack, dim3(nblk(7 * 256, 256)), dim3(256), 0, s, d->p + OFF_W1, 3, HID, 0, 4, d->wf[0]);
+    hipLaunchKernelGGL(k_pack, dim3(nblk(27 * 256, 256)), dim3(256), 0, s, d->p + OFF_W2, HID, HID, 0, 16, d->wf[1]);
+    hipLaunchKernelGGL(k_pack, dim3(nblk(27 * 256, 256)), dim3(256), 0, s, d->p + OFF_W3, HID, 3, 0, 16, d->wf[2]);
+    hipLaunchKernelGGL(k_pack, dim3(nblk(27 * 256, 256)), dim3(256), 0, s, d->p + OFF_W2, HID, HID, 1, 16, d->wt[0]);
+    hipLaunchKernelGGL(k_pack, dim3(nblk(7 * 256, 256)), dim3(256), 0, s, d->p + OFF_W3, HID, 3, 1, 4, d->wt[1]);
+    PK_HIP(d->ctx, hipGetLastError());
+    return PK_OK;
+}
+
+unsigned conv_blocks(const Vol& V)
+{
+    const long tiles = (V.nv + 15) / 16;
+    const long b = (tiles + 3) / 4;
+    return (unsigned)(b < 8192 ? b : 8192);
+}
+
+// layers 1..3 on d->xb; the output lands in d->out ([3][nv] FP32)
+int forward(pk_dn* d, const Vol& V)
+{
+    hipStream_t s = d->ctx->stream;
+    const unsigned nb = conv_blocks(V);
+    hipLaunchKernelGGL((k_conv<4, OUT_RELU_BF16>), dim3(nb), dim3(256), 0, s, V, d->xb, d->wf[0], d->p + OFF_B1, HID,
+                       d->a1, (float*)nullptr, (const uint16_t*)nullptr);
+    hipLaunchKernelGGL((k_conv<16, OUT_RELU_BF16>), dim3(nb), dim3(256), 0, s, V, d->a1, d->wf[1], d->p + OFF_B2, HID,
+                       d->a2, (float*)nullptr, (const uint16_t*)nullptr);
+    hipLaunchKernelGGL((k_conv<16, OUT_F32>), dim3(nb), dim3(256), 0, s, V, d->a2, d->wf[2], d->p + OFF_B3, 3,
+                       (uint16_t*)nullptr, d->out, (const uint16_t*)nullptr);
+    PK_HIP(d->ctx, hipGetLastError());
+    return PK_OK;
+}
+
+int wgrad(pk_dn* d, const Vol& V, const uint16_t* in, int cin_pad, int cin, const uint16_t* g, int gpad, int cout,
+          float* dW)
+{
+    const long chunk = 1024;
+    const unsigned bx = (unsigned)((V.nv + 4 * chunk - 1) / (4 * chunk));
+    if (cin_pad == 16)
+        hipLaunchKernelGGL(k_wgrad<16>, dim3(bx, 27), dim3(256), 0, d->ctx->stream, V, in, g, gpad, cin, cout, chunk, dW);
+    else
+        hipLaunchKernelGGL(k_wgrad<4>, dim3(bx, 7), dim3(256), 0, d->ctx->stream, V, in, g, gpad, cin, cout, chunk, dW);
+    PK_HIP(d->ctx, hipGetLastError());
+    return PK_OK;
+}
+
+// loss of d->out against d->tgt (added to *loss) and gradients accumulated into d->g
+int backward(pk_dn* d, const Vol& V, double inv_count, double* loss)
+{
+    hipStream_t s = d->ctx->stream;
+    int r;
+    hipLaunchKernelGGL(k_loss, dim3(nblk(V.nv, 256)), dim3(256), 0, s, V.nv, d->out, d->tgt, inv_count, d->g3, loss);
+    const unsigned nbb = nblk(V.nv, 256) < 1024 ? nblk(V.nv, 256) : 1024;
+    if ((r = wgrad(d, V, d->a2, 16, HID, d->g3, 4, 3, d->g + OFF_W3))) return r;
+    hipLaunchKernelGGL(k_bgrad, dim3(nbb), dim3(256), 0, s, V.nv, d->g3, 4, 3, d->g + OFF_B3);
+    const unsigned nb = conv_blocks(V);
+    hipLaunchKernelGGL((k_conv<4, OUT_MASKED_BF16>), dim3(nb), dim3(256), 0, s, V, d->g3, d->wt[1], (const float*)nullptr,
+                       HID, d->dz2, (float*)nullptr, d->a2);
+    if ((r = wgrad(d, V, d->a1, 16, HID, d->dz2, 16, HID, d->g + OFF_W2))) return r;
+    hipLaunchKernelGGL(k_bgrad, dim3(nbb), dim3(256), 0, s, V.nv, d->dz2, 16, HID, d->g + OFF_B2);
+    hipLaunchKernelGGL((k_conv<16, OUT_MASKED_BF16>), dim3(nb), dim3(256), 0, s, V, d->dz2, d->wt[0],
+                       (const float*)nullptr, HID, d->dz1, (float*)nullptr, d->a1);
+    if ((r = wgrad(d, V, d->xb, 4, 3, d->dz1, 16, HID, d->g + OFF_W1))) return r;
+    hipLaunchKernelGGL(k_bgrad, dim3(nbb), dim3(256), 0, s, V.nv, d->dz1, 16, HID, d->g + OFF_B1);
+    PK_HIP(d->ctx, hipGetLastError());
+    return PK_OK;
+}
+
+int check_dims(pk_dn* d, const int n[3], Vol& V)
+{
+    if (!n) return pk_fail(d->ctx, PK_E_ARG, "denoiser: null dimensions");
+    for (int k = 0; k < 3; k++)
+        if (n[k] < 1 || n[k] > 4096) return pk_fail(d->ctx, PK_E_ARG, "denoiser: volume edge out of range");
+    V.n0 = n[0];
+    V.n1 = n[1];
+    V.n2 = n[2];
+    V.nv = (long)n[0] * n[1] * n[2];
+    return ensure_cap(d, V.nv);
+}
+
+// host [3][nv] FP32 -> d->xb (BF16, channel-last)
+int load_input_host(pk_dn* d, const Vol& V, const float* x)
+{
+    pk_ctx* c = d->ctx;
+    PK_HIP(c, hipMemcpyAsync(d->tgt, x, sizeof(float) * 3 * V.nv, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_input_f32, dim3(nblk(V.nv, 256)), dim3(256), 0, c->stream, V.nv, d->tgt, 1.0, d->xb);
+    PK_HIP(c, hipGetLastError());
+    return PK_OK;
+}
+
+uint64_t mix64(uint64_t z)
+{
+    z += 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+}  // namespace
+
+extern "C" {
+
+int pk_dn_create(pk_ctx* c, pk_dn** out)
+{
+    if (!c || !out) return PK_E_ARG;
+    *out = nullptr;
+    pk_dn* d = new pk_dn();
+    d->ctx = c;
+    int r = 0;
+    for (float** p : {&d->p, &d->g, &d->m, &d->v})
+        if (!r) r = dalloc(c, (void**)p, sizeof(float) * NPARAM);
+    if (!r) r = dalloc(c, (void**)&d->wf[0], sizeof(uint16_t) * 7 * 256);
+    if (!r) r = dalloc(c, (void**)&d->wf[1], sizeof(uint16_t) * 27 * 256);
+    if (!r) r = dalloc(c, (void**)&d->wf[2], sizeof(uint16_t) * 27 * 256);
+    if (!r) r = dalloc(c, (void**)&d->wt[0], sizeof(uint16_t) * 27 * 256);
+    if (!r) r = dalloc(c, (void**)&d->wt[1], sizeof(uint16_t) * 7 * 256);
+    if (!r) r = dalloc(c, (void**)&d->loss, sizeof(double));
+    if (r) {
+        pk_dn_destroy(d);
+        return r;
+    }
+    for (float* p : {d->p, d->g, d->m, d->v})
+        hipLaunchKernelGGL(k_zero_f32, dim3(nblk(NPARAM, 256)), dim3(256), 0, c->stream, (long)NPARAM, p);
+    r = pack_weights(d);
+    if (r) {
+        pk_dn_destroy(d);
+        return r;
+    }
+    *out = d;
+    return PK_OK;
+}
+
+void pk_dn_destroy(pk_dn* d)
+{
+    if (!d) return;
+    (void)hipStreamSynchronize(d->ctx->stream);
+    for (void* p : {(void*)d->p, (void*)d->g, (void*)d->m, (void*)d->v, (void*)d->wf[0], (void*)d->wf[1],
+                    (void*)d->wf[2], (void*)d->wt[0], (void*)d->wt[1], (void*)d->xb, (void*)d->a1, (void*)d->a2,
+                    (void*)d->g3, (void*)d->dz2, (void*)d->dz1, (void*)d->out, (void*)d->tgt, (void*)d->loss})
+        if (p) (void)hipFree(p);
+    for (auto& pr : d->pairs) {
+        if (pr.low) (void)hipFree(pr.low);
+        if (pr.high) (void)hipFree(pr.high);
+    }
+    delete d;
+}
+
+int pk_dn_num_params(void) { return NPARAM; }
+
+int pk_dn_set_params(pk_dn* d, const float* p)
+{
+    if (!d || !p) return PK_E_ARG;
+    pk_ctx* c = d->ctx;
+    PK_HIP(c, hipMemcpyAsync(d->p, p, sizeof(float) * NPARAM, hipMemcpyHostToDevice, c->stream));
+    for (float* q : {d->m, d->v})
+        hipLaunchKernelGGL(k_zero_f32, dim3(nblk(NPARAM, 256)), dim3(256), 0, c->stream, (long)NPARAM, q);
+    d->adam_t = 0;
+    int r = pack_weights(d);
+    if (r) return r;
+    PK_HIP(c, hipStreamSynchronize(c->stream));
+    return PK_OK;
+}
+
+int pk_dn_get_params(pk_dn* d, float* p)
+{
+    if (!d || !p) return PK_E_ARG;
+    PK_HIP(d->ctx, hipMemcpyAsync(p, d->p, sizeof(float) * NPARAM, hipMemcpyDeviceToHost, d->ctx->stream));
+    PK_HIP(d->ctx, hipStreamSynchronize(d->ctx->stream));
+    return PK_OK;
+}
+
+int pk_dn_set_scale(pk_dn* d, double scale)
+{
+    if (!d || !(scale > 0.0)) return PK_E_ARG;
+    d->scale = scale;
+    return PK_OK;
+}
+
+int pk_dn_forward_host(pk_dn* d, const int n[3], const float* x, float* y)
+{
+    if (!d || !x || !y) return PK_E_ARG;
+    Vol V;
+    int r = check_dims(d, n, V);
+    if (r) return r;
+    if ((r = load_input_host(d, V, x))) return r;
+    if ((r = forward(d, V))) return r;
+    PK_HIP(d->ctx, hipMemcpyAsync(y, d->out, sizeof(float) * 3 * V.nv, hipMemcpyDeviceToHost, d->ctx->stream));
+    PK_HIP(d->ctx, hipStreamSynchronize(d->ctx->stream));
+    return PK_OK;
+}
+
+int pk_dn_layer_host(pk_dn* d, int layer, const int n[3], const float* in, float* out)
+{
+    if (!d || !in || !out || layer < 1 || layer > 3) return PK_E_ARG;
+    Vol V;
+    int r = check_dims(d, n, V);
+    if (r) return r;
+    pk_ctx* c = d->ctx;
+    const int cin = layer == 1 ? 3 : HID, cout = layer == 3 ? 3 : HID;
+    // input [cin][nv] FP32 -> BF16 channel-last via host rounding (exactly the device's rounding)
+    const int pad = layer == 1 ? 4 : 16;
+    std::vector<uint16_t> xb((size_t)pad * V.nv, 0);
+    for (long v = 0; v < V.nv; v++)
+        for (int ch = 0; ch < cin; ch++) xb[(size_t)v * pad + ch] = to_bf16(in[(size_t)ch * V.nv + v]);
+    uint16_t* buf = layer == 1 ? d->xb : d->a1;
+    PK_HIP(c, hipMemcpyAsync(buf, xb.data(), sizeof(uint16_t) * xb.size(), hipMemcpyHostToDevice, c->stream));
+    float* res = nullptr;
+    if ((r = dalloc(c, (void**)&res, sizeof(float) * (size_t)cout * V.nv))) return r;
+    const unsigned nb = conv_blocks(V);
+    const float* bias = d->p + (layer == 1 ? OFF_B1 : layer == 2 ? OFF_B2 : OFF_B3);
+    if (layer == 1)
+        hipLaunchKernelGGL((k_conv<4, OUT_F32>), dim3(nb), dim3(256), 0, c->stream, V, buf, d->wf[0], bias, cout,
+                           (uint16_t*)nullptr, res, (const uint16_t*)nullptr);
+    else
+        hipLaunchKernelGGL((k_conv<16, OUT_F32>), dim3(nb), dim3(256), 0, c->stream, V, buf, d->wf[layer - 1], bias,
+                           cout, (uint16_t*)nullptr, res, (const uint16_t*)nullptr);
+    r = hipGetLastError() == hipSuccess ? PK_OK : pk_fail(c, PK_E_HIP, "pk_dn_layer_host: launch failed");
+    if (!r && (hipMemcpyAsync(out, res, sizeof(float) * (size_t)cout * V.nv, hipMemcpyDeviceToHost, c->stream) !=
+                   hipSuccess ||
+               hipStreamSynchronize(c->stream) != hipSuccess))
+        r = pk_fail(c, PK_E_HIP, "pk_dn_layer_host: copy failed");
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(res);
+    return r;
+}
+
+int pk_dn_grad_host(pk_dn* d, const int n[3], const float* x, const float* y, float* grad, double* loss)
+{
+    if (!d || !x || !y || !grad) return PK_E_ARG;
+    Vol V;
+    int r = check_dims(d, n, V);
+    if (r) return r;
+    pk_ctx* c = d->ctx;
+    hipLaunchKernelGGL(k_zero_f32, dim3(nblk(NPARAM, 256)), dim3(256), 0, c->stream, (long)NPARAM, d->g);
+    hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(1), 0, c->stream, d->loss);
+    if ((r = load_input_host(d, V, x))) return r;
+    if ((r = forward(d, V))) return r;
+    PK_HIP(c, hipMemcpyAsync(d->tgt, y, sizeof(float) * 3 * V.nv, hipMemcpyHostToDevice, c->stream));
+    if ((r = backward(d, V, 1.0 / (3.0 * V.nv), d->loss))) return r;
+    PK_HIP(c, hipMemcpyAsync(grad, d->g, sizeof(float) * NPARAM, hipMemcpyDeviceToHost, c->stream));
+    double l = 0.0;
+    PK_HIP(c, hipMemcpyAsync(&l, d->loss, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    PK_HIP(c, hipStreamSynchronize(c->stream));
+    if (loss) *loss = l;
+    return PK_OK;
+}
+
+int pk_dn_add_pair(pk_dn* d, pk_grid* low, pk_grid* high)
+{
+    if (!d || !low || !high) return PK_E_ARG;
+    if (low->ctx != d->ctx || high->ctx != d->ctx) return pk_fail(d->ctx, PK_E_ARG, "pk_dn_add_pair: other context");
+    pk_dn_pair pr;
+    for (int k = 0; k < 3; k++) {
+        pr.n[k] = low->s[k] - 2 * low->ghost;
+        if (high->s[k] - 2 * high->ghost != pr.n[k] || low->prec != high->prec)
+            return pk_fail(d->ctx, PK_E_ARG, "pk_dn_add_pair: grids differ in size or precision");
+    }
+    Vol V{pr.n[0], pr.n[1], pr.n[2], (long)pr.n[0] * pr.n[1] * pr.n[2]};
+    int r;
+    if ((r = dalloc(d->ctx, (void**)&pr.low, sizeof(float) * 3 * V.nv))) return r;
+    if ((r = dalloc(d->ctx, (void**)&pr.high, sizeof(float) * 3 * V.nv))) {
+        (void)hipFree(pr.low);
+        return r;
+    }
+    hipStream_t s = d->ctx->stream;
+    for (int k = 0; k < 2; k++) {
+        pk_grid* g = k ? high : low;
+        float* dst = k ? pr.high : pr.low;
+        if (g->prec == PK_F64)
+            hipLaunchKernelGGL(k_grid_to_f32<double>, dim3(nblk(V.nv, 256)), dim3(256), 0, s, grid_view<double>(g), V,
+                               d->scale, dst);
+        else
+            hipLaunchKernelGGL(k_grid_to_f32<float>, dim3(nblk(V.nv, 256)), dim3(256), 0, s, grid_view<float>(g), V,
+                               d->scale, dst);
+    }
+    PK_HIP(d->ctx, hipGetLastError());
+    d->pairs.push_back(pr);
+    return PK_OK;
+}
+
+int pk_dn_num_pairs(pk_dn* d) { return d ? (int)d->pairs.size() : -1; }
+
+int pk_dn_clear_pairs(pk_dn* d)
+{
+    if (!d) return PK_E_ARG;
+    (void)hipStreamSynchronize(d->ctx->stream);
+    for (auto& pr : d->pairs) {
+        if (pr.low) (void)hipFree(pr.low);
+        if (pr.high) (void)hipFree(pr.high);
+    }
+    d->pairs.clear();
+    return PK_OK;
+}
+
+int pk_dn_train(pk_dn* d, int iters, int batch, double lr, uint64_t seed, float* losses)
+{
+    if (!d || iters < 0 || batch < 1 || !(lr > 0.0)) return PK_E_ARG;
+    if (d->pairs.empty()) return pk_fail(d->ctx, PK_E_STATE, "pk_dn_train: no training pairs");
+    pk_ctx* c = d->ctx;
+    int r;
+    double* dev_losses = nullptr;
+    if ((r = dalloc(c, (void**)&dev_losses, sizeof(double) * (iters > 0 ? iters : 1)))) return r;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    for (int it = 0; it < iters && !r; it++) {
+        hipLaunchKernelGGL(k_zero_f32, dim3(nblk(NPARAM, 256)), dim3(256), 0, c->stream, (long)NPARAM, d->g);
+        hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(1), 0, c->stream, dev_losses + it);
+        for (int b = 0; b < batch && !r; b++) {
+            const pk_dn_pair& pr = d->pairs[mix64(seed ^ mix64(((uint64_t)it << 20) + b)) % d->pairs.size()];
+            Vol V;
+            if ((r = check_dims(d, pr.n, V))) break;
+            hipLaunchKernelGGL(k_input_f32, dim3(nblk(V.nv, 256)), dim3(256), 0, c->stream, V.nv, pr.low, 1.0, d->xb);
+            if ((r = forward(d, V))) break;
+            PK_HIP(c, hipMemcpyAsync(d->tgt, pr.high, sizeof(float) * 3 * V.nv, hipMemcpyDeviceToDevice, c->stream));
+            r = backward(d, V, 1.0 / (3.0 * V.nv * batch), dev_losses + it);
+        }
+        if (r) break;
+        d->adam_t += 1;
+        const float c1 = 1.0f - powf(b1, (float)d->adam_t), c2 = 1.0f - powf(b2, (float)d->adam_t);
+        hipLaunchKernelGGL(k_adam, dim3(nblk(NPARAM, 256)), dim3(256), 0, c->stream, NPARAM, d->p, d->g, d->m, d->v,
+                           (float)lr, b1, b2, eps, c1, c2);
+        if ((r = pack_weights(d))) break;
+    }
+    if (!r && losses && iters > 0) {
+        std::vector<double> h(iters);
+        if (hipMemcpyAsync(h.data(), dev_losses, sizeof(double) * iters, hipMemcpyDeviceToHost, c->stream) !=
+                hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            r = pk_fail(c, PK_E_HIP, "pk_dn_train: loss copy failed");
+        for (int it = 0; it < iters && !r; it++) losses[it] = (float)h[it];
+    }
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(dev_losses);
+    return r;
+}
+
+int pk_dn_apply(pk_dn* d, pk_grid* g)
+{
+    if (!d || !g) return PK_E_ARG;
+    if (g->ctx != d->ctx) return pk_fail(d->ctx, PK_E_ARG, "pk_dn_apply: other context");
+    int n[3];
+    for (int k = 0; k < 3; k++) n[k] = g->s[k] - 2 * g->ghost;
+    Vol V;
+    int r = check_dims(d, n, V);
+    if (r) return r;
+    hipStream_t s = d->ctx->stream;
+    pk_ktimer tm(d->ctx, "denoise");
+    if (g->prec == PK_F64)
+        hipLaunchKernelGGL(k_grid_to_f32<double>, dim3(nblk(V.nv, 256)), dim3(256), 0, s, grid_view<double>(g), V,
+                           d->scale, d->tgt);
+    else
+        hipLaunchKernelGGL(k_grid_to_f32<float>, dim3(nblk(V.nv, 256)), dim3(256), 0, s, grid_view<float>(g), V,
+                           d->scale, d->tgt);
+    hipLaunchKernelGGL(k_input_f32, dim3(nblk(V.nv, 256)), dim3(256), 0, s, V.nv, d->tgt, 1.0, d->xb);
+    if ((r = forward(d, V))) return r;
+    if (g->prec == PK_F64)
+        hipLaunchKernelGGL(k_f32_to_grid<double>, dim3(nblk(V.nv, 256)), dim3(256), 0, s, grid_view<double>(g), V,
+                           1.0 / d->scale, d->out);
+    else
+        hipLaunchKernelGGL(k_f32_to_grid<float>, dim3(nblk(V.nv, 256)), dim3(256), 0, s, grid_view<float>(g), V,
+                           1.0 / d->scale, d->out);
+    PK_HIP(d->ctx, hipGetLastError());
+    if (g->ghost > 0) return pk_field_ghost(g, PK_FILL_J);
+    return PK_OK;
+}
+
+}  // extern "C"

@@ -244,8 +244,10 @@ void launch_rho(pk_ens* e, pk_grid* g, double* rho)
     }
 }
 
+}  // namespace
+
 // FP64 rho (folded, filled) into g->scratch[0 .. n)
-int rho_fp64(pk_ens* e, pk_grid* g, int shape)
+int pk_rho_fp64(pk_ens* e, pk_grid* g, int shape)
 {
     pk_ctx* c = e->ctx;
     int r = ensure_scratch(g, e);
@@ -268,11 +270,9 @@ int rho_fp64(pk_ens* e, pk_grid* g, int shape)
     return r;
 }
 
-}  // namespace
-
 int pk_rho_impl(pk_ens* e, pk_grid* g, int shape)
 {
-    int r = rho_fp64(e, g, shape);
+    int r = pk_rho_fp64(e, g, shape);
     if (r) return r;
     const long n = (long)g->n;
     if (g->prec == PK_F64)
@@ -289,7 +289,7 @@ int pk_rho_impl(pk_ens* e, pk_grid* g, int shape)
 int pk_diag_impl(pk_ens* e, pk_grid* g, int shape, double* out, bool set_ref)
 {
     pk_ctx* c = e->ctx;
-    int r = rho_fp64(e, g, shape);
+    int r = pk_rho_fp64(e, g, shape);
     if (r) return r;
     const long nphys = (long)(g->s[0] - 2 * g->ghost) * (g->s[1] - 2 * g->ghost) * (g->s[2] - 2 * g->ghost);
     if (set_ref && !g->gauss_ref) {

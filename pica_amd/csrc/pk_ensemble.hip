@@ -309,6 +309,8 @@ struct LoadParams {
     double uth, drift, weight;
     uint64_t seed;
     double step[3];
+    int nmodes;              // velocity modes: u += a sin(2 pi k . x / L + phase)
+    const double* modes;     // nmodes x (kx, ky, kz, ax, ay, az, phase), device
 };
 
 __device__ __forceinline__ void thermal_particle(const EnsGeom& eg, const LoadParams& lp, long s, int l, int gc[3],
@@ -334,6 +336,13 @@ __device__ __forceinline__ void thermal_particle(const EnsGeom& eg, const LoadPa
     u[0] = lp.uth * r1 * cos(a1) + ((id & 1ull) ? -lp.drift : lp.drift);
     u[1] = lp.uth * r1 * sin(a1);
     u[2] = lp.uth * r2 * cos(a2);
+    for (int k = 0; k < lp.nmodes; k++) {
+        const double* md = lp.modes + 7 * k;
+        double ph = md[6];
+        for (int d = 0; d < 3; d++) ph += 2.0 * PK_PI * md[d] * ((double)gc[d] + U[d]) / (double)eg.ncell[d];
+        const double sn = sin(ph);
+        for (int d = 0; d < 3; d++) u[d] += md[3 + d] * sn;
+    }
 }
 
 __global__ void __launch_bounds__(256) k_load_f64(EnsGeom eg, LoadParams lp, const int64_t* off, int per_sc, PartF64 pm)
@@ -731,9 +740,15 @@ int pk_ens_upload_impl(pk_ens* e, size_t n, const double* const* host)
     return rebuild_from(e, (long)n, keys, srcs, &st);
 }
 
-int pk_ens_load_impl(pk_ens* e, int ppc, double uth, double drift, double weight, uint64_t seed)
+int pk_ens_load_impl(pk_ens* e, int ppc, double uth, double drift, double weight, uint64_t seed, int nmodes,
+                     const double* modes)
 {
     pk_ctx* c = e->ctx;
+    DBuf dmodes;
+    if (nmodes > 0) {
+        PK_HIP(c, dmodes.alloc(sizeof(double) * 7 * nmodes));
+        PK_HIP(c, hipMemcpy(dmodes.p, modes, sizeof(double) * 7 * nmodes, hipMemcpyHostToDevice));
+    }
     const int per_sc = e->ncps[0] * e->ncps[1] * e->ncps[2] * ppc;
     const long cap = (long)per_sc + (long)(per_sc * e->slack_frac) + e->slack_add;
     if (cap > PK_MAX_SC_PARTICLES)
@@ -752,6 +767,8 @@ int pk_ens_load_impl(pk_ens* e, int ppc, double uth, double drift, double weight
     lp.drift = drift;
     lp.weight = weight;
     lp.seed = seed;
+    lp.nmodes = nmodes;
+    lp.modes = nmodes > 0 ? dmodes.as<double>() : nullptr;
     for (int d = 0; d < 3; d++) lp.step[d] = e->step[d];
     EnsGeom eg = ens_geom(e);
     pk_ktimer tm(c, "load");
@@ -762,6 +779,7 @@ int pk_ens_load_impl(pk_ens* e, int ppc, double uth, double drift, double weight
         hipLaunchKernelGGL(k_load_split, dim3((unsigned)e->nsct), dim3(256), 0, c->stream, eg, lp, e->off, per_sc,
                            part_split(e, e->main));
     PK_HIP(c, hipGetLastError());
+    if (nmodes > 0) PK_HIP(c, hipStreamSynchronize(c->stream));   // before the mode table is freed
     e->maxcap = (int)cap;
     e->np = (int64_t)per_sc * e->nsct;
     e->pending_movers = false;

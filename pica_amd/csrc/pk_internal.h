@@ -57,6 +57,9 @@ struct pk_grid {
     double* scratch = nullptr;  // FP64 reduction scratch (partials + results)
     size_t scratch_n = 0;
     double* gauss_ref = nullptr;  // reference residual for PK_DIAG_GAUSS_DRIFT
+    double* rho_prev = nullptr;   // saved FP64 charge density (n) of the continuity projection
+    double* proj = nullptr;       // projection work: complex physical volume + twiddles
+    size_t proj_n = 0;
     size_t esize() const { return prec == PK_F64 ? 8 : 4; }
     void* comp(int c) const { return (char*)data + (size_t)c * pitch * esize(); }
 };
@@ -238,8 +241,10 @@ int pk_ens_rebuild(pk_ens* e, int source);   // source: 0 = main+outbox, 1 = sta
 int pk_ens_upload_impl(pk_ens* e, size_t n, const double* const* host);
 int pk_ens_download_impl(pk_ens* e, double* const* host, int64_t* off);
 int pk_ens_cells_impl(pk_ens* e, int32_t* counts, int32_t* cells);
-int pk_ens_load_impl(pk_ens* e, int ppc, double uth, double drift, double weight, uint64_t seed);
+int pk_ens_load_impl(pk_ens* e, int ppc, double uth, double drift, double weight, uint64_t seed, int nmodes,
+                     const double* modes);
 int pk_ens_alloc(pk_ens* e, size_t capacity);
 int pk_check_flags(pk_ens* e, bool* overflow);
 int pk_diag_impl(pk_ens* e, pk_grid* g, int shape, double* out, bool set_ref);
 int pk_rho_impl(pk_ens* e, pk_grid* g, int shape);
+int pk_rho_fp64(pk_ens* e, pk_grid* g, int shape);   // folded, filled FP64 rho in g->scratch[0, n)

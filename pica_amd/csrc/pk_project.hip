@@ -1,0 +1,276 @@
+// pk_project.hip -- charge-continuity projection of J, and the PIC step with the learned
+// denoiser (no counterpart in pica).
+//
+// After J has been changed by anything that ignores charge conservation (the denoiser), the
+// projection subtracts a discrete gradient: J <- J - G phi with  L phi = D J + (rho1 - rho0)/dt,
+// where D is the forward-difference divergence at the cell-centre nodes (the stencil under which
+// Esirkepov deposition conserves charge exactly), G = -D^T the backward-difference gradient onto
+// the J faces, and L = D G the 7-point periodic Laplacian.  Then D J = -(rho1 - rho0)/dt.  L is
+// diagonal in the discrete Fourier basis, eigenvalues -sum_d 4 sin^2(pi k_d / N_d) / h_d^2, so
+// phi comes from three passes of batched radix-2 FFTs (one pencil per workgroup, in LDS) in FP64.
+#include <cmath>
+
+#include "pk_internal.h"
+
+int pk_dn_apply(pk_dn* d, pk_grid* g);
+
+namespace {
+
+inline unsigned nblk(long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+struct Phys {
+    int n[3];
+    long nv;
+};
+
+__device__ __forceinline__ long grid_index(const int s1, const int s2, int G, int i, int j, int k)
+{
+    return ((long)(i + G) * s1 + (j + G)) * s2 + (k + G);
+}
+
+// w[v] = (D J + (rho1 - rho0) / dt, 0) at the physical nodes; J ghosts must be filled
+template <class T>
+__global__ void k_residual(GridView<T> g, Phys P, const double* __restrict__ rho1, const double* __restrict__ rho0,
+                           double inv_dt, double2* __restrict__ w)
+{
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= P.nv) return;
+    const int i = (int)(v / ((long)P.n[1] * P.n[2])), j = (int)((v / P.n[2]) % P.n[1]), k = (int)(v % P.n[2]);
+    const long c = grid_index(g.s1, g.s2, g.G, i, j, k);
+    const long s12 = (long)g.s1 * g.s2;
+    double r = ((double)g.f[PK_JX][c + s12] - (double)g.f[PK_JX][c]) / g.step[0];
+    r += ((double)g.f[PK_JY][c + g.s2] - (double)g.f[PK_JY][c]) / g.step[1];
+    r += ((double)g.f[PK_JZ][c + 1] - (double)g.f[PK_JZ][c]) / g.step[2];
+    r += (rho1[c] - rho0[c]) * inv_dt;
+    w[v] = make_double2(r, 0.0);
+}
+
+// in-place radix-2 FFT of the pencils w[base + j * stride], j < n, base = (p / inner) * ostride
+// + p % inner; tw[k] = exp(-2 pi i k / n), k < n / 2; inverse = conjugate twiddles, no 1/n
+__global__ void __launch_bounds__(256) k_fft(double2* __restrict__ w, int n, int logn, long inner, long ostride,
+                                             long stride, const double2* __restrict__ tw, int inverse)
+{
+    extern __shared__ double2 a[];
+    const long p = blockIdx.x;
+    const long base = (p / inner) * ostride + (p % inner);
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        const int rj = (int)(__brev((unsigned)j) >> (32 - logn));
+        a[rj] = w[base + (long)j * stride];
+    }
+    __syncthreads();
+    for (int len = 2; len <= n; len <<= 1) {
+        const int half = len >> 1, tstep = n / len;
+        for (int t = threadIdx.x; t < n / 2; t += blockDim.x) {
+            const int grp = t / half, pos = t - grp * half;
+            const int i0 = grp * len + pos, i1 = i0 + half;
+            double2 wv = tw[pos * tstep];
+            if (inverse) wv.y = -wv.y;
+            const double2 u = a[i0], x = a[i1];
+            const double2 vv = make_double2(x.x * wv.x - x.y * wv.y, x.x * wv.y + x.y * wv.x);
+            a[i0] = make_double2(u.x + vv.x, u.y + vv.y);
+            a[i1] = make_double2(u.x - vv.x, u.y - vv.y);
+        }
+        __syncthreads();
+    }
+    for (int j = threadIdx.x; j < n; j += blockDim.x) w[base + (long)j * stride] = a[j];
+}
+
+// phi^ = r^ / (lambda(k) N), phi^(0) = 0
+__global__ void k_solve(Phys P, const double* __restrict__ lam0, const double* __restrict__ lam1,
+                        const double* __restrict__ lam2, double2* __restrict__ w)
+{
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= P.nv) return;
+    const int i = (int)(v / ((long)P.n[1] * P.n[2])), j = (int)((v / P.n[2]) % P.n[1]), k = (int)(v % P.n[2]);
+    const double lam = lam0[i] + lam1[j] + lam2[k];
+    if (v == 0) {
+        w[v] = make_double2(0.0, 0.0);
+        return;
+    }
+    const double f = 1.0 / (lam * (double)P.nv);
+    w[v] = make_double2(w[v].x * f, w[v].y * f);
+}
+
+// J_d[c] -= (phi[c] - phi[c - e_d]) / h_d on the physical faces
+template <class T>
+__global__ void k_correct(GridView<T> g, Phys P, const double2* __restrict__ w)
+{
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= P.nv) return;
+    const int i = (int)(v / ((long)P.n[1] * P.n[2])), j = (int)((v / P.n[2]) % P.n[1]), k = (int)(v % P.n[2]);
+    const long c = grid_index(g.s1, g.s2, g.G, i, j, k);
+    const double ph = w[v].x;
+    const int im = i == 0 ? P.n[0] - 1 : i - 1, jm = j == 0 ? P.n[1] - 1 : j - 1, km = k == 0 ? P.n[2] - 1 : k - 1;
+    const double px = w[((long)im * P.n[1] + j) * P.n[2] + k].x;
+    const double py = w[((long)i * P.n[1] + jm) * P.n[2] + k].x;
+    const double pz = w[((long)i * P.n[1] + j) * P.n[2] + km].x;
+    g.f[PK_JX][c] = (T)((double)g.f[PK_JX][c] - (ph - px) / g.step[0]);
+    g.f[PK_JY][c] = (T)((double)g.f[PK_JY][c] - (ph - py) / g.step[1]);
+    g.f[PK_JZ][c] = (T)((double)g.f[PK_JZ][c] - (ph - pz) / g.step[2]);
+}
+
+__global__ void k_copy_d(const double* __restrict__ a, double* __restrict__ b, long n)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
+int log2_exact(int n)
+{
+    int l = 0;
+    while ((1 << l) < n) l++;
+    return (1 << l) == n ? l : -1;
+}
+
+// work layout in g->proj (doubles): complex volume (2 nv), twiddles (n0 + n1 + n2), eigenvalues
+// (n0 + n1 + n2)
+int ensure_proj(pk_grid* g, const Phys& P)
+{
+    const size_t need = 2 * (size_t)P.nv + 2 * (size_t)(P.n[0] + P.n[1] + P.n[2]) + 16;
+    if (g->proj && g->proj_n >= need) return PK_OK;
+    PK_HIP(g->ctx, hipStreamSynchronize(g->ctx->stream));
+    if (g->proj) (void)hipFree(g->proj);
+    g->proj = nullptr;
+    g->proj_n = 0;
+    if (hipMalloc((void**)&g->proj, sizeof(double) * need) != hipSuccess) {
+        g->proj = nullptr;
+        (void)hipGetLastError();
+        return pk_fail(g->ctx, PK_E_ALLOC, "projection: allocation failed");
+    }
+    // twiddles exp(-2 pi i k / n), k < n / 2, and eigenvalues per dimension (host FP64)
+    std::vector<double> tab;
+    for (int d = 0; d < 3; d++)
+        for (int k = 0; k < P.n[d] / 2; k++) {
+            const double a = -2.0 * M_PI * k / P.n[d];
+            tab.push_back(cos(a));
+            tab.push_back(sin(a));
+        }
+    for (int d = 0; d < 3; d++)
+        for (int k = 0; k < P.n[d]; k++) {
+            const double s = sin(M_PI * k / P.n[d]);
+            tab.push_back(-4.0 * s * s / (g->step[d] * g->step[d]));
+        }
+    PK_HIP(g->ctx, hipMemcpy(g->proj + 2 * P.nv, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
+    g->proj_n = need;
+    return PK_OK;
+}
+
+int fft3(pk_grid* g, const Phys& P, double2* w, const double2* tw[3], int inverse)
+{
+    hipStream_t s = g->ctx->stream;
+    const long n12 = (long)P.n[1] * P.n[2];
+    // pencils along z (contiguous), y (stride n2), x (stride n1 n2)
+    const long inner[3] = {n12, P.n[2], 1};
+    const long ostride[3] = {0, n12, P.n[2]};
+    const long stride[3] = {n12, P.n[2], 1};
+    for (int dd = 0; dd < 3; dd++) {
+        const int d = inverse ? dd : 2 - dd;
+        const int n = P.n[d];
+        const long npencil = P.nv / n;
+        hipLaunchKernelGGL(k_fft, dim3((unsigned)npencil), dim3(256), sizeof(double2) * n, s, w, n, log2_exact(n),
+                           inner[d], ostride[d] ? ostride[d] : (long)P.nv, stride[d], tw[d], inverse);
+    }
+    PK_HIP(g->ctx, hipGetLastError());
+    return PK_OK;
+}
+
+template <class T>
+int project(pk_grid* g, const Phys& P, const double* rho1, double dt)
+{
+    hipStream_t s = g->ctx->stream;
+    double2* w = (double2*)g->proj;
+    const double2* tw[3];
+    const double* tab = g->proj + 2 * P.nv;
+    tw[0] = (const double2*)tab;
+    tw[1] = tw[0] + P.n[0] / 2;
+    tw[2] = tw[1] + P.n[1] / 2;
+    const double* lam = tab + (P.n[0] + P.n[1] + P.n[2]);
+    hipLaunchKernelGGL(k_residual<T>, dim3(nblk(P.nv, 256)), dim3(256), 0, s, grid_view<T>(g), P, rho1, g->rho_prev,
+                       1.0 / dt, w);
+    int r;
+    if ((r = fft3(g, P, w, tw, 0))) return r;
+    hipLaunchKernelGGL(k_solve, dim3(nblk(P.nv, 256)), dim3(256), 0, s, P, lam, lam + P.n[0], lam + P.n[0] + P.n[1], w);
+    if ((r = fft3(g, P, w, tw, 1))) return r;
+    hipLaunchKernelGGL(k_correct<T>, dim3(nblk(P.nv, 256)), dim3(256), 0, s, grid_view<T>(g), P, w);
+    PK_HIP(g->ctx, hipGetLastError());
+    return PK_OK;
+}
+
+int save_rho(pk_ens* e, pk_grid* g, int shape)
+{
+    int r;
+    if ((r = pk_rho_fp64(e, g, shape))) return r;
+    if (!g->rho_prev && hipMalloc((void**)&g->rho_prev, sizeof(double) * g->n) != hipSuccess) {
+        g->rho_prev = nullptr;
+        (void)hipGetLastError();
+        return pk_fail(g->ctx, PK_E_ALLOC, "pk_rho_save: allocation failed");
+    }
+    hipLaunchKernelGGL(k_copy_d, dim3(nblk((long)g->n, 256)), dim3(256), 0, g->ctx->stream, g->scratch, g->rho_prev,
+                       (long)g->n);
+    PK_HIP(g->ctx, hipGetLastError());
+    return PK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pk_rho_save(pk_ens* e, pk_grid* g, int shape)
+{
+    int r = pk_check_pair(e, g);
+    if (r) return r;
+    if ((r = pk_ens_resort(e))) return r;
+    return save_rho(e, g, shape);
+}
+
+int pk_project_continuity(pk_ens* e, pk_grid* g, int shape, double dt)
+{
+    int r = pk_check_pair(e, g);
+    if (r) return r;
+    if (!(dt > 0.0)) return pk_fail(e->ctx, PK_E_ARG, "pk_project_continuity: dt must be positive");
+    if (!g->rho_prev) return pk_fail(e->ctx, PK_E_STATE, "pk_project_continuity: no saved charge density");
+    Phys P;
+    for (int d = 0; d < 3; d++) {
+        P.n[d] = g->s[d] - 2 * g->ghost;
+        if (P.n[d] < 2 || P.n[d] > 1024 || log2_exact(P.n[d]) < 0)
+            return pk_fail(e->ctx, PK_E_UNSUPPORTED, "pk_project_continuity: physical edges must be powers of two");
+    }
+    P.nv = (long)P.n[0] * P.n[1] * P.n[2];
+    if ((r = pk_ens_resort(e))) return r;
+    if ((r = ensure_proj(g, P))) return r;
+    if ((r = pk_rho_fp64(e, g, shape))) return r;
+    if ((r = pk_field_ghost(g, PK_FILL_J))) return r;
+    {
+        pk_ktimer tm(e->ctx, "project");
+        r = g->prec == PK_F64 ? project<double>(g, P, g->scratch, dt) : project<float>(g, P, g->scratch, dt);
+        if (r) return r;
+    }
+    if ((r = pk_field_ghost(g, PK_FILL_J))) return r;
+    hipLaunchKernelGGL(k_copy_d, dim3(nblk((long)g->n, 256)), dim3(256), 0, g->ctx->stream, g->scratch, g->rho_prev,
+                       (long)g->n);
+    PK_HIP(g->ctx, hipGetLastError());
+    return PK_OK;
+}
+
+int pk_step_denoised(pk_ens* e, pk_grid* g, pk_dn* d, double dt, int shape, int dep)
+{
+    int r = pk_check_pair(e, g);
+    if (r) return r;
+    if (!d) return pk_fail(e->ctx, PK_E_ARG, "pk_step_denoised: no denoiser");
+    if ((r = pk_ens_resort(e))) return r;
+    if ((r = save_rho(e, g, shape))) return r;              // rho at the positions before the push
+    for (int comp = PK_JX; comp <= PK_JZ; comp++)
+        if ((r = pk_field_zero(g, comp))) return r;
+    if ((r = pk_particle_step(e, g, dt, shape, dep))) return r;
+    if ((r = pk_field_ghost(g, PK_FOLD_J | PK_FILL_J))) return r;
+    if ((r = pk_dn_apply(d, g))) return r;
+    if ((r = pk_project_continuity(e, g, shape, dt))) return r;
+    if ((r = pk_field_yee_b(g, dt * 0.5))) return r;
+    if ((r = pk_field_ghost(g, PK_FILL_B))) return r;
+    if ((r = pk_field_yee_e(g, dt))) return r;
+    if ((r = pk_field_ghost(g, PK_FILL_E))) return r;
+    if ((r = pk_field_yee_b(g, dt * 0.5))) return r;
+    return pk_field_ghost(g, PK_FILL_B);
+}
+
+}  // extern "C"

@@ -1,0 +1,163 @@
+"""Learned form on the device against its CPU FP64 oracle (oracle/denoiser.py): each BF16-MFMA
+convolution layer, the whole forward pass, the gradient, the continuity projection, and the
+charge conservation of the denoised PIC step.
+
+Tolerances: a single layer sees identical BF16 inputs and weights on both sides and differs only
+by FP32 vs FP64 accumulation (1e-5 of the output scale); the network and its gradient add BF16
+roundings of intermediate values, where FP32-vs-FP64 differences occasionally flip one BF16 ulp
+(2^-8 relative), so they are compared at 5e-3 (output) and 2e-2 (gradient) of their scale."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+import pica_amd as P
+from helpers import courant_dt, plasma, random_particles, rel_err
+from oracle import denoiser as D
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print("[case] " + msg, flush=True)
+
+
+def _params(seed):
+    rng = np.random.default_rng(seed)
+    p = D.init_params(seed).astype(np.float64)
+    p += 0.05 * rng.standard_normal(p.shape)
+    return p.astype(np.float32)
+
+
+@pytest.fixture()
+def dn(ctx):
+    d = P.Denoiser(ctx)
+    yield d
+    d.close()
+
+
+@pytest.mark.parametrize("layer", [1, 2, 3])
+@pytest.mark.parametrize("n", [(6, 5, 8), (5, 3, 7)])
+def test_conv_layer_matches_oracle(dn, layer, n):
+    _say("conv layer %d n=%s" % (layer, n))
+    p = _params(1)
+    dn.set_params(p)
+    rng = np.random.default_rng(2)
+    cin = 3 if layer == 1 else 16
+    x = rng.standard_normal((cin,) + n).astype(np.float32)
+    got = dn.layer(layer, x)
+    W = D.unpack(p)
+    ref = D.conv3d_periodic(D.bf16(x), D.bf16(W["W%d" % layer]), W["b%d" % layer])
+    err = rel_err(got, ref)
+    print("layer %d n=%s: rel err %.3g" % (layer, n, err), flush=True)
+    assert err < 1e-5
+
+
+def test_forward_matches_oracle(dn):
+    _say("denoiser forward")
+    p = _params(3)
+    dn.set_params(p)
+    x = np.random.default_rng(4).standard_normal((3, 8, 6, 10)).astype(np.float32)
+    got = dn.forward(x)
+    ref, _ = D.net_forward(p, x, emulate=True)
+    err = rel_err(got, ref)
+    print("forward rel err %.3g" % err, flush=True)
+    assert err < 5e-3
+
+
+def test_gradient_matches_oracle(dn):
+    _say("denoiser gradient")
+    p = _params(5)
+    dn.set_params(p)
+    rng = np.random.default_rng(6)
+    x = rng.standard_normal((3, 8, 8, 6)).astype(np.float32)
+    y = rng.standard_normal((3, 8, 8, 6)).astype(np.float32)
+    loss, g = dn.grad(x, y)
+    ref_loss, ref_g = D.net_grad(p, x, y, emulate=True)
+    assert abs(loss - ref_loss) < 1e-4 * abs(ref_loss)
+    o = 0
+    for name, shp in D.SHAPES:
+        k = int(np.prod(shp))
+        err = rel_err(g[o:o + k], ref_g[o:o + k])
+        print("grad %s rel err %.3g" % (name, err), flush=True)
+        assert err < 2e-2, name
+        o += k
+
+
+@pytest.mark.parametrize("prec,tol", [(P.F64, 1e-11), (P.F32, 2e-4)])
+def test_projection_matches_oracle_and_restores_continuity(ctx, prec, tol):
+    """direct (non-conserving) deposition, then the projection: J equals the oracle's projection
+    and the discrete continuity residual drops from O(1) to round-off"""
+    _say("projection prec=%d" % prec)
+    ncell, dt = (16, 8, 16), courant_dt()
+    rng = np.random.default_rng(7)
+    _, w0 = plasma(dt, 4)
+    x, u = random_particles(rng, ncell, 4, uth=0.05, drift=0.1)
+    g = P.YeeGrid.periodic(ctx, ncell, prec=prec)
+    e = P.EnsembleSupercells(ctx, ncell, (1.0, 1.0, 1.0), (4, 4, 4), prec=prec)
+    e.add(x, u, np.full(x.shape[0], w0))
+    P.run(e, g, dt, 2, P.TSC, P.DIRECT)
+    P.deposit_rho(e, g, P.TSC)
+    rho0 = g.get(P.RHO).astype(np.float64)
+    P.rho_save(e, g, P.TSC)
+    for c in (P.JX, P.JY, P.JZ):
+        g.zero(c)
+    P.particle_step(e, g, dt, P.TSC, P.DIRECT)
+    g.ghost_ops(P.FOLD_J | P.FILL_J)
+    j_raw = [g.get(c).astype(np.float64) for c in (P.JX, P.JY, P.JZ)]
+    P.project_continuity(e, g, dt, P.TSC)
+    j_new = [g.get(c).astype(np.float64) for c in (P.JX, P.JY, P.JZ)]
+    P.deposit_rho(e, g, P.TSC)
+    rho1 = g.get(P.RHO).astype(np.float64)
+    G = O.GHOST
+    phys = lambda a: a[G:-G, G:-G, G:-G]
+    ref = D.project_continuity(*[phys(a) for a in j_raw], phys(rho0), phys(rho1), dt, (1.0, 1.0, 1.0))
+    err = max(rel_err(phys(a), b) for a, b in zip(j_new, ref))
+    before = D.divergence_periodic(*[phys(a) for a in j_raw], (1, 1, 1)) + (phys(rho1) - phys(rho0)) / dt
+    after = D.divergence_periodic(*[phys(a) for a in j_new], (1, 1, 1)) + (phys(rho1) - phys(rho0)) / dt
+    scale = np.abs(D.divergence_periodic(*[phys(a) for a in j_raw], (1, 1, 1))).max()
+    print("projection prec=%d: vs oracle %.3g, continuity residual %.3g -> %.3g" %
+          (prec, err, np.abs(before).max() / scale, np.abs(after).max() / scale), flush=True)
+    assert np.abs(before).max() / scale > 1e-3
+    assert np.abs(after).max() / scale < tol
+    assert err < (1e-12 if prec == P.F64 else 1e-5)
+
+
+@pytest.mark.parametrize("prec,tol", [(P.F64, 1e-10), (P.F32, 2e-4)])
+def test_denoised_step_keeps_gauss_law(ctx, dn, prec, tol):
+    """with any denoiser (here untrained, random), the projected current keeps div E - 4 pi rho
+    at its initial value: the learned form cannot break charge conservation"""
+    _say("denoised step gauss prec=%d" % prec)
+    ncell, dt = (16, 16, 16), courant_dt()
+    n0, w0 = plasma(dt, 4)
+    g = P.YeeGrid.periodic(ctx, ncell, prec=prec)
+    e = P.EnsembleSupercells(ctx, ncell, (1.0, 1.0, 1.0), (4, 4, 4), prec=prec)
+    e.load_modes(4, 0.02, 0.0, w0, 11, [[1, 0, 0, 0.05, 0, 0, 0.3], [0, 2, 1, 0, 0.03, 0.02, 1.1]])
+    dn.set_params(_params(8))
+    dn.set_scale(1.0 / (abs(P.ELECTRON_CHARGE) * n0 * P.C * 0.05))
+    P.diag_set_reference(e, g, P.TSC)
+    for _ in range(5):
+        P.step_denoised(e, g, dn, dt, P.TSC, P.ESIRKEPOV)
+    d = P.diag(e, g, P.TSC)
+    drift = d[P.DIAG_GAUSS_DRIFT] / d[P.DIAG_RHO_MAX]
+    print("denoised step prec=%d: gauss drift %.3g" % (prec, drift), flush=True)
+    assert drift < tol
+
+
+def test_mode_loader(ctx):
+    """u - a sin(2 pi k.x / L + phase) is the thermal part: zero mean, standard deviation uth"""
+    _say("mode loader")
+    ncell, uth = (16, 8, 32), 0.01
+    modes = np.array([[1, 0, 2, 0.05, -0.02, 0.01, 0.7], [0, 1, 0, 0.0, 0.03, 0.0, 2.0]])
+    e = P.EnsembleSupercells(ctx, ncell, (1.0, 1.0, 1.0), (4, 4, 4), prec=P.F64)
+    e.load_modes(8, uth, 0.0, 1.0, 5, modes)
+    x, u, w, _ = e.particles()
+    L = np.array(ncell, np.float64)
+    mean = np.zeros_like(u)
+    for m in modes:
+        mean += np.outer(np.sin(2 * math.pi * (x @ (m[:3] / L)) + m[6]), m[3:6])
+    th = u - mean
+    print("mode loader: thermal mean %s, std %s" % (th.mean(0), th.std(0)), flush=True)
+    assert np.all(np.abs(th.mean(0)) < 5 * uth / math.sqrt(x.shape[0]))
+    assert np.all(np.abs(th.std(0) / uth - 1) < 0.02)
