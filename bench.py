@@ -17,6 +17,9 @@ batched: 64 independent periodic plasmas with edges drawn from {32,48,64,96,128}
 {4,8,16,32} (4.8e8 particles, FP32 split positions, TSC, Esirkepov), and 512 small ones (edges
 {16,24,32}, ppc {4,8}), each stepped one by one (pk_step per plasma) and as one batch
 (pk_batch_run after pk_batch_autotune).
+learned: the current denoiser trained on 32^3 pairs of 4- and 256-ppc runs, evaluated at 128^3
+against a 256-ppc reference: J error of 4 ppc raw, denoised, and 64 ppc, and the cost of a 64-ppc
+step over a 4-ppc denoised step.
 """
 import argparse
 import json
@@ -183,6 +186,21 @@ def run_batched(args, specs):
     return out
 
 
+def run_learned():
+    """train the current denoiser on 32^3 pairs (4 vs 256 ppc) and evaluate it at 128^3"""
+    import pica_amd as P
+    from pica_amd import learned
+    ctx = P.Context(0)
+    report, dn = learned.run(ctx)
+    dn.close()
+    ctx.close()
+    ev = report["eval"]
+    print("learned: mean rel L2 vs %d ppc %s, cost ratio %.2f" % (ev["ppc"]["reference"], ev["mean_rel_l2_vs_reference"],
+                                                                  ev["cost"]["mid_over_low_denoised"]),
+          file=sys.stderr, flush=True)
+    return report
+
+
 def cpu_reference(args):
     """pica's own supercell CIC path (oracle/_ref: pica's headers compiled) on a bounded sample:
     pica has no TSC or Esirkepov, so this is its CIC + direct deposition, FP64."""
@@ -260,6 +278,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-events", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-plasma batched workload")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-denoiser workload")
     ap.add_argument("--batch-n", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--kernel", default="auto", choices=["auto", "per_particle", "cell_sorted"],
@@ -314,6 +333,8 @@ def main():
         line["batched_workload"] = run_batched(args, batch_specs(args.batch_n))
         # many small plasmas, where one launch per stage for all of them matters
         line["batched_small_workload"] = run_batched(args, batch_specs(512, 5, (16, 24, 32), (4, 8)))
+    if not args.no_learned:
+        line["learned"] = run_learned()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args)
         line["gpu_vs_cpu"] = line["value"] / line["cpu_baseline"]["value"]

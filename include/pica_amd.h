@@ -169,6 +169,7 @@ int pk_dn_create(pk_ctx* ctx, pk_dn** out);
 void pk_dn_destroy(pk_dn* d);
 int pk_dn_num_params(void);
 int pk_dn_set_params(pk_dn* d, const float* params);   /* also resets the Adam state */
+int pk_dn_init(pk_dn* d, uint64_t seed);                 /* He-normal weights, zero biases */
 int pk_dn_get_params(pk_dn* d, float* params);
 int pk_dn_set_scale(pk_dn* d, double scale);            /* network units = scale x J (CGS) */
 /* one forward pass on a host volume; y = network output [3][n] */

@@ -52,7 +52,7 @@ ABI_SYMBOLS = (
     "pk_ens_cells", "pk_ens_load_thermal", "pk_particle_step", "pk_ens_resort", "pk_step", "pk_run",
     "pk_deposit_rho", "pk_diag", "pk_diag_set_reference", "pk_batch_create", "pk_batch_destroy", "pk_batch_set",
     "pk_batch_get", "pk_batch_run", "pk_batch_autotune", "pk_ens_load_modes", "pk_dn_create", "pk_dn_destroy",
-    "pk_dn_num_params", "pk_dn_set_params", "pk_dn_get_params", "pk_dn_set_scale", "pk_dn_forward_host",
+    "pk_dn_num_params", "pk_dn_init", "pk_dn_set_params", "pk_dn_get_params", "pk_dn_set_scale", "pk_dn_forward_host",
     "pk_dn_layer_host", "pk_dn_grad_host", "pk_dn_add_pair", "pk_dn_num_pairs", "pk_dn_clear_pairs", "pk_dn_train",
     "pk_dn_apply", "pk_rho_save", "pk_project_continuity", "pk_step_denoised",
 )
@@ -128,6 +128,7 @@ def lib():
         for name in ("pk_dn_set_params", "pk_dn_get_params"):
             getattr(L, name).argtypes = [vp, vp]
         L.pk_dn_set_scale.argtypes = [vp, ctypes.c_double]
+        L.pk_dn_init.argtypes = [vp, ctypes.c_uint64]
         L.pk_dn_forward_host.argtypes = [vp, vp, vp, vp]
         L.pk_dn_layer_host.argtypes = [vp, ctypes.c_int, vp, vp, vp]
         L.pk_dn_grad_host.argtypes = [vp, vp, vp, vp, vp, ctypes.POINTER(ctypes.c_double)]
@@ -450,6 +451,10 @@ class Denoiser:
 
     def set_scale(self, scale):
         self.ctx.check(lib().pk_dn_set_scale(self.h, float(scale)))
+
+    def init(self, seed):
+        """He-normal weights, zero biases"""
+        self.ctx.check(lib().pk_dn_init(self.h, int(seed) & 0xFFFFFFFFFFFFFFFF))
 
     def forward(self, x):
         x = np.ascontiguousarray(x, np.float32)

@@ -2,12 +2,14 @@
 // denoiser (no counterpart in pica).
 //
 // After J has been changed by anything that ignores charge conservation (the denoiser), the
-// projection subtracts a discrete gradient: J <- J - G phi with  L phi = D J + (rho1 - rho0)/dt,
+// projection subtracts a discrete gradient: J <- J - G phi with  L phi = D J + (rho1 - rho0)/dt
+// (or, when the current before the change was Esirkepov's, L phi = D (J - J_esirkepov)),
 // where D is the forward-difference divergence at the cell-centre nodes (the stencil under which
 // Esirkepov deposition conserves charge exactly), G = -D^T the backward-difference gradient onto
 // the J faces, and L = D G the 7-point periodic Laplacian.  Then D J = -(rho1 - rho0)/dt.  L is
 // diagonal in the discrete Fourier basis, eigenvalues -sum_d 4 sin^2(pi k_d / N_d) / h_d^2, so
-// phi comes from three passes of batched radix-2 FFTs (one pencil per workgroup, in LDS) in FP64.
+// phi comes from three passes of batched radix-2 FFTs (16 pencils per workgroup, in LDS), in the
+// grid's precision.
 #include <cmath>
 
 #include "pk_internal.h"
@@ -23,6 +25,13 @@ struct Phys {
     long nv;
 };
 
+// complex value of the projection's work volume: FP64 for FP64 grids, FP32 for FP32 grids (the
+// correction is stored in the grid's precision either way)
+template <class R>
+struct alignas(2 * sizeof(R)) Cx {
+    R x, y;
+};
+
 __device__ __forceinline__ long grid_index(const int s1, const int s2, int G, int i, int j, int k)
 {
     return ((long)(i + G) * s1 + (j + G)) * s2 + (k + G);
@@ -31,7 +40,7 @@ __device__ __forceinline__ long grid_index(const int s1, const int s2, int G, in
 // w[v] = (D J + (rho1 - rho0) / dt, 0) at the physical nodes; J ghosts must be filled
 template <class T>
 __global__ void k_residual(GridView<T> g, Phys P, const double* __restrict__ rho1, const double* __restrict__ rho0,
-                           double inv_dt, double2* __restrict__ w)
+                           double inv_dt, Cx<T>* __restrict__ w)
 {
     const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= P.nv) return;
@@ -42,68 +51,130 @@ __global__ void k_residual(GridView<T> g, Phys P, const double* __restrict__ rho
     r += ((double)g.f[PK_JY][c + g.s2] - (double)g.f[PK_JY][c]) / g.step[1];
     r += ((double)g.f[PK_JZ][c + 1] - (double)g.f[PK_JZ][c]) / g.step[2];
     r += (rho1[c] - rho0[c]) * inv_dt;
-    w[v] = make_double2(r, 0.0);
+    w[v] = Cx<T>{(T)r, (T)0};
 }
 
-// in-place radix-2 FFT of the pencils w[base + j * stride], j < n, base = (p / inner) * ostride
-// + p % inner; tw[k] = exp(-2 pi i k / n), k < n / 2; inverse = conjugate twiddles, no 1/n
-__global__ void __launch_bounds__(256) k_fft(double2* __restrict__ w, int n, int logn, long inner, long ostride,
-                                             long stride, const double2* __restrict__ tw, int inverse)
+// w[v] = (D (J - Jref), 0): the residual when Jref (saved physical J, FP64, [3][nv]) satisfies
+// continuity exactly, as the Esirkepov current does
+template <class T>
+__global__ void k_residual_ref(GridView<T> g, Phys P, const double* __restrict__ jref, Cx<T>* __restrict__ w)
 {
-    extern __shared__ double2 a[];
-    const long p = blockIdx.x;
-    const long base = (p / inner) * ostride + (p % inner);
-    for (int j = threadIdx.x; j < n; j += blockDim.x) {
-        const int rj = (int)(__brev((unsigned)j) >> (32 - logn));
-        a[rj] = w[base + (long)j * stride];
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= P.nv) return;
+    const int i = (int)(v / ((long)P.n[1] * P.n[2])), j = (int)((v / P.n[2]) % P.n[1]), k = (int)(v % P.n[2]);
+    const long c = grid_index(g.s1, g.s2, g.G, i, j, k);
+    const long s12 = (long)g.s1 * g.s2;
+    const int ip = i + 1 == P.n[0] ? 0 : i + 1, jp = j + 1 == P.n[1] ? 0 : j + 1, kp = k + 1 == P.n[2] ? 0 : k + 1;
+    const long vx = ((long)ip * P.n[1] + j) * P.n[2] + k, vy = ((long)i * P.n[1] + jp) * P.n[2] + k,
+               vz = ((long)i * P.n[1] + j) * P.n[2] + kp;
+    double r = (((double)g.f[PK_JX][c + s12] - jref[vx]) - ((double)g.f[PK_JX][c] - jref[v])) / g.step[0];
+    r += (((double)g.f[PK_JY][c + g.s2] - jref[P.nv + vy]) - ((double)g.f[PK_JY][c] - jref[P.nv + v])) / g.step[1];
+    r += (((double)g.f[PK_JZ][c + 1] - jref[2 * P.nv + vz]) - ((double)g.f[PK_JZ][c] - jref[2 * P.nv + v])) / g.step[2];
+    w[v] = Cx<T>{(T)r, (T)0};
+}
+
+template <class T>
+__global__ void k_save_j(GridView<T> g, Phys P, double* __restrict__ jref)
+{
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= P.nv) return;
+    const int i = (int)(v / ((long)P.n[1] * P.n[2])), j = (int)((v / P.n[2]) % P.n[1]), k = (int)(v % P.n[2]);
+    const long c = grid_index(g.s1, g.s2, g.G, i, j, k);
+#pragma unroll
+    for (int d = 0; d < 3; d++) jref[d * P.nv + v] = (double)g.f[PK_JX + d][c];
+}
+
+// in-place radix-2 FFTs of pb pencils per workgroup: pencil p holds w[base(p) + j * stride],
+// j < n, base(p) = (p / inner) * ostride + p % inner (n, pb, inner powers of two, passed as
+// logarithms); tw[k] = exp(-2 pi i k / n), k < n / 2;
+// inverse = conjugate twiddles, no 1/n.  `along` = 1 when a pencil is contiguous (z), else the
+// pencils of a workgroup are adjacent in memory and are read across pencils (coalesced).
+template <class R>
+__global__ void __launch_bounds__(256) k_fft(Cx<R>* __restrict__ w, int logn, long npencil, int logpb, int loginner,
+                                             long ostride, long stride, int along, const double2* __restrict__ tw,
+                                             int inverse)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    Cx<R>* a = (Cx<R>*)smem;          // pb x n
+    const int n = 1 << logn, pb = 1 << logpb;
+    const long inner_mask = (1l << loginner) - 1;
+    const long p0 = (long)blockIdx.x << logpb;
+    const int tot = pb << logn;
+    constexpr int CH = 8;             // loads in flight per thread before their LDS writes
+    for (int i0 = threadIdx.x; i0 < tot; i0 += CH * 256) {
+        Cx<R> v[CH];
+#pragma unroll
+        for (int k = 0; k < CH; k++) {
+            const int idx = i0 + 256 * k;
+            const int pl = along ? idx >> logn : idx & (pb - 1), j = along ? idx & (n - 1) : idx >> logpb;
+            const long p = p0 + pl;
+            if (idx < tot && p < npencil)
+                v[k] = w[((p >> loginner) * ostride + (p & inner_mask)) + (long)j * stride];
+        }
+#pragma unroll
+        for (int k = 0; k < CH; k++) {
+            const int idx = i0 + 256 * k;
+            const int pl = along ? idx >> logn : idx & (pb - 1), j = along ? idx & (n - 1) : idx >> logpb;
+            if (idx < tot && p0 + pl < npencil)
+                a[(pl << logn) + (int)(__brev((unsigned)j) >> (32 - logn))] = v[k];
+        }
     }
     __syncthreads();
-    for (int len = 2; len <= n; len <<= 1) {
-        const int half = len >> 1, tstep = n / len;
-        for (int t = threadIdx.x; t < n / 2; t += blockDim.x) {
-            const int grp = t / half, pos = t - grp * half;
-            const int i0 = grp * len + pos, i1 = i0 + half;
-            double2 wv = tw[pos * tstep];
-            if (inverse) wv.y = -wv.y;
-            const double2 u = a[i0], x = a[i1];
-            const double2 vv = make_double2(x.x * wv.x - x.y * wv.y, x.x * wv.y + x.y * wv.x);
-            a[i0] = make_double2(u.x + vv.x, u.y + vv.y);
-            a[i1] = make_double2(u.x - vv.x, u.y - vv.y);
+    const int nb = pb << (logn - 1);
+    for (int lh = 0; lh < logn; lh++) {               // half = 2^lh, len = 2 half
+        const int half = 1 << lh;
+        for (int t = threadIdx.x; t < nb; t += blockDim.x) {
+            const int pl = t >> (logn - 1), tt = t & ((n >> 1) - 1);
+            const int grp = tt >> lh, pos = tt & (half - 1);
+            const int i0 = (pl << logn) + (grp << (lh + 1)) + pos, i1 = i0 + half;
+            const double2 wd = tw[pos << (logn - 1 - lh)];
+            const R wx = (R)wd.x, wy = inverse ? (R)-wd.y : (R)wd.y;
+            const Cx<R> u = a[i0], x = a[i1];
+            const R vx = x.x * wx - x.y * wy, vy = x.x * wy + x.y * wx;
+            a[i0] = Cx<R>{u.x + vx, u.y + vy};
+            a[i1] = Cx<R>{u.x - vx, u.y - vy};
         }
         __syncthreads();
     }
-    for (int j = threadIdx.x; j < n; j += blockDim.x) w[base + (long)j * stride] = a[j];
+    for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
+        const int pl = along ? idx >> logn : idx & (pb - 1), j = along ? idx & (n - 1) : idx >> logpb;
+        const long p = p0 + pl;
+        if (p >= npencil) continue;
+        const long base = (p >> loginner) * ostride + (p & inner_mask);
+        w[base + (long)j * stride] = a[(pl << logn) + j];
+    }
 }
 
 // phi^ = r^ / (lambda(k) N), phi^(0) = 0
+template <class R>
 __global__ void k_solve(Phys P, const double* __restrict__ lam0, const double* __restrict__ lam1,
-                        const double* __restrict__ lam2, double2* __restrict__ w)
+                        const double* __restrict__ lam2, Cx<R>* __restrict__ w)
 {
     const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= P.nv) return;
     const int i = (int)(v / ((long)P.n[1] * P.n[2])), j = (int)((v / P.n[2]) % P.n[1]), k = (int)(v % P.n[2]);
     const double lam = lam0[i] + lam1[j] + lam2[k];
     if (v == 0) {
-        w[v] = make_double2(0.0, 0.0);
+        w[v] = Cx<R>{(R)0, (R)0};
         return;
     }
-    const double f = 1.0 / (lam * (double)P.nv);
-    w[v] = make_double2(w[v].x * f, w[v].y * f);
+    const R f = (R)(1.0 / (lam * (double)P.nv));
+    w[v] = Cx<R>{w[v].x * f, w[v].y * f};
 }
 
 // J_d[c] -= (phi[c] - phi[c - e_d]) / h_d on the physical faces
 template <class T>
-__global__ void k_correct(GridView<T> g, Phys P, const double2* __restrict__ w)
+__global__ void k_correct(GridView<T> g, Phys P, const Cx<T>* __restrict__ w)
 {
     const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= P.nv) return;
     const int i = (int)(v / ((long)P.n[1] * P.n[2])), j = (int)((v / P.n[2]) % P.n[1]), k = (int)(v % P.n[2]);
     const long c = grid_index(g.s1, g.s2, g.G, i, j, k);
-    const double ph = w[v].x;
+    const double ph = (double)w[v].x;
     const int im = i == 0 ? P.n[0] - 1 : i - 1, jm = j == 0 ? P.n[1] - 1 : j - 1, km = k == 0 ? P.n[2] - 1 : k - 1;
-    const double px = w[((long)im * P.n[1] + j) * P.n[2] + k].x;
-    const double py = w[((long)i * P.n[1] + jm) * P.n[2] + k].x;
-    const double pz = w[((long)i * P.n[1] + j) * P.n[2] + km].x;
+    const double px = (double)w[((long)im * P.n[1] + j) * P.n[2] + k].x;
+    const double py = (double)w[((long)i * P.n[1] + jm) * P.n[2] + k].x;
+    const double pz = (double)w[((long)i * P.n[1] + j) * P.n[2] + km].x;
     g.f[PK_JX][c] = (T)((double)g.f[PK_JX][c] - (ph - px) / g.step[0]);
     g.f[PK_JY][c] = (T)((double)g.f[PK_JY][c] - (ph - py) / g.step[1]);
     g.f[PK_JZ][c] = (T)((double)g.f[PK_JZ][c] - (ph - pz) / g.step[2]);
@@ -123,10 +194,12 @@ int log2_exact(int n)
 }
 
 // work layout in g->proj (doubles): complex volume (2 nv), twiddles (n0 + n1 + n2), eigenvalues
-// (n0 + n1 + n2)
+// (n0 + n1 + n2), reference current (3 nv)
+double* jref_of(pk_grid* g, const Phys& P) { return g->proj + 2 * P.nv + 2 * (P.n[0] + P.n[1] + P.n[2]); }
+
 int ensure_proj(pk_grid* g, const Phys& P)
 {
-    const size_t need = 2 * (size_t)P.nv + 2 * (size_t)(P.n[0] + P.n[1] + P.n[2]) + 16;
+    const size_t need = 5 * (size_t)P.nv + 2 * (size_t)(P.n[0] + P.n[1] + P.n[2]) + 16;
     if (g->proj && g->proj_n >= need) return PK_OK;
     PK_HIP(g->ctx, hipStreamSynchronize(g->ctx->stream));
     if (g->proj) (void)hipFree(g->proj);
@@ -155,44 +228,65 @@ int ensure_proj(pk_grid* g, const Phys& P)
     return PK_OK;
 }
 
-int fft3(pk_grid* g, const Phys& P, double2* w, const double2* tw[3], int inverse)
+template <class R>
+int fft3(pk_grid* g, const Phys& P, Cx<R>* w, const double2* tw[3], int inverse)
 {
     hipStream_t s = g->ctx->stream;
     const long n12 = (long)P.n[1] * P.n[2];
-    // pencils along z (contiguous), y (stride n2), x (stride n1 n2)
+    // pencils along x (stride n1 n2), y (stride n2), z (contiguous)
     const long inner[3] = {n12, P.n[2], 1};
-    const long ostride[3] = {0, n12, P.n[2]};
+    const long ostride[3] = {P.nv, n12, P.n[2]};
     const long stride[3] = {n12, P.n[2], 1};
     for (int dd = 0; dd < 3; dd++) {
         const int d = inverse ? dd : 2 - dd;
         const int n = P.n[d];
         const long npencil = P.nv / n;
-        hipLaunchKernelGGL(k_fft, dim3((unsigned)npencil), dim3(256), sizeof(double2) * n, s, w, n, log2_exact(n),
-                           inner[d], ostride[d] ? ostride[d] : (long)P.nv, stride[d], tw[d], inverse);
+        const int pb = n <= 256 ? 16 : 4096 / n;          // at most 64 KB of LDS
+        hipLaunchKernelGGL(k_fft<R>, dim3((unsigned)((npencil + pb - 1) / pb)), dim3(256), sizeof(Cx<R>) * pb * n, s,
+                           w, log2_exact(n), npencil, log2_exact(pb), log2_exact((int)inner[d]), ostride[d], stride[d],
+                           d == 2 ? 1 : 0, tw[d], inverse);
     }
     PK_HIP(g->ctx, hipGetLastError());
     return PK_OK;
 }
 
+// rho1 != null: residual D J + (rho1 - rho_prev) / dt; else D (J - jref) with jref after the
+// complex volume in g->proj
 template <class T>
 int project(pk_grid* g, const Phys& P, const double* rho1, double dt)
 {
     hipStream_t s = g->ctx->stream;
-    double2* w = (double2*)g->proj;
+    Cx<T>* w = (Cx<T>*)g->proj;
     const double2* tw[3];
     const double* tab = g->proj + 2 * P.nv;
     tw[0] = (const double2*)tab;
     tw[1] = tw[0] + P.n[0] / 2;
     tw[2] = tw[1] + P.n[1] / 2;
     const double* lam = tab + (P.n[0] + P.n[1] + P.n[2]);
-    hipLaunchKernelGGL(k_residual<T>, dim3(nblk(P.nv, 256)), dim3(256), 0, s, grid_view<T>(g), P, rho1, g->rho_prev,
-                       1.0 / dt, w);
+    if (rho1)
+        hipLaunchKernelGGL(k_residual<T>, dim3(nblk(P.nv, 256)), dim3(256), 0, s, grid_view<T>(g), P, rho1,
+                           g->rho_prev, 1.0 / dt, w);
+    else
+        hipLaunchKernelGGL(k_residual_ref<T>, dim3(nblk(P.nv, 256)), dim3(256), 0, s, grid_view<T>(g), P,
+                           jref_of(g, P), w);
     int r;
     if ((r = fft3(g, P, w, tw, 0))) return r;
-    hipLaunchKernelGGL(k_solve, dim3(nblk(P.nv, 256)), dim3(256), 0, s, P, lam, lam + P.n[0], lam + P.n[0] + P.n[1], w);
+    hipLaunchKernelGGL(k_solve<T>, dim3(nblk(P.nv, 256)), dim3(256), 0, s, P, lam, lam + P.n[0],
+                       lam + P.n[0] + P.n[1], w);
     if ((r = fft3(g, P, w, tw, 1))) return r;
     hipLaunchKernelGGL(k_correct<T>, dim3(nblk(P.nv, 256)), dim3(256), 0, s, grid_view<T>(g), P, w);
     PK_HIP(g->ctx, hipGetLastError());
+    return PK_OK;
+}
+
+int phys_of(pk_grid* g, Phys& P)
+{
+    for (int d = 0; d < 3; d++) {
+        P.n[d] = g->s[d] - 2 * g->ghost;
+        if (P.n[d] < 2 || P.n[d] > 1024 || log2_exact(P.n[d]) < 0)
+            return pk_fail(g->ctx, PK_E_UNSUPPORTED, "continuity projection: physical edges must be powers of two");
+    }
+    P.nv = (long)P.n[0] * P.n[1] * P.n[2];
     return PK_OK;
 }
 
@@ -230,12 +324,7 @@ int pk_project_continuity(pk_ens* e, pk_grid* g, int shape, double dt)
     if (!(dt > 0.0)) return pk_fail(e->ctx, PK_E_ARG, "pk_project_continuity: dt must be positive");
     if (!g->rho_prev) return pk_fail(e->ctx, PK_E_STATE, "pk_project_continuity: no saved charge density");
     Phys P;
-    for (int d = 0; d < 3; d++) {
-        P.n[d] = g->s[d] - 2 * g->ghost;
-        if (P.n[d] < 2 || P.n[d] > 1024 || log2_exact(P.n[d]) < 0)
-            return pk_fail(e->ctx, PK_E_UNSUPPORTED, "pk_project_continuity: physical edges must be powers of two");
-    }
-    P.nv = (long)P.n[0] * P.n[1] * P.n[2];
+    if ((r = phys_of(g, P))) return r;
     if ((r = pk_ens_resort(e))) return r;
     if ((r = ensure_proj(g, P))) return r;
     if ((r = pk_rho_fp64(e, g, shape))) return r;
@@ -257,14 +346,37 @@ int pk_step_denoised(pk_ens* e, pk_grid* g, pk_dn* d, double dt, int shape, int 
     int r = pk_check_pair(e, g);
     if (r) return r;
     if (!d) return pk_fail(e->ctx, PK_E_ARG, "pk_step_denoised: no denoiser");
+    Phys P;
+    if ((r = phys_of(g, P))) return r;
     if ((r = pk_ens_resort(e))) return r;
-    if ((r = save_rho(e, g, shape))) return r;              // rho at the positions before the push
+    // the direct current needs the charge density before the push; the Esirkepov current is its
+    // own continuity reference (D J_esirkepov = -(rho1 - rho0) / dt exactly)
+    if (dep != PK_ESIRKEPOV && (r = save_rho(e, g, shape))) return r;
     for (int comp = PK_JX; comp <= PK_JZ; comp++)
         if ((r = pk_field_zero(g, comp))) return r;
     if ((r = pk_particle_step(e, g, dt, shape, dep))) return r;
     if ((r = pk_field_ghost(g, PK_FOLD_J | PK_FILL_J))) return r;
-    if ((r = pk_dn_apply(d, g))) return r;
-    if ((r = pk_project_continuity(e, g, shape, dt))) return r;
+    if (dep == PK_ESIRKEPOV) {
+        if ((r = ensure_proj(g, P))) return r;
+        if (g->prec == PK_F64)
+            hipLaunchKernelGGL(k_save_j<double>, dim3(nblk(P.nv, 256)), dim3(256), 0, g->ctx->stream,
+                               grid_view<double>(g), P, jref_of(g, P));
+        else
+            hipLaunchKernelGGL(k_save_j<float>, dim3(nblk(P.nv, 256)), dim3(256), 0, g->ctx->stream,
+                               grid_view<float>(g), P, jref_of(g, P));
+        PK_HIP(e->ctx, hipGetLastError());
+        if ((r = pk_dn_apply(d, g))) return r;
+        {
+            pk_ktimer tm(e->ctx, "project");
+            r = g->prec == PK_F64 ? project<double>(g, P, nullptr, dt) : project<float>(g, P, nullptr, dt);
+            if (r) return r;
+        }
+        if ((r = pk_field_ghost(g, PK_FILL_J))) return r;
+        if ((r = pk_ens_resort(e))) return r;
+    } else {
+        if ((r = pk_dn_apply(d, g))) return r;
+        if ((r = pk_project_continuity(e, g, shape, dt))) return r;
+    }
     if ((r = pk_field_yee_b(g, dt * 0.5))) return r;
     if ((r = pk_field_ghost(g, PK_FILL_B))) return r;
     if ((r = pk_field_yee_e(g, dt))) return r;
