@@ -412,205 +412,17 @@ struct F64Args {
     StepParams sp;
 };
 
-// one supercell s of one ensemble
-template <int SHAPE, int DEP>
-__device__ __forceinline__ void particles_f64_sc(const F64Args& A, long s)
-{
-    const GridView<double>& g = A.g;
-    const EnsGeom& eg = A.eg;
-    const PartF64& pm = A.pm;
-    const PartF64& po = A.po;
-    uint8_t* __restrict__ odir = A.odir;
-    const int64_t* __restrict__ off = A.off;
-    const int32_t* __restrict__ cnt = A.cnt;
-    int32_t* __restrict__ nstay = A.nstay;
-    int32_t* __restrict__ nmove = A.nmove;
-    int32_t* __restrict__ mvc = A.mvc;
-    int32_t* __restrict__ flags = A.flags;
-    const StepParams& sp = A.sp;
-    extern __shared__ __align__(16) unsigned char smem[];
-    const int sidx[3] = {(int)(s / ((long)eg.nsc[1] * eg.nsc[2])), (int)((s / eg.nsc[2]) % eg.nsc[1]),
-                         (int)(s % eg.nsc[2])};
-    ScGeom q;
-    sc_geom<SHAPE, DEP>(g, eg, sidx, q);
-    const int itn = q.its[0] * q.its[1] * q.its[2];
-    const int dtn = q.dts[0] * q.dts[1] * q.dts[2];
-    double* itile = (double*)smem;
-    double* jtile = itile + 6 * itn;
-    uint64_t* mask = (uint64_t*)(jtile + 3 * dtn);
-    int* wtot = (int*)(mask + sp.mask_words);
-    int* dircnt = wtot + WAVES;            // 27
-    int* bad = dircnt + 27;                // 1 (WAVES + 27 + 1 = 32 ints keeps wpre 8-byte aligned)
-    long* wpre = (long*)(wtot + 32);       // prefix of mover bits per mask word (mask_words + 1)
-
-    const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
-    // 1. gather the field stencil (6 components), zero J tile and bookkeeping
-    for (int t = tid; t < 6 * itn; t += PK_BLOCK) {
-        const int comp = t / itn, r = t % itn;
-        const int i = r / (q.its[1] * q.its[2]), j = (r / q.its[2]) % q.its[1], k = r % q.its[2];
-        itile[t] = g.f[PK_EX + comp][((long)(q.ibase[0] + i) * g.s1 + (q.ibase[1] + j)) * g.s2 + (q.ibase[2] + k)];
-    }
-    for (int t = tid; t < 3 * dtn; t += PK_BLOCK) jtile[t] = 0.0;
-    for (int t = tid; t < 27; t += PK_BLOCK) dircnt[t] = 0;
-    if (tid == 0) bad[0] = 0;
-    __syncthreads();
-
-    const long base = off[s];
-    const int n = cnt[s];
-    int running = 0;   // movers so far
-    for (int c0 = 0; c0 < n; c0 += PK_BLOCK) {
-        const int i = c0 + tid;
-        const bool act = i < n;
-        double x[3], u[3], w = 0.0;
-        int code = 13;
-        bool ok = true;
-        if (act) {
-            const long p = base + i;
-            x[0] = pm.x[p]; x[1] = pm.y[p]; x[2] = pm.z[p];
-            u[0] = pm.ux[p]; u[1] = pm.uy[p]; u[2] = pm.uz[p];
-            w = pm.w[p];
-            const double x0[3] = {x[0], x[1], x[2]};
-            double ef[3], bf[3], v[3];
-            ok = isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2]) && interp<SHAPE>(q, itile, x, ef, bf);
-            if (ok) {
-                boris(sp.coeff, sp.dt, ef, bf, u, x, v);
-                ok = isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2]);
-            }
-            if (ok) {
-                const double qw = sp.q * w;
-                if (DEP == PK_DIRECT) {
-                    double cur[3];
-#pragma unroll
-                    for (int d = 0; d < 3; d++) cur[d] = v[d] * qw;
-                    ok = deposit_direct<SHAPE>(q, jtile, x, cur);
-                } else {
-                    ok = deposit_esirkepov<SHAPE>(q, jtile, x0, x, qw, sp.dt, g.step);
-                }
-            }
-            if (ok) {
-                // periodic wrap and direction class of the destination supercell
-                code = 0;
-#pragma unroll
-                for (int d = 0; d < 3; d++) {
-                    if (x[d] < 0.0) x[d] += sp.L[d];
-                    if (x[d] >= sp.L[d]) x[d] -= sp.L[d];
-                    int dst = (int)floor((x[d] - 0.0) * sp.scinv[d]);
-                    dst = dst < 0 ? 0 : (dst > eg.nsc[d] - 1 ? eg.nsc[d] - 1 : dst);
-                    int dd = dst - sidx[d];
-                    if (dd > 1) dd -= eg.nsc[d];
-                    if (dd < -1) dd += eg.nsc[d];
-                    if (dd < -1 || dd > 1) ok = false;
-                    code = code * 3 + (dd + 1);
-                }
-            }
-            if (!ok) {
-                // leave the particle untouched and report; nothing outside the tiles is written
-                x[0] = x0[0]; x[1] = x0[1]; x[2] = x0[2];
-                u[0] = pm.ux[p]; u[1] = pm.uy[p]; u[2] = pm.uz[p];
-                code = 13;
-                bad[0] = 1;
-            }
-        }
-        const bool mover = act && code != 13;
-        const uint64_t mb = __ballot(mover);
-        if (lane == 0) {
-            wtot[wave] = __popcll(mb);
-            mask[(c0 >> 6) + wave] = mb;
-        }
-        __syncthreads();
-        int pre = running, tot = 0;
-#pragma unroll
-        for (int v = 0; v < WAVES; v++) {
-            pre += (v < wave) ? wtot[v] : 0;
-            tot += wtot[v];
-        }
-        if (act) {
-            const long p = base + i;
-            if (!mover) {
-                pm.x[p] = x[0]; pm.y[p] = x[1]; pm.z[p] = x[2];
-                pm.ux[p] = u[0]; pm.uy[p] = u[1]; pm.uz[p] = u[2];
-            } else {
-                const int rank = pre + __popcll(mb & ((1ull << lane) - 1ull));
-                const long o = base + rank;
-                po.x[o] = x[0]; po.y[o] = x[1]; po.z[o] = x[2];
-                po.ux[o] = u[0]; po.uy[o] = u[1]; po.uz[o] = u[2];
-                po.w[o] = w;
-                odir[o] = (uint8_t)code;
-                atomicAdd(&dircnt[code], 1);
-            }
-        }
-        running += tot;
-        __syncthreads();
-    }
-    const int nmv = running;
-    const int nfinal = n - nmv;
-    // 3. swap-fill: the k-th stayer at a slot >= nfinal fills the k-th hole below nfinal
-    const int words = (n + 63) >> 6;
-    if (nmv > 0) {
-        if (tid == 0) {
-            long acc = 0;
-            for (int wd = 0; wd < words; wd++) {
-                wpre[wd] = acc;
-                acc += __popcll(mask[wd]);
-            }
-            wpre[words] = acc;
-        }
-        __syncthreads();
-        // movers strictly below slot y
-        auto movers_below = [&](int y) -> long {
-            const int wd = y >> 6, bt = y & 63;
-            return wpre[wd] + (bt ? __popcll(mask[wd] & ((1ull << bt) - 1ull)) : 0);
-        };
-        const long below_final = movers_below(nfinal);
-        for (int t = tid; t < nmv; t += PK_BLOCK) {
-            const int j = nfinal + t;
-            if ((mask[j >> 6] >> (j & 63)) & 1ull) continue;    // mover slot, nothing to move
-            const long r = (long)t - (movers_below(j) - below_final);   // rank among tail stayers
-            // select the r-th mover slot (ascending): word by binary search on wpre, then bit
-            int lo = 0, hi = words - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (wpre[mid] <= r) lo = mid; else hi = mid - 1;
-            }
-            uint64_t m = mask[lo];
-            for (long z = r - wpre[lo]; z > 0; z--) m &= m - 1ull;
-            if (m == 0ull) {            // cannot happen for a consistent mask; never write outside
-                bad[0] = 1;
-                continue;
-            }
-            const int h = (lo << 6) + (__ffsll((long long)m) - 1);
-            const long src = base + j, dst = base + h;
-            pm.x[dst] = pm.x[src]; pm.y[dst] = pm.y[src]; pm.z[dst] = pm.z[src];
-            pm.ux[dst] = pm.ux[src]; pm.uy[dst] = pm.uy[src]; pm.uz[dst] = pm.uz[src];
-            pm.w[dst] = pm.w[src];
-        }
-    }
-    __syncthreads();
-    // 4. flush the J tile x 1/V (CurrentDepositorYeeGrid.h:201-213)
-    const double normalization = 1.0 / (g.step[0] * g.step[1] * g.step[2]);
-    for (int t = tid; t < 3 * dtn; t += PK_BLOCK) {
-        const double v = jtile[t];
-        if (v == 0.0) continue;
-        const int comp = t / dtn, r = t % dtn;
-        const int i = r / (q.dts[1] * q.dts[2]), j = (r / q.dts[2]) % q.dts[1], k = r % q.dts[2];
-        atomicAdd(&g.f[PK_JX + comp][((long)(q.dbase[0] + i) * g.s1 + (q.dbase[1] + j)) * g.s2 + (q.dbase[2] + k)],
-                  v * normalization);
-    }
-    if (tid < 27) mvc[s * 27 + tid] = dircnt[tid];
-    if (tid == 0) {
-        nstay[s] = nfinal;
-        nmove[s] = nmv;
-        if (bad[0]) atomicOr(&flags[PK_FLAG_ERR], 1);
-    }
-}
 
 template <int SHAPE, int DEP>
-__global__ void __launch_bounds__(PK_BLOCK) k_particles(F64Args A)
+__global__ void __launch_bounds__(PK_BLOCK) k_particles(GridView<double> g, EnsGeom eg, PartF64 pm, PartF64 po,
+                                                        uint8_t* __restrict__ odir, const int64_t* __restrict__ off,
+                                                        const int32_t* __restrict__ cnt, int32_t* __restrict__ nstay,
+                                                        int32_t* __restrict__ nmove, int32_t* __restrict__ mvc,
+                                                        int32_t* __restrict__ flags, StepParams sp)
 {
-    const long s = pk_xcd_supercell(blockIdx.x, A.eg.xcd_chunk);
-    if (s >= A.eg.nsct) return;
-    particles_f64_sc<SHAPE, DEP>(A, s);
+    const long s = pk_xcd_supercell(blockIdx.x, eg.xcd_chunk);
+    if (s >= eg.nsct) return;
+#include "pk_particles_f64_body.inc"
 }
 
 // many ensembles: a list of (ensemble, supercell) items; gridDim.x < nitems makes it persistent
@@ -623,7 +435,21 @@ __global__ void __launch_bounds__(PK_BLOCK) k_particles_batch(const F64Args* __r
     for (long w = blockIdx.x; w < nitems; w += gridDim.x) {
         __syncthreads();
         const int2 it = items[w];
-        particles_f64_sc<SHAPE, DEP>(jobs[it.x], it.y);     // arguments read in place (a copy would live in scratch)
+        const long s = it.y;
+        const F64Args& A = jobs[it.x];      // arguments read in place (a copy would live in scratch)
+        const GridView<double>& g = A.g;
+        const EnsGeom& eg = A.eg;
+        const PartF64& pm = A.pm;
+        const PartF64& po = A.po;
+        uint8_t* __restrict__ odir = A.odir;
+        const int64_t* __restrict__ off = A.off;
+        const int32_t* __restrict__ cnt = A.cnt;
+        int32_t* __restrict__ nstay = A.nstay;
+        int32_t* __restrict__ nmove = A.nmove;
+        int32_t* __restrict__ mvc = A.mvc;
+        int32_t* __restrict__ flags = A.flags;
+        const StepParams& sp = A.sp;
+#include "pk_particles_f64_body.inc"
     }
 }
 
@@ -684,7 +510,8 @@ int launch(pk_ens* e, pk_grid* g, double dt)
     const unsigned nblocks = (unsigned)(PK_XCD * (long)eg.xcd_chunk);
     const F64Args A = f64_args(e, g, eg, sp);
     pk_ktimer tm(e->ctx, "particles");
-    hipLaunchKernelGGL((k_particles<SHAPE, DEP>), dim3(nblocks), dim3(PK_BLOCK), lds, e->ctx->stream, A);
+    hipLaunchKernelGGL((k_particles<SHAPE, DEP>), dim3(nblocks), dim3(PK_BLOCK), lds, e->ctx->stream, A.g, A.eg, A.pm,
+                       A.po, A.odir, A.off, A.cnt, A.nstay, A.nmove, A.mvc, A.flags, A.sp);
     PK_HIP(e->ctx, hipGetLastError());
     return PK_OK;
 }
