@@ -23,6 +23,7 @@
 namespace {
 
 constexpr int WAVES = PK_BLOCK / 64;
+constexpr int REC_WORDS = 8;     // deposit record: packed cell/shift/class, o[3], on[3], q w
 
 __device__ __forceinline__ bool colloc(int comp, int d)
 {
@@ -322,6 +323,10 @@ __device__ __forceinline__ void particles_split_sc(const SplitArgs& A, long s)
     int* dircnt = wtot + WAVES;
     int* bad = dircnt + 27;
     long* wpre = (long*)(wtot + 32);
+    // Esirkepov only: this chunk's deposit records ranked by window class, and per-wave class counts
+    const int rec_off = (int)(((size_t)((unsigned char*)(wpre + sp.mask_words + 1) - smem) + 15) & ~(size_t)15);
+    int* rec = (int*)(smem + rec_off);                  // PK_BLOCK x REC_WORDS
+    int* ccnt = (int*)(smem + rec_off + 4 * REC_WORDS * PK_BLOCK);   // WAVES x 9
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -374,10 +379,56 @@ __device__ __forceinline__ void particles_split_sc(const SplitArgs& A, long s)
         // ---- deposition
         const float qw = sp.q * w;
         if (DEP == PK_ESIRKEPOV) {
+            // A wavefront deposits on the window its widest lane needs, so one cell-changing lane
+            // costs the whole wavefront the larger window.  Rank the chunk's deposits by window
+            // class (bit d: the particle changes node in dimension d) through LDS: the particles
+            // that need larger windows then share as few wavefronts as possible.
+            int cls = 8;                                  // 8: nothing to deposit
+            if (ok)
+                cls = (node_change<SHAPE>(o[0], on[0], sh[0]) ? 4 : 0) |
+                      (node_change<SHAPE>(o[1], on[1], sh[1]) ? 2 : 0) | (node_change<SHAPE>(o[2], on[2], sh[2]) ? 1 : 0);
+            int rank = 0;
+#pragma unroll
+            for (int c = 0; c < 9; c++) {
+                const uint64_t bc = __ballot(cls == c);
+                if (lane == 0) ccnt[wave * 9 + c] = __popcll(bc);
+                if (cls == c) rank = __popcll(bc & ((1ull << lane) - 1ull));
+            }
+            __syncthreads();
+            int slot = rank, nrec = 0;
+            for (int c = 0; c < 9; c++)
+#pragma unroll
+                for (int vv = 0; vv < WAVES; vv++) {
+                    const int k = ccnt[vv * 9 + c];
+                    if (c < cls || (c == cls && vv < wave)) slot += k;
+                    if (c < 8) nrec += k;
+                }
+            if (cls < 8) {
+                int* r = rec + slot * REC_WORDS;
+                r[0] = l[0] | (l[1] << 4) | (l[2] << 8) | ((sh[0] + 1) << 12) | ((sh[1] + 1) << 14) |
+                       ((sh[2] + 1) << 16) | (cls << 18);
+                r[1] = __float_as_int(o[0]); r[2] = __float_as_int(o[1]); r[3] = __float_as_int(o[2]);
+                r[4] = __float_as_int(on[0]); r[5] = __float_as_int(on[1]); r[6] = __float_as_int(on[2]);
+                r[7] = __float_as_int(qw);
+            }
+            __syncthreads();
+            const bool has = tid < nrec;
+            int dl[3] = {0, 0, 0}, dsh[3] = {0, 0, 0}, dcls = 0;
+            float d_o[3] = {0.5f, 0.5f, 0.5f}, d_on[3] = {0.5f, 0.5f, 0.5f}, dqw = 0.0f;
+            if (has) {
+                const int* r = rec + tid * REC_WORDS;
+                const int w0 = r[0];
+                dl[0] = w0 & 15; dl[1] = (w0 >> 4) & 15; dl[2] = (w0 >> 8) & 15;
+                dsh[0] = ((w0 >> 12) & 3) - 1; dsh[1] = ((w0 >> 14) & 3) - 1; dsh[2] = ((w0 >> 16) & 3) - 1;
+                dcls = (w0 >> 18) & 7;
+                d_o[0] = __int_as_float(r[1]); d_o[1] = __int_as_float(r[2]); d_o[2] = __int_as_float(r[3]);
+                d_on[0] = __int_as_float(r[4]); d_on[1] = __int_as_float(r[5]); d_on[2] = __int_as_float(r[6]);
+                dqw = __int_as_float(r[7]);
+            }
             bool big[3];
 #pragma unroll
-            for (int d = 0; d < 3; d++) big[d] = __ballot(ok && node_change<SHAPE>(o[d], on[d], sh[d])) != 0ull;
-            if (ok) esirkepov_window<SHAPE>(jtile, dtn, dsx, dsy, l, sh, o, on, big, qw, sp.hdt);
+            for (int d = 0; d < 3; d++) big[d] = __ballot(has && ((dcls >> (2 - d)) & 1)) != 0ull;
+            if (has) esirkepov_window<SHAPE>(jtile, dtn, dsx, dsy, dl, dsh, d_o, d_on, big, dqw, sp.hdt);
         } else if (ok) {
             // direct deposition of q w v at the new position: the CIC tile is pica's (base m+1,
             // the interpolation indices), the TSC tile has base m (indices + 1)
@@ -1002,8 +1053,10 @@ size_t split_lds(const pk_ens* e, int shape, int dep, int mask_words)
         dts[d] = e->ncps[d] + ((shape == PK_CIC && dep == PK_DIRECT) ? 3 : 4);
     }
     const size_t joff = (sizeof(float) * 6 * its[0] * its[1] * its[2] + 7) & ~(size_t)7;
-    return joff + sizeof(double) * 3 * dts[0] * dts[1] * dts[2] + sizeof(uint64_t) * mask_words + sizeof(int) * 32 +
-           sizeof(long) * (mask_words + 1) + 16;
+    const size_t base = joff + sizeof(double) * 3 * dts[0] * dts[1] * dts[2] + sizeof(uint64_t) * mask_words +
+                        sizeof(int) * 32 + sizeof(long) * (mask_words + 1);
+    if (dep != PK_ESIRKEPOV) return base + 16;
+    return ((base + 15) & ~(size_t)15) + sizeof(int) * (REC_WORDS * PK_BLOCK + WAVES * 9) + 16;
 }
 
 SplitArgs split_args(pk_ens* e, pk_grid* g, const EnsGeom& eg, const SplitParams& sp)
