@@ -395,14 +395,17 @@ __device__ __forceinline__ void particles_split_sc(const SplitArgs& A, long s)
                 if (cls == c) rank = __popcll(bc & ((1ull << lane) - 1ull));
             }
             __syncthreads();
-            int slot = rank, nrec = 0;
-            for (int c = 0; c < 9; c++)
+            // every wavefront scans the 9 x WAVES counts in (class, wave) order with shuffles
+            static_assert(9 * WAVES <= 64, "one count per lane");
+            const int kc = lane < 9 * WAVES ? ccnt[(lane % WAVES) * 9 + lane / WAVES] : 0;
+            int incl = kc;
 #pragma unroll
-                for (int vv = 0; vv < WAVES; vv++) {
-                    const int k = ccnt[vv * 9 + c];
-                    if (c < cls || (c == cls && vv < wave)) slot += k;
-                    if (c < 8) nrec += k;
-                }
+            for (int o2 = 1; o2 < 64; o2 <<= 1) {
+                const int y = __shfl_up(incl, o2);
+                if (lane >= o2) incl += y;
+            }
+            const int slot = rank + __shfl(incl - kc, cls * WAVES + wave);
+            const int nrec = __shfl(incl, 8 * WAVES - 1);
             if (cls < 8) {
                 int* r = rec + slot * REC_WORDS;
                 r[0] = l[0] | (l[1] << 4) | (l[2] << 8) | ((sh[0] + 1) << 12) | ((sh[1] + 1) << 14) |
