@@ -17,7 +17,8 @@ batched: 64 independent periodic plasmas with edges drawn from {32,48,64,96,128}
 {4,8,16,32} (4.8e8 particles, FP32 split positions, TSC, Esirkepov), and 512 small ones (edges
 {16,24,32}, ppc {4,8}), each stepped one by one (pk_step per plasma) and as one batch
 (pk_batch_run after pk_batch_autotune).
-learned: the current denoiser trained on 32^3 pairs of 4- and 256-ppc runs, evaluated at 128^3
+learned: the current denoiser trained on 32^3 pairs of 4- and 256-ppc runs (48 runs x 10 snapshots,
+8,000 Adam steps), evaluated at 128^3
 against a 256-ppc reference: J error of 4 ppc raw, denoised, and 64 ppc, and the cost of a 64-ppc
 step over a 4-ppc denoised step.
 """
@@ -282,8 +283,18 @@ def main():
     ap.add_argument("--batch-n", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--kernel", default="auto", choices=["auto", "per_particle", "cell_sorted"],
-                    help="FP32 Esirkepov particle kernel (auto: cell-sorted when it fits)")
+                    help="FP32 Esirkepov particle kernel (auto: the per-particle kernel)")
+    ap.add_argument("--copy-gib", type=float, default=0.0,
+                    help="only copy this many GiB device-to-device once (counter calibration for profiles/)")
     args = ap.parse_args()
+    if args.copy_gib > 0:
+        import torch
+        a = torch.ones(int(args.copy_gib * 2 ** 28), device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print("copied %.3f GiB" % args.copy_gib, file=sys.stderr)
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (multi-GPU domain decomposition is design-only); "
               "run with --gpus 1", file=sys.stderr)

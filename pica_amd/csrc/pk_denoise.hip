@@ -3,15 +3,17 @@
 // No counterpart in pica.
 //
 // Each convolution is an implicit GEMM: M = voxels, N = output channels, K = taps x input
-// channels.  A wavefront owns 16 consecutive voxels and chains v_mfma_f32_16x16x16_bf16 over K:
-//   A (16 voxels x 16 k): lane l holds A[l % 16][4 (l / 16) .. +3]  -- 4 BF16 of one voxel's
-//                         neighbour (one 8-byte load from the channel-last activation)
-//   B (16 k x 16 cout):   lane l holds B[4 (l / 16) .. +3][l % 16]  -- pre-packed weights,
+// channels.  A wavefront owns 16 consecutive voxels and chains gfx950's v_mfma_f32_16x16x32_bf16
+// over K:
+//   A (16 voxels x 32 k): lane l holds A[l % 16][8 (l / 16) .. +7]  -- 8 BF16 of neighbours of
+//                         one voxel (one 16-byte LDS read, or two 8-byte ones)
+//   B (32 k x 16 cout):   lane l holds B[8 (l / 16) .. +7][l % 16]  -- pre-packed weights,
 //                         held in registers for the whole kernel
 //   D (16 x 16, FP32):    lane l holds D[4 (l / 16) + j][l % 16], j = 0..3
-// With 16 input channels a K-chunk is one tap; with 3 (padded to 4) it is 4 taps x 4 channels
-// (7 chunks, the 28th tap zero).  The input gradient is the same forward kernel with the weights
-// flipped (tap -> 26 - tap) and transposed; the weight gradient is a GEMM over voxels per tap.
+// With 16 input channels a K-chunk is 2 taps x 16 channels (14 chunks, the 28th tap zero); with
+// 3 (padded to 4) it is 8 taps x 4 channels (4 chunks).  The input gradient is the same forward
+// kernel with the weights flipped (tap -> 26 - tap) and transposed; the weight gradient is a GEMM
+// over voxels per tap on v_mfma_f32_16x16x16_bf16.
 //
 // Tap t = (dx+1)*9 + (dy+1)*3 + (dz+1) reads the input at (x+dx, y+dy, z+dz), periodic.
 // Activations are channel-last BF16 ([voxel][16] or [voxel][4]); the network output and the
@@ -23,10 +25,12 @@
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef short bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16v8 __attribute__((ext_vector_type(8)));   // operand type of the 16x16x32 MFMA
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int HID = 16;
 constexpr int NPARAM = 27 * 3 * HID + HID + 27 * HID * HID + HID + 27 * HID * 3 + 3;   // 9539
+constexpr int WF16 = 14 * 64 * 8, WF4 = 4 * 64 * 8;   // packed B fragments (BF16 values) per layer
 constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + 27 * 3 * HID, OFF_W2 = OFF_B1 + HID, OFF_B2 = OFF_W2 + 27 * HID * HID,
               OFF_W3 = OFF_B2 + HID, OFF_B3 = OFF_W3 + 27 * HID * 3;
 
@@ -70,13 +74,14 @@ __host__ __device__ __forceinline__ uint16_t to_bf16(float f)
 
 __device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
+// volumes hold fewer than 2^31 voxels (check_dims), so 32-bit unsigned division suffices
 __device__ __forceinline__ void voxel_xyz(const Vol& V, long v, int& x, int& y, int& z)
 {
-    const long n12 = (long)V.n1 * V.n2;
-    x = (int)(v / n12);
-    const int r = (int)(v - (long)x * n12);
-    y = r / V.n2;
-    z = r - y * V.n2;
+    const unsigned n12 = (unsigned)V.n1 * (unsigned)V.n2, u = (unsigned)v;
+    x = (int)(u / n12);
+    const unsigned r = u - (unsigned)x * n12;
+    y = (int)(r / (unsigned)V.n2);
+    z = (int)(r - (unsigned)y * (unsigned)V.n2);
 }
 
 __device__ __forceinline__ long neighbour(const Vol& V, int x, int y, int z, int tap)
@@ -96,7 +101,7 @@ enum { OUT_RELU_BF16 = 0, OUT_F32 = 1, OUT_MASKED_BF16 = 2 };
 // One convolution layer over a brick of BX x BY x BZ voxels per workgroup: the brick and its
 // one-voxel periodic halo are staged in LDS (each input voxel leaves L2 about 2.8 times instead
 // of 27), then every wavefront runs 16-voxel z-rows of the brick through the MFMA chain.
-// CIN = 16 (one tap per K-chunk) or 4 (4 taps x 4 channels per chunk).
+// CIN = 16 (2 taps x 16 channels per K-chunk) or 4 (8 taps x 4 channels per chunk).
 constexpr int BX = 2, BY = 8, BZ = 16, HX = BX + 2, HY = BY + 2, HZ = BZ + 2;
 constexpr int BRICK_TILES = BX * BY * (BZ / 16);
 static_assert(BRICK_TILES % 8 == 0, "4 wavefronts x 2 rows per pass");
@@ -115,7 +120,7 @@ __global__ void __launch_bounds__(256) k_conv(Vol V, const uint16_t* __restrict_
                                               const float* __restrict__ bias, int cout, uint16_t* __restrict__ out_b,
                                               float* __restrict__ out_f, const uint16_t* __restrict__ mask_act)
 {
-    constexpr int KC = CIN == 16 ? 27 : 7;
+    constexpr int KC = CIN == 16 ? 14 : 4;
     constexpr int UNITS = CIN == 16 ? 2 : 1;              // 16- or 8-byte units per voxel
     extern __shared__ __align__(16) unsigned char smem[];
     uint16_t* halo = (uint16_t*)smem;                     // [HX][HY][HZ][CIN]
@@ -148,9 +153,9 @@ __global__ void __launch_bounds__(256) k_conv(Vol V, const uint16_t* __restrict_
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int m = lane & 15, q = lane >> 4;
-    bf16x4 B[KC];
+    bf16x8 B[KC];
 #pragma unroll
-    for (int kc = 0; kc < KC; kc++) B[kc] = *(const bf16x4*)(wf + ((size_t)kc * 64 + lane) * 4);
+    for (int kc = 0; kc < KC; kc++) B[kc] = *(const bf16x8*)(wf + ((size_t)kc * 64 + lane) * 8);
     const float bco = (bias && m < cout) ? bias[m] : 0.0f;
     __syncthreads();
     // two z-rows per pass: independent accumulators hide the MFMA dependency latency
@@ -168,20 +173,31 @@ __global__ void __launch_bounds__(256) k_conv(Vol V, const uint16_t* __restrict_
         for (int kc = 0; kc < KC; kc++) {
 #pragma unroll
             for (int r = 0; r < 2; r++) {
-                bf16x4 a;
+                bf16x8 a;
                 if (CIN == 16) {
-                    const int h = ((tx[r] + kc / 9) * HY + (ty[r] + (kc / 3) % 3)) * HZ + (tz[r] + m + kc % 3);
-                    a = *(const bf16x4*)(halo + h * 16 + 4 * q);
-                } else {
-                    const int tap = 4 * kc + q;
+                    const int tap = 2 * kc + q / 2;     // 8 channels of one tap per lane
                     if (tap < 27) {
-                        const int h = ((tx[r] + tap / 9) * HY + (ty[r] + (tap / 3) % 3)) * HZ + (tz[r] + m + tap % 3);
-                        a = *(const bf16x4*)(halo + h * 4);
+                        const int h =
+                            ((tx[r] + tap / 9) * HY + (ty[r] + (tap / 3) % 3)) * HZ + (tz[r] + m + tap % 3);
+                        a = *(const bf16x8*)(halo + h * 16 + 8 * (q % 2));
                     } else {
-                        a = bf16x4{0, 0, 0, 0};
+                        a = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
                     }
+                } else {
+                    bf16x4 lo = bf16x4{0, 0, 0, 0}, hi = bf16x4{0, 0, 0, 0};   // 4 channels of two taps
+                    const int t0 = 8 * kc + 2 * q, t1 = t0 + 1;
+                    if (t0 < 27) {
+                        const int h = ((tx[r] + t0 / 9) * HY + (ty[r] + (t0 / 3) % 3)) * HZ + (tz[r] + m + t0 % 3);
+                        lo = *(const bf16x4*)(halo + h * 4);
+                    }
+                    if (t1 < 27) {
+                        const int h = ((tx[r] + t1 / 9) * HY + (ty[r] + (t1 / 3) % 3)) * HZ + (tz[r] + m + t1 % 3);
+                        hi = *(const bf16x4*)(halo + h * 4);
+                    }
+                    a = bf16x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
                 }
-                acc[r] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, B[kc], acc[r], 0, 0, 0);
+                acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16v8, a),
+                                                                 __builtin_bit_cast(bf16v8, B[kc]), acc[r], 0, 0, 0);
             }
         }
         const int co = m;
@@ -260,30 +276,38 @@ __global__ void __launch_bounds__(256) k_bgrad(long nv, const uint16_t* __restri
 #pragma unroll
         for (int c = 0; c < 16; c++)
             if (c < cout) s[c] += from_bf16(g[v * gpad + c]);
+    // wavefront sums, then one sum per workgroup in LDS: 16 atomics per workgroup, not per wavefront
+    __shared__ float part[4][16];
 #pragma unroll
     for (int c = 0; c < 16; c++) {
         float t = s[c];
         for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
-        if ((threadIdx.x & 63) == 0 && c < cout && t != 0.0f) atomicAdd(&db[c], t);
+        if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6][c] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < 16 && (int)threadIdx.x < cout) {
+        const float t = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] + part[3][threadIdx.x];
+        if (t != 0.0f) atomicAdd(&db[threadIdx.x], t);
     }
 }
 
-// weights -> MFMA B fragments.  Effective layer: cin_e x cout_e with W_e(t, a, b) = W(t, a, b), or
-// for the input gradient (flip) W(26 - t, b, a); CIN_e = 16 or 4 sets the K-chunk layout.
+// weights -> MFMA B fragments of v_mfma_f32_16x16x32_bf16, wf[kc][lane][8].  Effective layer:
+// cin_e x cout_e with W_e(t, a, b) = W(t, a, b), or for the input gradient (flip) W(26 - t, b, a);
+// cin_pad 16: chunk kc holds taps 2kc, 2kc + 1 x 16 channels; cin_pad 4: taps 8kc .. 8kc + 7 x 4.
 __global__ void k_pack(const float* __restrict__ W, int cin, int cout, int flip, int cin_pad, uint16_t* __restrict__ wf)
 {
-    const int kcs = cin_pad == 16 ? 27 : 7;
+    const int kcs = cin_pad == 16 ? 14 : 4;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= kcs * 64 * 4) return;
-    const int j = i & 3, lane = (i >> 2) & 63, kc = i >> 8;
-    const int n = lane & 15, q = lane >> 4;
+    if (i >= kcs * 64 * 8) return;
+    const int j = i & 7, lane = (i >> 3) & 63, kc = i >> 9;
+    const int n = lane & 15, k = 8 * (lane >> 4) + j;
     int tap, a;
     if (cin_pad == 16) {
-        tap = kc;
-        a = 4 * q + j;
+        tap = 2 * kc + k / 16;
+        a = k % 16;
     } else {
-        tap = 4 * kc + q;
-        a = j;
+        tap = 8 * kc + k / 4;
+        a = k % 4;
     }
     const int cin_e = flip ? cout : cin, cout_e = flip ? cin : cout;
     float w = 0.0f;
@@ -402,11 +426,11 @@ int ensure_cap(pk_dn* d, long nv)
 int pack_weights(pk_dn* d)
 {
     hipStream_t s = d->ctx->stream;
-    hipLaunchKernelGGL(k_pack, dim3(nblk(7 * 256, 256)), dim3(256), 0, s, d->p + OFF_W1, 3, HID, 0, 4, d->wf[0]);
-    hipLaunchKernelGGL(k_pack, dim3(nblk(27 * 256, 256)), dim3(256), 0, s, d->p + OFF_W2, HID, HID, 0, 16, d->wf[1]);
-    hipLaunchKernelGGL(k_pack, dim3(nblk(27 * 256, 256)), dim3(256), 0, s, d->p + OFF_W3, HID, 3, 0, 16, d->wf[2]);
-    hipLaunchKernelGGL(k_pack, dim3(nblk(27 * 256, 256)), dim3(256), 0, s, d->p + OFF_W2, HID, HID, 1, 16, d->wt[0]);
-    hipLaunchKernelGGL(k_pack, dim3(nblk(7 * 256, 256)), dim3(256), 0, s, d->p + OFF_W3, HID, 3, 1, 4, d->wt[1]);
+    hipLaunchKernelGGL(k_pack, dim3(nblk(WF4, 256)), dim3(256), 0, s, d->p + OFF_W1, 3, HID, 0, 4, d->wf[0]);
+    hipLaunchKernelGGL(k_pack, dim3(nblk(WF16, 256)), dim3(256), 0, s, d->p + OFF_W2, HID, HID, 0, 16, d->wf[1]);
+    hipLaunchKernelGGL(k_pack, dim3(nblk(WF16, 256)), dim3(256), 0, s, d->p + OFF_W3, HID, 3, 0, 16, d->wf[2]);
+    hipLaunchKernelGGL(k_pack, dim3(nblk(WF16, 256)), dim3(256), 0, s, d->p + OFF_W2, HID, HID, 1, 16, d->wt[0]);
+    hipLaunchKernelGGL(k_pack, dim3(nblk(WF4, 256)), dim3(256), 0, s, d->p + OFF_W3, HID, 3, 1, 4, d->wt[1]);
     PK_HIP(d->ctx, hipGetLastError());
     return PK_OK;
 }
@@ -454,7 +478,7 @@ int backward(pk_dn* d, const Vol& V, double inv_count, double* loss)
     hipStream_t s = d->ctx->stream;
     int r;
     hipLaunchKernelGGL(k_loss, dim3(nblk(V.nv, 256)), dim3(256), 0, s, V.nv, d->out, d->tgt, inv_count, d->g3, loss);
-    const unsigned nbb = nblk(V.nv, 256) < 1024 ? nblk(V.nv, 256) : 1024;
+    const unsigned nbb = nblk(V.nv, 1024) < 256 ? nblk(V.nv, 1024) : 256;   // 4+ voxels per thread
     if ((r = wgrad(d, V, d->a2, 16, HID, d->g3, 4, 3, d->g + OFF_W3))) return r;
     hipLaunchKernelGGL(k_bgrad, dim3(nbb), dim3(256), 0, s, V.nv, d->g3, 4, 3, d->g + OFF_B3);
     const unsigned nb = conv_blocks(V);
@@ -479,6 +503,7 @@ int check_dims(pk_dn* d, const int n[3], Vol& V)
     V.n1 = n[1];
     V.n2 = n[2];
     V.nv = (long)n[0] * n[1] * n[2];
+    if (V.nv >= (1l << 31)) return pk_fail(d->ctx, PK_E_ARG, "denoiser: volume above 2^31 voxels");
     return ensure_cap(d, V.nv);
 }
 
@@ -513,11 +538,11 @@ int pk_dn_create(pk_ctx* c, pk_dn** out)
     int r = 0;
     for (float** p : {&d->p, &d->g, &d->m, &d->v})
         if (!r) r = dalloc(c, (void**)p, sizeof(float) * NPARAM);
-    if (!r) r = dalloc(c, (void**)&d->wf[0], sizeof(uint16_t) * 7 * 256);
-    if (!r) r = dalloc(c, (void**)&d->wf[1], sizeof(uint16_t) * 27 * 256);
-    if (!r) r = dalloc(c, (void**)&d->wf[2], sizeof(uint16_t) * 27 * 256);
-    if (!r) r = dalloc(c, (void**)&d->wt[0], sizeof(uint16_t) * 27 * 256);
-    if (!r) r = dalloc(c, (void**)&d->wt[1], sizeof(uint16_t) * 7 * 256);
+    if (!r) r = dalloc(c, (void**)&d->wf[0], sizeof(uint16_t) * WF4);
+    if (!r) r = dalloc(c, (void**)&d->wf[1], sizeof(uint16_t) * WF16);
+    if (!r) r = dalloc(c, (void**)&d->wf[2], sizeof(uint16_t) * WF16);
+    if (!r) r = dalloc(c, (void**)&d->wt[0], sizeof(uint16_t) * WF16);
+    if (!r) r = dalloc(c, (void**)&d->wt[1], sizeof(uint16_t) * WF4);
     if (!r) r = dalloc(c, (void**)&d->loss, sizeof(double));
     if (r) {
         pk_dn_destroy(d);

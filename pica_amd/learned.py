@@ -78,7 +78,7 @@ def generate_pairs(ctx, dn, runs, ncell=(32, 32, 32), ppc_low=4, ppc_high=256, s
             obj.close()
 
 
-def train(dn, schedule=((1500, 2e-3), (500, 5e-4)), batch=8, seed=1):
+def train(dn, schedule=((4000, 2e-3), (3000, 5e-4), (1000, 1e-4)), batch=8, seed=1):
     """Adam over the pairs; returns the per-step losses"""
     losses = []
     for k, (iters, lr) in enumerate(schedule):
@@ -120,13 +120,22 @@ def evaluate(ctx, dn, ncell=(128, 128, 128), ppc_low=4, ppc_mid=64, ppc_ref=256,
     # step costs on fresh plasmas
     cost = {}
     mid = plasma(ctx, ncell, ppc_mid, modes, 1)
-    cost["mid_plain_ms"] = 1e3 * timed_steps(lambda: step(*mid, DT, TSC, ESIRKEPOV), 5, ctx.sync)
+    cost["mid_plain_ms"] = 1e3 * timed_steps(lambda: step(*mid, DT, TSC, ESIRKEPOV), 10, ctx.sync)
     for obj in mid:
         obj.close()
     low = plasma(ctx, ncell, ppc_low, modes, 2)
-    cost["low_plain_ms"] = 1e3 * timed_steps(lambda: step(*low, DT, TSC, ESIRKEPOV), 5, ctx.sync)
-    cost["low_denoised_ms"] = 1e3 * timed_steps(lambda: step_denoised(low[0], low[1], dn, DT, TSC, ESIRKEPOV), 5,
+    cost["low_plain_ms"] = 1e3 * timed_steps(lambda: step(*low, DT, TSC, ESIRKEPOV), 10, ctx.sync)
+    cost["low_denoised_ms"] = 1e3 * timed_steps(lambda: step_denoised(low[0], low[1], dn, DT, TSC, ESIRKEPOV), 10,
                                                 ctx.sync)
+    # per-kernel HIP-event times of the denoised step
+    ctx.timer_reset()
+    ctx.timer_enable(True)
+    for _ in range(5):
+        step_denoised(low[0], low[1], dn, DT, TSC, ESIRKEPOV)
+    ctx.sync()
+    ctx.timer_enable(False)
+    cost["low_denoised_kernels_ms"] = {k: ctx.timer(k)[0] / 5 for k in ("particles", "insert", "denoise", "project",
+                                                                        "yee_e", "yee_b", "ghost", "zero")}
     for obj in low:
         obj.close()
     cost["mid_over_low_denoised"] = cost["mid_plain_ms"] / cost["low_denoised_ms"]
@@ -135,18 +144,20 @@ def evaluate(ctx, dn, ncell=(128, 128, 128), ppc_low=4, ppc_mid=64, ppc_ref=256,
             "mean_rel_l2_vs_reference": mean, "cost": cost}
 
 
-def run(ctx, train_runs=12, train_grid=(32, 32, 32), eval_grid=(128, 128, 128), schedule=None, seed=0):
+def run(ctx, train_runs=48, train_grid=(32, 32, 32), eval_grid=(128, 128, 128), schedule=None, seed=0,
+        ppc_target=256):
     """the whole learned-form pipeline; returns (report dict, trained Denoiser)"""
     dn = Denoiser(ctx)
     dn.set_scale(network_scale())
     dn.init(seed + 17)
     t0 = time.perf_counter()
-    generate_pairs(ctx, dn, train_runs, train_grid, seed=seed)
+    generate_pairs(ctx, dn, train_runs, train_grid, ppc_high=ppc_target, seed=seed)
     t1 = time.perf_counter()
     losses = train(dn, **({"schedule": schedule} if schedule else {}))
     ctx.sync()
     t2 = time.perf_counter()
-    report = {"train": {"grid": list(train_grid), "pairs": dn.num_pairs(), "iters": int(losses.size),
+    report = {"train": {"grid": list(train_grid), "ppc": {"input": 4, "target": ppc_target}, "pairs": dn.num_pairs(),
+                        "iters": int(losses.size),
                         "loss_first": float(np.mean(losses[:20])), "loss_last": float(np.mean(losses[-20:])),
                         "data_s": t1 - t0, "train_s": t2 - t1}}
     report["eval"] = evaluate(ctx, dn, eval_grid)
