@@ -11,8 +11,11 @@
 // The LDS J tile is accumulated in FP64: on gfx950 ds_add_f64 sustains ~12x the rate of
 // ds_add_f32 (measured 2.4e12 vs 2.0e11 LDS atomics/s on MI355X), and the extra precision
 // is free.  Esirkepov deposition uses a per-wavefront window: 4 (TSC) or 3 (CIC) nodes in a
-// dimension only when some lane of the wavefront changes its nearest node there, else 3 / 2,
-// so a typical wavefront issues about half of the worst-case atomics.
+// dimension only when some lane of the wavefront changes its nearest node there, else 3 / 2.
+// Particles that change node go to a workgroup change queue deposited by full wavefronts after
+// the push, so most wavefronts issue only the small window.  The push loop has no workgroup
+// barrier: each wavefront owns every fourth 64-particle group and the four drift apart, which
+// lets one wavefront's LDS atomics overlap another's arithmetic.
 //
 // Stencil conventions (grid origin -2h, two ghost layers; l = cell - supercell start):
 //   interpolation tile base = array index m+1 (pica's baseIdx), CIC n+2 / TSC n+3 wide;
@@ -24,6 +27,7 @@ namespace {
 
 constexpr int WAVES = PK_BLOCK / 64;
 constexpr int REC_WORDS = 8;     // deposit record: packed cell/shift/class, o[3], on[3], q w
+constexpr int QCAP = 256;        // change-queue records per workgroup
 
 __device__ __forceinline__ bool colloc(int comp, int d)
 {
@@ -318,18 +322,18 @@ __device__ __forceinline__ void particles_split_sc(const SplitArgs& A, long s)
     float* itile = (float*)smem;
     const int joff = (int)((sizeof(float) * 6 * itn + 7) & ~(size_t)7);
     double* jtile = (double*)(smem + joff);
-    uint64_t* mask = (uint64_t*)(smem + joff + sizeof(double) * 3 * dtn);
-    int* wtot = (int*)(mask + sp.mask_words);
-    int* dircnt = wtot + WAVES;
+    uint64_t* mask = (uint64_t*)(smem + joff + sizeof(double) * 3 * dtn);   // mover bit per particle
+    int* dircnt = (int*)(mask + sp.mask_words);                                 // 27 direction counts
     int* bad = dircnt + 27;
-    long* wpre = (long*)(wtot + 32);
-    // Esirkepov only: this chunk's deposit records ranked by window class, and per-wave class counts
+    int* qlen = dircnt + 28;                                                    // change-queue length
+    long* wpre = (long*)(dircnt + 32);                                          // movers before each mask word
+    // Esirkepov only: the change queue, QCAP deposit records stored word-major (conflict-free)
     const int rec_off = (int)(((size_t)((unsigned char*)(wpre + sp.mask_words + 1) - smem) + 15) & ~(size_t)15);
-    int* rec = (int*)(smem + rec_off);                  // PK_BLOCK x REC_WORDS
-    int* ccnt = (int*)(smem + rec_off + 4 * REC_WORDS * PK_BLOCK);   // WAVES x 9
+    int* rec = (int*)(smem + rec_off);
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
+    const uint64_t below = (1ull << lane) - 1ull;
     for (int t = tid; t < 6 * itn; t += PK_BLOCK) {
         const int comp = t / itn, r = t % itn;
         const int i = r / isx, j = (r / isy) % its[1], k = r % isy;
@@ -337,17 +341,23 @@ __device__ __forceinline__ void particles_split_sc(const SplitArgs& A, long s)
     }
     for (int t = tid; t < 3 * dtn; t += PK_BLOCK) jtile[t] = 0.0;
     for (int t = tid; t < 27; t += PK_BLOCK) dircnt[t] = 0;
-    if (tid == 0) bad[0] = 0;
+    if (tid == 0) {
+        bad[0] = 0;
+        qlen[0] = 0;
+    }
     __syncthreads();
 
     const long base = off[s];
     const int n = cnt[s];
     const int N1N2 = eg.ncell[1] * eg.ncell[2];
-    int running = 0;
-    for (int c0 = 0; c0 < n; c0 += PK_BLOCK) {
-        const int i = c0 + tid;
+    // Each wavefront owns the 64-particle groups wave, wave + WAVES, ...  No workgroup barrier
+    // until every particle is pushed, so the wavefronts drift apart and one's LDS atomics overlap
+    // another's arithmetic.  Every pushed particle is written back in place; movers are copied to
+    // the outbox afterwards, in index order.
+    for (int c0 = 64 * wave; c0 < n; c0 += PK_BLOCK) {
+        const int i = c0 + lane;
         const bool act = i < n;
-        int cell = 0, code = 13;
+        int cell = 0;
         int c[3] = {0, 0, 0}, l[3] = {0, 0, 0}, sh[3] = {0, 0, 0};
         float o[3] = {0, 0, 0}, on[3] = {0, 0, 0}, u[3] = {0, 0, 0}, v[3] = {0, 0, 0}, w = 0.0f;
         bool ok = act;
@@ -376,68 +386,50 @@ __device__ __forceinline__ void particles_split_sc(const SplitArgs& A, long s)
             for (int d = 0; d < 3; d++) v[d] = u[d] * invGamma;    // in units of c
             move_split(o, v, sp.cdt_h, on, sh);
         }
+        if (act && !ok) bad[0] = 1;                     // left unchanged and reported
         // ---- deposition
         const float qw = sp.q * w;
         if (DEP == PK_ESIRKEPOV) {
-            // A wavefront deposits on the window its widest lane needs, so one cell-changing lane
-            // costs the whole wavefront the larger window.  Rank the chunk's deposits by window
-            // class (bit d: the particle changes node in dimension d) through LDS: the particles
-            // that need larger windows then share as few wavefronts as possible.
-            int cls = 8;                                  // 8: nothing to deposit
+            // A wavefront deposits on the window its widest lane needs.  Particles that change
+            // their node in some dimension (the larger window there) go to the workgroup's change
+            // queue, deposited after the loop by full wavefronts; the rest deposit now on the
+            // small window.  A full queue sends the overflow back to its own wavefront.
+            int cls = 0;                                  // bit d: node change in dimension d
             if (ok)
                 cls = (node_change<SHAPE>(o[0], on[0], sh[0]) ? 4 : 0) |
                       (node_change<SHAPE>(o[1], on[1], sh[1]) ? 2 : 0) | (node_change<SHAPE>(o[2], on[2], sh[2]) ? 1 : 0);
-            int rank = 0;
-#pragma unroll
-            for (int c = 0; c < 9; c++) {
-                const uint64_t bc = __ballot(cls == c);
-                if (lane == 0) ccnt[wave * 9 + c] = __popcll(bc);
-                if (cls == c) rank = __popcll(bc & ((1ull << lane) - 1ull));
+            const uint64_t cb = __ballot(cls != 0);
+            bool queued = false;
+            if (cb) {
+                int q0 = 0;
+                if (lane == 0) q0 = atomicAdd(qlen, __popcll(cb));
+                q0 = __shfl(q0, 0);
+                const int slot = q0 + __popcll(cb & below);
+                if (cls != 0 && slot < QCAP) {
+                    queued = true;
+                    rec[0 * QCAP + slot] = l[0] | (l[1] << 4) | (l[2] << 8) | ((sh[0] + 1) << 12) |
+                                           ((sh[1] + 1) << 14) | ((sh[2] + 1) << 16) | (cls << 18);
+                    rec[1 * QCAP + slot] = __float_as_int(o[0]);
+                    rec[2 * QCAP + slot] = __float_as_int(o[1]);
+                    rec[3 * QCAP + slot] = __float_as_int(o[2]);
+                    rec[4 * QCAP + slot] = __float_as_int(on[0]);
+                    rec[5 * QCAP + slot] = __float_as_int(on[1]);
+                    rec[6 * QCAP + slot] = __float_as_int(on[2]);
+                    rec[7 * QCAP + slot] = __float_as_int(qw);
+                }
             }
-            __syncthreads();
-            // every wavefront scans the 9 x WAVES counts in (class, wave) order with shuffles
-            static_assert(9 * WAVES <= 64, "one count per lane");
-            const int kc = lane < 9 * WAVES ? ccnt[(lane % WAVES) * 9 + lane / WAVES] : 0;
-            int incl = kc;
-#pragma unroll
-            for (int o2 = 1; o2 < 64; o2 <<= 1) {
-                const int y = __shfl_up(incl, o2);
-                if (lane >= o2) incl += y;
-            }
-            const int slot = rank + __shfl(incl - kc, cls * WAVES + wave);
-            const int nrec = __shfl(incl, 8 * WAVES - 1);
-            if (cls < 8) {
-                int* r = rec + slot * REC_WORDS;
-                r[0] = l[0] | (l[1] << 4) | (l[2] << 8) | ((sh[0] + 1) << 12) | ((sh[1] + 1) << 14) |
-                       ((sh[2] + 1) << 16) | (cls << 18);
-                r[1] = __float_as_int(o[0]); r[2] = __float_as_int(o[1]); r[3] = __float_as_int(o[2]);
-                r[4] = __float_as_int(on[0]); r[5] = __float_as_int(on[1]); r[6] = __float_as_int(on[2]);
-                r[7] = __float_as_int(qw);
-            }
-            __syncthreads();
-            const bool has = tid < nrec;
-            int dl[3] = {0, 0, 0}, dsh[3] = {0, 0, 0}, dcls = 0;
-            float d_o[3] = {0.5f, 0.5f, 0.5f}, d_on[3] = {0.5f, 0.5f, 0.5f}, dqw = 0.0f;
-            if (has) {
-                const int* r = rec + tid * REC_WORDS;
-                const int w0 = r[0];
-                dl[0] = w0 & 15; dl[1] = (w0 >> 4) & 15; dl[2] = (w0 >> 8) & 15;
-                dsh[0] = ((w0 >> 12) & 3) - 1; dsh[1] = ((w0 >> 14) & 3) - 1; dsh[2] = ((w0 >> 16) & 3) - 1;
-                dcls = (w0 >> 18) & 7;
-                d_o[0] = __int_as_float(r[1]); d_o[1] = __int_as_float(r[2]); d_o[2] = __int_as_float(r[3]);
-                d_on[0] = __int_as_float(r[4]); d_on[1] = __int_as_float(r[5]); d_on[2] = __int_as_float(r[6]);
-                dqw = __int_as_float(r[7]);
-            }
+            const bool now = ok && !queued;
             bool big[3];
 #pragma unroll
-            for (int d = 0; d < 3; d++) big[d] = __ballot(has && ((dcls >> (2 - d)) & 1)) != 0ull;
-            if (has) esirkepov_window<SHAPE>(jtile, dtn, dsx, dsy, dl, dsh, d_o, d_on, big, dqw, sp.hdt);
+            for (int d = 0; d < 3; d++) big[d] = __ballot(now && ((cls >> (2 - d)) & 1)) != 0ull;
+            if (now) esirkepov_window<SHAPE>(jtile, dtn, dsx, dsy, l, sh, o, on, big, qw, sp.hdt);
         } else if (ok) {
             // direct deposition of q w v at the new position: the CIC tile is pica's (base m+1,
             // the interpolation indices), the TSC tile has base m (indices + 1)
             constexpr int KK = SHAPE == PK_TSC ? 3 : 2;
             int bc[3], bs[3];
             float wc[3][3], ws[3][3];
+            bool fits = true;
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 interp_stencil<SHAPE>(l[d] + sh[d], on[d], bc[d], wc[d], bs[d], ws[d]);
@@ -445,9 +437,9 @@ __device__ __forceinline__ void particles_split_sc(const SplitArgs& A, long s)
                     bc[d] += 1;
                     bs[d] += 1;
                 }
-                if (bc[d] < 0 || bc[d] + KK > dts[d] || bs[d] < 0 || bs[d] + KK > dts[d]) ok = false;
+                if (bc[d] < 0 || bc[d] + KK > dts[d] || bs[d] < 0 || bs[d] + KK > dts[d]) fits = false;
             }
-            if (ok) {
+            if (fits) {
 #pragma unroll
                 for (int comp = 0; comp < 3; comp++) {
                     const float val = v[comp] * (float)PK_C * qw;
@@ -470,84 +462,97 @@ __device__ __forceinline__ void particles_split_sc(const SplitArgs& A, long s)
                             for (int cc = 0; cc < KK; cc++)
                                 atomicAdd(&tb[a * dsx + bb * dsy + cc], (double)(wt[0][a] * wt[1][bb] * wt[2][cc] * val));
                 }
+            } else {
+                ok = false;
+                bad[0] = 1;
             }
         }
-        // ---- periodic wrap of the cell and direction class
-        int ncell_lin = cell;
+        // ---- periodic wrap of the cell; a mover leaves the supercell
+        bool mover = false;
         if (ok) {
             int cn[3];
-            code = 0;
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 cn[d] = c[d] + sh[d];
                 if (cn[d] < 0) cn[d] += eg.ncell[d];
                 if (cn[d] >= eg.ncell[d]) cn[d] -= eg.ncell[d];
-                int dd = cn[d] / nn[d] - sidx[d];
-                if (dd > 1) dd -= eg.nsc[d];
-                if (dd < -1) dd += eg.nsc[d];
-                code = code * 3 + (dd + 1);
-                o[d] = on[d];
+                mover = mover || cn[d] / nn[d] != sidx[d];
             }
-            ncell_lin = (cn[0] * eg.ncell[1] + cn[1]) * eg.ncell[2] + cn[2];
-        } else if (act) {
-            // leave the particle unchanged and report
             const long p = base + i;
-            ncell_lin = pm.cell[p];
-            o[0] = pm.ox[p]; o[1] = pm.oy[p]; o[2] = pm.oz[p];
-            u[0] = pm.ux[p]; u[1] = pm.uy[p]; u[2] = pm.uz[p];
-            code = 13;
-            bad[0] = 1;
+            pm.cell[p] = (cn[0] * eg.ncell[1] + cn[1]) * eg.ncell[2] + cn[2];
+            pm.ox[p] = on[0]; pm.oy[p] = on[1]; pm.oz[p] = on[2];
+            pm.ux[p] = u[0]; pm.uy[p] = u[1]; pm.uz[p] = u[2];
         }
-        const bool mover = act && code != 13;
         const uint64_t mb = __ballot(mover);
-        if (lane == 0) {
-            wtot[wave] = __popcll(mb);
-            mask[(c0 >> 6) + wave] = mb;
-        }
-        __syncthreads();
-        int pre = running, tot = 0;
-#pragma unroll
-        for (int vv = 0; vv < WAVES; vv++) {
-            pre += (vv < wave) ? wtot[vv] : 0;
-            tot += wtot[vv];
-        }
-        if (act) {
-            const long p = base + i;
-            if (!mover) {
-                pm.cell[p] = ncell_lin;
-                pm.ox[p] = o[0]; pm.oy[p] = o[1]; pm.oz[p] = o[2];
-                pm.ux[p] = u[0]; pm.uy[p] = u[1]; pm.uz[p] = u[2];
-            } else {
-                const int rank = pre + __popcll(mb & ((1ull << lane) - 1ull));
-                const long q = base + rank;
-                po.cell[q] = ncell_lin;
-                po.ox[q] = o[0]; po.oy[q] = o[1]; po.oz[q] = o[2];
-                po.ux[q] = u[0]; po.uy[q] = u[1]; po.uz[q] = u[2];
-                po.w[q] = w;
-                odir[q] = (uint8_t)code;
-                atomicAdd(&dircnt[code], 1);
-            }
-        }
-        running += tot;
-        __syncthreads();
+        if (lane == 0) mask[c0 >> 6] = mb;
     }
-    const int nmv = running;
-    const int nfinal = n - nmv;
-    const int words = (n + 63) >> 6;
-    if (nmv > 0) {
-        if (tid == 0) {
-            long acc = 0;
-            for (int wd = 0; wd < words; wd++) {
-                wpre[wd] = acc;
-                acc += __popcll(mask[wd]);
+    __syncthreads();
+    if (DEP == PK_ESIRKEPOV) {
+        // the change queue, by full wavefronts
+        const int nq = qlen[0] < QCAP ? qlen[0] : QCAP;
+        for (int t = tid; t - lane < nq; t += PK_BLOCK) {
+            const bool has = t < nq;
+            int dl[3] = {0, 0, 0}, dsh[3] = {0, 0, 0}, dcls = 0;
+            float d_o[3] = {0.5f, 0.5f, 0.5f}, d_on[3] = {0.5f, 0.5f, 0.5f}, dqw = 0.0f;
+            if (has) {
+                const int w0 = rec[t];
+                dl[0] = w0 & 15; dl[1] = (w0 >> 4) & 15; dl[2] = (w0 >> 8) & 15;
+                dsh[0] = ((w0 >> 12) & 3) - 1; dsh[1] = ((w0 >> 14) & 3) - 1; dsh[2] = ((w0 >> 16) & 3) - 1;
+                dcls = (w0 >> 18) & 7;
+                d_o[0] = __int_as_float(rec[1 * QCAP + t]);
+                d_o[1] = __int_as_float(rec[2 * QCAP + t]);
+                d_o[2] = __int_as_float(rec[3 * QCAP + t]);
+                d_on[0] = __int_as_float(rec[4 * QCAP + t]);
+                d_on[1] = __int_as_float(rec[5 * QCAP + t]);
+                d_on[2] = __int_as_float(rec[6 * QCAP + t]);
+                dqw = __int_as_float(rec[7 * QCAP + t]);
             }
-            wpre[words] = acc;
+            bool big[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) big[d] = __ballot(has && ((dcls >> (2 - d)) & 1)) != 0ull;
+            if (has) esirkepov_window<SHAPE>(jtile, dtn, dsx, dsy, dl, dsh, d_o, d_on, big, dqw, sp.hdt);
         }
-        __syncthreads();
+    }
+    // ---- movers: prefix over the mask words, copy to the outbox in index order, fill the holes
+    const int words = (n + 63) >> 6;
+    if (tid == 0) {
+        long acc = 0;
+        for (int wd = 0; wd < words; wd++) {
+            wpre[wd] = acc;
+            acc += __popcll(mask[wd]);
+        }
+        wpre[words] = acc;
+    }
+    __syncthreads();
+    const int nmv = (int)wpre[words];
+    const int nfinal = n - nmv;
+    if (nmv > 0) {
         auto movers_below = [&](int y) -> long {
             const int wd = y >> 6, bt = y & 63;
             return wpre[wd] + (bt ? __popcll(mask[wd] & ((1ull << bt) - 1ull)) : 0);
         };
+        for (int i = tid; i < n; i += PK_BLOCK) {
+            if (!((mask[i >> 6] >> (i & 63)) & 1ull)) continue;
+            const long src = base + i, q = base + movers_below(i);
+            const int cl = pm.cell[src];
+            po.cell[q] = cl;
+            po.ox[q] = pm.ox[src]; po.oy[q] = pm.oy[src]; po.oz[q] = pm.oz[src];
+            po.ux[q] = pm.ux[src]; po.uy[q] = pm.uy[src]; po.uz[q] = pm.uz[src];
+            po.w[q] = pm.w[src];
+            int cn[3], code = 0;
+            cell_coords(cl, eg, cn);
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                int dd = cn[d] / nn[d] - sidx[d];
+                if (dd > 1) dd -= eg.nsc[d];
+                if (dd < -1) dd += eg.nsc[d];
+                code = code * 3 + (dd + 1);
+            }
+            odir[q] = (uint8_t)code;
+            atomicAdd(&dircnt[code], 1);
+        }
+        __syncthreads();
+        // tail stayers fill the holes the movers left
         const long below_final = movers_below(nfinal);
         for (int t = tid; t < nmv; t += PK_BLOCK) {
             const int j = nfinal + t;
@@ -1059,7 +1064,7 @@ size_t split_lds(const pk_ens* e, int shape, int dep, int mask_words)
     const size_t base = joff + sizeof(double) * 3 * dts[0] * dts[1] * dts[2] + sizeof(uint64_t) * mask_words +
                         sizeof(int) * 32 + sizeof(long) * (mask_words + 1);
     if (dep != PK_ESIRKEPOV) return base + 16;
-    return ((base + 15) & ~(size_t)15) + sizeof(int) * (REC_WORDS * PK_BLOCK + WAVES * 9) + 16;
+    return ((base + 15) & ~(size_t)15) + sizeof(int) * REC_WORDS * QCAP + 16;
 }
 
 SplitArgs split_args(pk_ens* e, pk_grid* g, const EnsGeom& eg, const SplitParams& sp)
