@@ -28,6 +28,11 @@ namespace {
 constexpr int WAVES = PK_BLOCK / 64;
 constexpr int REC_WORDS = 8;     // deposit record: packed cell/shift/class, o[3], on[3], q w
 constexpr int QCAP = 256;        // change-queue records per workgroup
+// x-stride padding of the FP64 J tile (doubles).  With 8 x 8 x 8 nodes unpadded, the 4^3 cells a
+// wavefront's particles sit in fall on 8 of the 16 double-wide LDS bank pairs; +4 spreads them 4
+// per pair, which lowers the bank conflicts of the deposit atomics.
+constexpr int JPAD = 4;
+constexpr int IPAD = 4;          // x-stride padding of the field tile (floats), same purpose
 
 __device__ __forceinline__ bool colloc(int comp, int d)
 {
@@ -314,8 +319,8 @@ __device__ __forceinline__ void particles_split_sc(const SplitArgs& A, long s)
         dts[d] = nn[d] + (PICA_CIC ? 3 : 4);
         dbase[d] = PICA_CIC ? m0[d] + 1 : m0[d];
     }
-    const int isx = its[1] * its[2], isy = its[2];
-    const int dsx = dts[1] * dts[2], dsy = dts[2];
+    const int isx = its[1] * its[2] + IPAD, isy = its[2];
+    const int dsx = dts[1] * dts[2] + JPAD, dsy = dts[2];
     const int itn = its[0] * isx;
     const int dtn = dts[0] * dsx;
     // every LDS pointer is smem + a byte offset, so that the compiler keeps DS instructions
@@ -336,8 +341,9 @@ __device__ __forceinline__ void particles_split_sc(const SplitArgs& A, long s)
     const uint64_t below = (1ull << lane) - 1ull;
     for (int t = tid; t < 6 * itn; t += PK_BLOCK) {
         const int comp = t / itn, r = t % itn;
-        const int i = r / isx, j = (r / isy) % its[1], k = r % isy;
-        itile[t] = g.f[PK_EX + comp][((long)(ibase[0] + i) * g.s1 + (ibase[1] + j)) * g.s2 + (ibase[2] + k)];
+        const int i = r / isx, rr = r % isx, j = rr / isy, k = rr % isy;
+        if (j < its[1])
+            itile[t] = g.f[PK_EX + comp][((long)(ibase[0] + i) * g.s1 + (ibase[1] + j)) * g.s2 + (ibase[2] + k)];
     }
     for (int t = tid; t < 3 * dtn; t += PK_BLOCK) jtile[t] = 0.0;
     for (int t = tid; t < 27; t += PK_BLOCK) dircnt[t] = 0;
@@ -582,7 +588,8 @@ __device__ __forceinline__ void particles_split_sc(const SplitArgs& A, long s)
         const double v = jtile[t];
         if (v == 0.0) continue;
         const int comp = t / dtn, r = t % dtn;
-        const int i = r / dsx, j = (r / dsy) % dts[1], k = r % dsy;
+        const int i = r / dsx, rr = r % dsx, j = rr / dsy, k = rr % dsy;
+        if (j >= dts[1]) continue;                     // padding
         atomicAdd(&g.f[PK_JX + comp][((long)(dbase[0] + i) * g.s1 + (dbase[1] + j)) * g.s2 + (dbase[2] + k)],
                   (float)(v * (double)sp.norm));
     }
@@ -1060,8 +1067,8 @@ size_t split_lds(const pk_ens* e, int shape, int dep, int mask_words)
         its[d] = e->ncps[d] + (shape == PK_TSC ? 3 : 2);
         dts[d] = e->ncps[d] + ((shape == PK_CIC && dep == PK_DIRECT) ? 3 : 4);
     }
-    const size_t joff = (sizeof(float) * 6 * its[0] * its[1] * its[2] + 7) & ~(size_t)7;
-    const size_t base = joff + sizeof(double) * 3 * dts[0] * dts[1] * dts[2] + sizeof(uint64_t) * mask_words +
+    const size_t joff = (sizeof(float) * 6 * its[0] * (its[1] * its[2] + IPAD) + 7) & ~(size_t)7;
+    const size_t base = joff + sizeof(double) * 3 * dts[0] * (dts[1] * dts[2] + JPAD) + sizeof(uint64_t) * mask_words +
                         sizeof(int) * 32 + sizeof(long) * (mask_words + 1);
     if (dep != PK_ESIRKEPOV) return base + 16;
     return ((base + 15) & ~(size_t)15) + sizeof(int) * REC_WORDS * QCAP + 16;
