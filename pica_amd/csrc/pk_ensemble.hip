@@ -68,9 +68,19 @@ __device__ __forceinline__ void insert_sc(const InsertArgs<W>& A, long s, int* p
     const int lane = threadIdx.x;
     const int nfinal = A.nstay[s];
     const long cap = A.off[s + 1] - A.off[s];
-    int term = 0;
-    if (lane < 27 && lane != 13) term = A.mvc[sc_neighbour(eg, s, lane, -1) * 27 + lane];
-    int nin = term;
+    // lane d < 27 (d != 13): the neighbour the class-d leavers come from, their count, and that
+    // neighbour's outbox (all loaded at once: one memory latency, not one per class)
+    int m_d = 0, nm_d = 0;
+    long ob_d = 0;
+    if (lane < 27 && lane != 13) {
+        const long src = sc_neighbour(eg, s, lane, -1);
+        m_d = A.mvc[src * 27 + lane];
+        if (m_d > 0) {
+            nm_d = A.nmove[src];
+            ob_d = A.off[src];
+        }
+    }
+    int nin = m_d;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) nin += __shfl_xor(nin, o);
     if ((long)nfinal + nin > cap) {
@@ -82,13 +92,11 @@ __device__ __forceinline__ void insert_sc(const InsertArgs<W>& A, long s, int* p
     }
     if (lane == 0) A.cnt[s] = nfinal + nin;
     long p = A.off[s] + nfinal;
-    for (int d = 0; d < 27; d++) {
-        if (d == 13) continue;
-        const long src = sc_neighbour(eg, s, d, -1);
-        const int m = A.mvc[src * 27 + d];
-        if (m == 0) continue;
-        const int nm = A.nmove[src];
-        const long ob = A.off[src];
+    // the classes with leavers, in class order
+    for (uint64_t todo = __ballot(m_d > 0); todo; todo &= todo - 1ull) {
+        const int d = __ffsll((long long)todo) - 1;
+        const int m = __shfl(m_d, d), nm = __shfl(nm_d, d);
+        const long ob = __shfl(ob_d, d);
         int found = 0;
         for (int k0 = 0; k0 < nm && found < m; k0 += 64) {
             const int k = k0 + lane;
