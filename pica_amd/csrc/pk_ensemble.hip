@@ -65,7 +65,7 @@ template <class W, int NF>
 __device__ __forceinline__ void insert_sc(const InsertArgs<W>& A, long s, int* pending)
 {
     const EnsGeom& eg = A.eg;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
     const int nfinal = A.nstay[s];
     const long cap = A.off[s + 1] - A.off[s];
     // lane d < 27 (d != 13): the neighbour the class-d leavers come from, their count, and that
@@ -113,10 +113,13 @@ __device__ __forceinline__ void insert_sc(const InsertArgs<W>& A, long s, int* p
     }
 }
 
+// INSERT_SC supercells per workgroup, one per wavefront (XCD-aware over groups of them)
+constexpr int INSERT_SC = 4;
+
 template <class W, int NF>
-__global__ void __launch_bounds__(64) k_insert(InsertArgs<W> A)
+__global__ void __launch_bounds__(64 * INSERT_SC) k_insert(InsertArgs<W> A, int gchunk)
 {
-    const long s = pk_xcd_supercell(blockIdx.x, A.eg.xcd_chunk);
+    const long s = pk_xcd_supercell(blockIdx.x, gchunk) * INSERT_SC + (threadIdx.x >> 6);
     if (s >= A.eg.nsct) return;
     insert_sc<W, NF>(A, s, nullptr);
 }
@@ -549,12 +552,16 @@ int pk_ens_alloc(pk_ens* e, size_t capacity)
 int pk_ens_insert_launch(pk_ens* e)
 {
     EnsGeom eg = ens_geom(e);
-    const unsigned nb = (unsigned)(PK_XCD * (long)eg.xcd_chunk);
+    const long ngroups = (eg.nsct + INSERT_SC - 1) / INSERT_SC;
+    const int gchunk = (int)((ngroups + PK_XCD - 1) / PK_XCD);
+    const unsigned nb = (unsigned)(PK_XCD * (long)gchunk);
     pk_ktimer tm(e->ctx, "insert");
     if (e->prec == PK_F64)
-        hipLaunchKernelGGL((k_insert<uint64_t, 7>), dim3(nb), dim3(64), 0, e->ctx->stream, insert_args<uint64_t>(e));
+        hipLaunchKernelGGL((k_insert<uint64_t, 7>), dim3(nb), dim3(64 * INSERT_SC), 0, e->ctx->stream, insert_args<uint64_t>(e),
+                           gchunk);
     else
-        hipLaunchKernelGGL((k_insert<uint32_t, 8>), dim3(nb), dim3(64), 0, e->ctx->stream, insert_args<uint32_t>(e));
+        hipLaunchKernelGGL((k_insert<uint32_t, 8>), dim3(nb), dim3(64 * INSERT_SC), 0, e->ctx->stream, insert_args<uint32_t>(e),
+                           gchunk);
     PK_HIP(e->ctx, hipGetLastError());
     return PK_OK;
 }
