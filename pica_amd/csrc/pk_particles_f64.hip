@@ -5,12 +5,13 @@
 //   1. gather the E/B stencil of the supercell into LDS (FieldInterpolatorCICSupercell ctor,
 //      FieldInterpolatorYeeGrid.h:197-223) and zero an LDS J tile
 //      (CurrentDepositorCICSupercell ctor, CurrentDepositorYeeGrid.h:178-199);
-//   2. stream the supercell's SoA particles, 256 at a time, coalesced: interpolate
-//      (:225-242, 257-281), push (BorisPusher.h:32-47), deposit into the LDS tile with LDS
-//      atomics (CurrentDepositorYeeGrid.h:215-262, or Esirkepov);
-//   3. particles that stay write back x, u in place; particles that leave go to the outbox
-//      (with their direction class), and the stayers found past the final count fill the
-//      holes movers left below it (ascending order on both sides);
+//   2. stream the supercell's SoA particles, each wavefront its own 64-particle groups with no
+//      workgroup barrier, coalesced: interpolate (:225-242, 257-281), push
+//      (BorisPusher.h:32-47), deposit into the LDS tile with LDS atomics
+//      (CurrentDepositorYeeGrid.h:215-262, or Esirkepov), write x, u back in place;
+//   3. particles that leave are copied to the outbox in slot order (with their direction
+//      class), and the stayers found past the final count fill the holes movers left below it
+//      (ascending order on both sides);
 //   4. flush the J tile x 1/V with one pass of global FP64 atomics (the destructor, :201-213).
 // This file is compiled with -ffp-contract=off: per particle, the arithmetic is pica's, in
 // pica's order, so positions and momenta agree with pica to the last bit given the same
@@ -20,7 +21,6 @@
 namespace {
 
 constexpr int WAVES = PK_BLOCK / 64;
-static_assert(WAVES + 27 + 1 <= 32, "LDS bookkeeping layout");
 
 struct ScGeom {
     double inv[3], no[3], nos[3], nom[3];
