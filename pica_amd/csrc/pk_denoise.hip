@@ -339,6 +339,21 @@ __global__ void k_grid_to_f32(GridView<T> g, Vol V, double scale, float* __restr
     for (int c = 0; c < 3; c++) dst[(long)c * V.nv + v] = (float)(scale * (double)g.f[PK_JX + c][gi]);
 }
 
+// physical J of a grid -> the network input xb[v][0..3] (BF16, scaled; the same roundings as
+// k_grid_to_f32 then k_input_f32)
+template <class T>
+__global__ void k_grid_to_input(GridView<T> g, Vol V, double scale, uint16_t* __restrict__ xb)
+{
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V.nv) return;
+    int x, y, z;
+    voxel_xyz(V, v, x, y, z);
+    const long gi = ((long)(x + g.G) * g.s1 + (y + g.G)) * g.s2 + (z + g.G);
+#pragma unroll
+    for (int c = 0; c < 3; c++) xb[v * 4 + c] = to_bf16((float)(scale * (double)g.f[PK_JX + c][gi]));
+    xb[v * 4 + 3] = 0;
+}
+
 template <class T>
 __global__ void k_f32_to_grid(GridView<T> g, Vol V, double inv_scale, const float* __restrict__ src)
 {
@@ -445,6 +460,7 @@ unsigned conv_blocks(const Vol& V)
 constexpr size_t conv_lds(int cin) { return sizeof(uint16_t) * HX * HY * HZ * cin; }
 
 // layers 1..3 on d->xb; the output lands in d->out ([3][nv] FP32)
+// the network on d->xb, output to d->out (channel-first FP32)
 int forward(pk_dn* d, const Vol& V)
 {
     hipStream_t s = d->ctx->stream;
@@ -453,8 +469,8 @@ int forward(pk_dn* d, const Vol& V)
                        d->a1, (float*)nullptr, (const uint16_t*)nullptr);
     hipLaunchKernelGGL((k_conv<16, OUT_RELU_BF16>), dim3(nb), dim3(256), conv_lds(16), s, V, d->a1, d->wf[1], d->p + OFF_B2, HID,
                        d->a2, (float*)nullptr, (const uint16_t*)nullptr);
-    hipLaunchKernelGGL((k_conv<16, OUT_F32>), dim3(nb), dim3(256), conv_lds(16), s, V, d->a2, d->wf[2], d->p + OFF_B3, 3,
-                       (uint16_t*)nullptr, d->out, (const uint16_t*)nullptr);
+    hipLaunchKernelGGL((k_conv<16, OUT_F32>), dim3(nb), dim3(256), conv_lds(16), s, V, d->a2, d->wf[2], d->p + OFF_B3,
+                       3, (uint16_t*)nullptr, d->out, (const uint16_t*)nullptr);
     PK_HIP(d->ctx, hipGetLastError());
     return PK_OK;
 }
@@ -792,13 +808,14 @@ int pk_dn_apply(pk_dn* d, pk_grid* g)
     if (r) return r;
     hipStream_t s = d->ctx->stream;
     pk_ktimer tm(d->ctx, "denoise");
+    // J -> network input in one pass; the output goes back through k_f32_to_grid (coalesced),
+    // which measured faster than writing the grid from the last layer's epilogue
     if (g->prec == PK_F64)
-        hipLaunchKernelGGL(k_grid_to_f32<double>, dim3(nblk(V.nv, 256)), dim3(256), 0, s, grid_view<double>(g), V,
-                           d->scale, d->tgt);
+        hipLaunchKernelGGL(k_grid_to_input<double>, dim3(nblk(V.nv, 256)), dim3(256), 0, s, grid_view<double>(g), V,
+                           d->scale, d->xb);
     else
-        hipLaunchKernelGGL(k_grid_to_f32<float>, dim3(nblk(V.nv, 256)), dim3(256), 0, s, grid_view<float>(g), V,
-                           d->scale, d->tgt);
-    hipLaunchKernelGGL(k_input_f32, dim3(nblk(V.nv, 256)), dim3(256), 0, s, V.nv, d->tgt, 1.0, d->xb);
+        hipLaunchKernelGGL(k_grid_to_input<float>, dim3(nblk(V.nv, 256)), dim3(256), 0, s, grid_view<float>(g), V,
+                           d->scale, d->xb);
     if ((r = forward(d, V))) return r;
     if (g->prec == PK_F64)
         hipLaunchKernelGGL(k_f32_to_grid<double>, dim3(nblk(V.nv, 256)), dim3(256), 0, s, grid_view<double>(g), V,

@@ -54,34 +54,35 @@ __global__ void k_residual(GridView<T> g, Phys P, const double* __restrict__ rho
     w[v] = Cx<T>{(T)r, (T)0};
 }
 
-// w[v] = (D (J - Jref), 0): the residual when Jref (saved physical J, FP64, [3][nv]) satisfies
-// continuity exactly, as the Esirkepov current does
+// D J at physical node v, FP64; J ghosts must be filled
 template <class T>
-__global__ void k_residual_ref(GridView<T> g, Phys P, const double* __restrict__ jref, Cx<T>* __restrict__ w)
+__device__ __forceinline__ double div_j(const GridView<T>& g, const Phys& P, long v)
 {
-    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= P.nv) return;
     const int i = (int)(v / ((long)P.n[1] * P.n[2])), j = (int)((v / P.n[2]) % P.n[1]), k = (int)(v % P.n[2]);
     const long c = grid_index(g.s1, g.s2, g.G, i, j, k);
     const long s12 = (long)g.s1 * g.s2;
-    const int ip = i + 1 == P.n[0] ? 0 : i + 1, jp = j + 1 == P.n[1] ? 0 : j + 1, kp = k + 1 == P.n[2] ? 0 : k + 1;
-    const long vx = ((long)ip * P.n[1] + j) * P.n[2] + k, vy = ((long)i * P.n[1] + jp) * P.n[2] + k,
-               vz = ((long)i * P.n[1] + j) * P.n[2] + kp;
-    double r = (((double)g.f[PK_JX][c + s12] - jref[vx]) - ((double)g.f[PK_JX][c] - jref[v])) / g.step[0];
-    r += (((double)g.f[PK_JY][c + g.s2] - jref[P.nv + vy]) - ((double)g.f[PK_JY][c] - jref[P.nv + v])) / g.step[1];
-    r += (((double)g.f[PK_JZ][c + 1] - jref[2 * P.nv + vz]) - ((double)g.f[PK_JZ][c] - jref[2 * P.nv + v])) / g.step[2];
-    w[v] = Cx<T>{(T)r, (T)0};
+    double r = ((double)g.f[PK_JX][c + s12] - (double)g.f[PK_JX][c]) / g.step[0];
+    r += ((double)g.f[PK_JY][c + g.s2] - (double)g.f[PK_JY][c]) / g.step[1];
+    r += ((double)g.f[PK_JZ][c + 1] - (double)g.f[PK_JZ][c]) / g.step[2];
+    return r;
 }
 
+// w[v] = (D J - d0[v], 0) = (D (J - Jref), 0): the residual when the current Jref, whose
+// divergence d0 was saved, satisfies continuity exactly, as the Esirkepov current does
 template <class T>
-__global__ void k_save_j(GridView<T> g, Phys P, double* __restrict__ jref)
+__global__ void k_residual_ref(GridView<T> g, Phys P, const double* __restrict__ d0, Cx<T>* __restrict__ w)
 {
     const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= P.nv) return;
-    const int i = (int)(v / ((long)P.n[1] * P.n[2])), j = (int)((v / P.n[2]) % P.n[1]), k = (int)(v % P.n[2]);
-    const long c = grid_index(g.s1, g.s2, g.G, i, j, k);
-#pragma unroll
-    for (int d = 0; d < 3; d++) jref[d * P.nv + v] = (double)g.f[PK_JX + d][c];
+    w[v] = Cx<T>{(T)(div_j(g, P, v) - d0[v]), (T)0};
+}
+
+template <class T>
+__global__ void k_save_div(GridView<T> g, Phys P, double* __restrict__ d0)
+{
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= P.nv) return;
+    d0[v] = div_j(g, P, v);
 }
 
 // in-place radix-2 FFTs of pb pencils per workgroup: pencil p holds w[base(p) + j * stride],
@@ -89,14 +90,23 @@ __global__ void k_save_j(GridView<T> g, Phys P, double* __restrict__ jref)
 // logarithms); tw[k] = exp(-2 pi i k / n), k < n / 2;
 // inverse = conjugate twiddles, no 1/n.  `along` = 1 when a pencil is contiguous (z), else the
 // pencils of a workgroup are adjacent in memory and are read across pencils (coalesced).
+// solve != null: each value is first divided by lambda(k) nv, the Laplacian's eigenvalue times
+// the volume (lambda = solve[i] + solve[n0 + j] + solve[n0 + n1 + k]; the mean mode is zeroed) --
+// the Poisson solve folded into the first inverse pass.
 template <class R>
 __global__ void __launch_bounds__(256) k_fft(Cx<R>* __restrict__ w, int logn, long npencil, int logpb, int loginner,
                                              long ostride, long stride, int along, const double2* __restrict__ tw,
-                                             int inverse)
+                                             int inverse, const double* __restrict__ solve, Phys P)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    Cx<R>* a = (Cx<R>*)smem;          // pb x n
     const int n = 1 << logn, pb = 1 << logpb;
+    Cx<R>* twl = (Cx<R>*)smem;        // n / 2 twiddles, conjugated for the inverse
+    Cx<R>* a = twl + (n >> 1);        // pb rows of n + 1 (the pad spreads a row's start over the banks)
+    const int row = n + 1;
+    for (int t = threadIdx.x; t < (n >> 1); t += blockDim.x) {
+        const double2 wd = tw[t];
+        twl[t] = Cx<R>{(R)wd.x, inverse ? (R)-wd.y : (R)wd.y};
+    }
     const long inner_mask = (1l << loginner) - 1;
     const long p0 = (long)blockIdx.x << logpb;
     const int tot = pb << logn;
@@ -108,15 +118,24 @@ __global__ void __launch_bounds__(256) k_fft(Cx<R>* __restrict__ w, int logn, lo
             const int idx = i0 + 256 * k;
             const int pl = along ? idx >> logn : idx & (pb - 1), j = along ? idx & (n - 1) : idx >> logpb;
             const long p = p0 + pl;
-            if (idx < tot && p < npencil)
-                v[k] = w[((p >> loginner) * ostride + (p & inner_mask)) + (long)j * stride];
+            if (idx < tot && p < npencil) {
+                const long gv = ((p >> loginner) * ostride + (p & inner_mask)) + (long)j * stride;
+                v[k] = w[gv];
+                if (solve) {
+                    const long n12 = (long)P.n[1] * P.n[2];
+                    const int vi = (int)(gv / n12), vj = (int)((gv / P.n[2]) % P.n[1]), vk = (int)(gv % P.n[2]);
+                    const double lam = solve[vi] + solve[P.n[0] + vj] + solve[P.n[0] + P.n[1] + vk];
+                    const R f = gv == 0 ? (R)0 : (R)(1.0 / (lam * (double)P.nv));
+                    v[k] = Cx<R>{v[k].x * f, v[k].y * f};
+                }
+            }
         }
 #pragma unroll
         for (int k = 0; k < CH; k++) {
             const int idx = i0 + 256 * k;
             const int pl = along ? idx >> logn : idx & (pb - 1), j = along ? idx & (n - 1) : idx >> logpb;
             if (idx < tot && p0 + pl < npencil)
-                a[(pl << logn) + (int)(__brev((unsigned)j) >> (32 - logn))] = v[k];
+                a[pl * row + (int)(__brev((unsigned)j) >> (32 - logn))] = v[k];
         }
     }
     __syncthreads();
@@ -126,9 +145,9 @@ __global__ void __launch_bounds__(256) k_fft(Cx<R>* __restrict__ w, int logn, lo
         for (int t = threadIdx.x; t < nb; t += blockDim.x) {
             const int pl = t >> (logn - 1), tt = t & ((n >> 1) - 1);
             const int grp = tt >> lh, pos = tt & (half - 1);
-            const int i0 = (pl << logn) + (grp << (lh + 1)) + pos, i1 = i0 + half;
-            const double2 wd = tw[pos << (logn - 1 - lh)];
-            const R wx = (R)wd.x, wy = inverse ? (R)-wd.y : (R)wd.y;
+            const int i0 = pl * row + (grp << (lh + 1)) + pos, i1 = i0 + half;
+            const Cx<R> wt = twl[pos << (logn - 1 - lh)];
+            const R wx = wt.x, wy = wt.y;
             const Cx<R> u = a[i0], x = a[i1];
             const R vx = x.x * wx - x.y * wy, vy = x.x * wy + x.y * wx;
             a[i0] = Cx<R>{u.x + vx, u.y + vy};
@@ -141,25 +160,8 @@ __global__ void __launch_bounds__(256) k_fft(Cx<R>* __restrict__ w, int logn, lo
         const long p = p0 + pl;
         if (p >= npencil) continue;
         const long base = (p >> loginner) * ostride + (p & inner_mask);
-        w[base + (long)j * stride] = a[(pl << logn) + j];
+        w[base + (long)j * stride] = a[pl * row + j];
     }
-}
-
-// phi^ = r^ / (lambda(k) N), phi^(0) = 0
-template <class R>
-__global__ void k_solve(Phys P, const double* __restrict__ lam0, const double* __restrict__ lam1,
-                        const double* __restrict__ lam2, Cx<R>* __restrict__ w)
-{
-    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= P.nv) return;
-    const int i = (int)(v / ((long)P.n[1] * P.n[2])), j = (int)((v / P.n[2]) % P.n[1]), k = (int)(v % P.n[2]);
-    const double lam = lam0[i] + lam1[j] + lam2[k];
-    if (v == 0) {
-        w[v] = Cx<R>{(R)0, (R)0};
-        return;
-    }
-    const R f = (R)(1.0 / (lam * (double)P.nv));
-    w[v] = Cx<R>{w[v].x * f, w[v].y * f};
 }
 
 // J_d[c] -= (phi[c] - phi[c - e_d]) / h_d on the physical faces
@@ -194,12 +196,12 @@ int log2_exact(int n)
 }
 
 // work layout in g->proj (doubles): complex volume (2 nv), twiddles (n0 + n1 + n2), eigenvalues
-// (n0 + n1 + n2), reference current (3 nv)
+// (n0 + n1 + n2), divergence of the reference current (nv)
 double* jref_of(pk_grid* g, const Phys& P) { return g->proj + 2 * P.nv + 2 * (P.n[0] + P.n[1] + P.n[2]); }
 
 int ensure_proj(pk_grid* g, const Phys& P)
 {
-    const size_t need = 5 * (size_t)P.nv + 2 * (size_t)(P.n[0] + P.n[1] + P.n[2]) + 16;
+    const size_t need = 3 * (size_t)P.nv + 2 * (size_t)(P.n[0] + P.n[1] + P.n[2]) + 16;
     if (g->proj && g->proj_n >= need) return PK_OK;
     PK_HIP(g->ctx, hipStreamSynchronize(g->ctx->stream));
     if (g->proj) (void)hipFree(g->proj);
@@ -228,8 +230,9 @@ int ensure_proj(pk_grid* g, const Phys& P)
     return PK_OK;
 }
 
+// forward: z, y, x passes; inverse: x (with the Poisson solve when lam != null), y, z
 template <class R>
-int fft3(pk_grid* g, const Phys& P, Cx<R>* w, const double2* tw[3], int inverse)
+int fft3(pk_grid* g, const Phys& P, Cx<R>* w, const double2* tw[3], int inverse, const double* lam)
 {
     hipStream_t s = g->ctx->stream;
     const long n12 = (long)P.n[1] * P.n[2];
@@ -241,17 +244,18 @@ int fft3(pk_grid* g, const Phys& P, Cx<R>* w, const double2* tw[3], int inverse)
         const int d = inverse ? dd : 2 - dd;
         const int n = P.n[d];
         const long npencil = P.nv / n;
-        const int pb = n <= 256 ? 16 : 4096 / n;          // at most 64 KB of LDS
-        hipLaunchKernelGGL(k_fft<R>, dim3((unsigned)((npencil + pb - 1) / pb)), dim3(256), sizeof(Cx<R>) * pb * n, s,
-                           w, log2_exact(n), npencil, log2_exact(pb), log2_exact((int)inner[d]), ostride[d], stride[d],
-                           d == 2 ? 1 : 0, tw[d], inverse);
+        const int pb = n <= 256 ? 16 : 4096 / n;          // at most 74 KB of LDS (FP64, n = 1024)
+        hipLaunchKernelGGL(k_fft<R>, dim3((unsigned)((npencil + pb - 1) / pb)), dim3(256),
+                           sizeof(Cx<R>) * (pb * (n + 1) + n / 2), s, w, log2_exact(n), npencil, log2_exact(pb),
+                           log2_exact((int)inner[d]), ostride[d], stride[d], d == 2 ? 1 : 0, tw[d], inverse,
+                           (inverse && dd == 0) ? lam : nullptr, P);
     }
     PK_HIP(g->ctx, hipGetLastError());
     return PK_OK;
 }
 
-// rho1 != null: residual D J + (rho1 - rho_prev) / dt; else D (J - jref) with jref after the
-// complex volume in g->proj
+// rho1 != null: residual D J + (rho1 - rho_prev) / dt; else D J - d0 with the divergence d0 of
+// the reference current saved after the complex volume in g->proj
 template <class T>
 int project(pk_grid* g, const Phys& P, const double* rho1, double dt)
 {
@@ -270,10 +274,8 @@ int project(pk_grid* g, const Phys& P, const double* rho1, double dt)
         hipLaunchKernelGGL(k_residual_ref<T>, dim3(nblk(P.nv, 256)), dim3(256), 0, s, grid_view<T>(g), P,
                            jref_of(g, P), w);
     int r;
-    if ((r = fft3(g, P, w, tw, 0))) return r;
-    hipLaunchKernelGGL(k_solve<T>, dim3(nblk(P.nv, 256)), dim3(256), 0, s, P, lam, lam + P.n[0],
-                       lam + P.n[0] + P.n[1], w);
-    if ((r = fft3(g, P, w, tw, 1))) return r;
+    if ((r = fft3(g, P, w, tw, 0, nullptr))) return r;
+    if ((r = fft3(g, P, w, tw, 1, lam))) return r;
     hipLaunchKernelGGL(k_correct<T>, dim3(nblk(P.nv, 256)), dim3(256), 0, s, grid_view<T>(g), P, w);
     PK_HIP(g->ctx, hipGetLastError());
     return PK_OK;
@@ -359,10 +361,10 @@ int pk_step_denoised(pk_ens* e, pk_grid* g, pk_dn* d, double dt, int shape, int 
     if (dep == PK_ESIRKEPOV) {
         if ((r = ensure_proj(g, P))) return r;
         if (g->prec == PK_F64)
-            hipLaunchKernelGGL(k_save_j<double>, dim3(nblk(P.nv, 256)), dim3(256), 0, g->ctx->stream,
+            hipLaunchKernelGGL(k_save_div<double>, dim3(nblk(P.nv, 256)), dim3(256), 0, g->ctx->stream,
                                grid_view<double>(g), P, jref_of(g, P));
         else
-            hipLaunchKernelGGL(k_save_j<float>, dim3(nblk(P.nv, 256)), dim3(256), 0, g->ctx->stream,
+            hipLaunchKernelGGL(k_save_div<float>, dim3(nblk(P.nv, 256)), dim3(256), 0, g->ctx->stream,
                                grid_view<float>(g), P, jref_of(g, P));
         PK_HIP(e->ctx, hipGetLastError());
         if ((r = pk_dn_apply(d, g))) return r;

@@ -85,6 +85,32 @@ def test_gradient_matches_oracle(dn):
         o += k
 
 
+@pytest.mark.parametrize("prec", [P.F64, P.F32])
+def test_apply_on_grid_equals_forward(ctx, dn, prec):
+    """pk_dn_apply reads J from the grid and writes the network output back into it; it must give
+    what the forward pass gives on the same scaled J (same BF16 input rounding, FP32 output)"""
+    _say("apply on grid prec=%d" % prec)
+    ncell, scale = (8, 16, 8), 3.0e-4
+    dn.set_params(_params(9))
+    dn.set_scale(scale)
+    g = P.YeeGrid.periodic(ctx, ncell, prec=prec)
+    rng = np.random.default_rng(10)
+    G = O.GHOST
+    j = rng.standard_normal((3,) + ncell) / scale
+    for c, comp in enumerate((P.JX, P.JY, P.JZ)):
+        a = np.zeros(g.get(comp).shape)
+        a[G:-G, G:-G, G:-G] = j[c]
+        g.set(comp, a)
+    jg = np.stack([g.get(c)[G:-G, G:-G, G:-G] for c in (P.JX, P.JY, P.JZ)]).astype(np.float64)
+    ref = dn.forward((jg * scale).astype(np.float32)).astype(np.float64) / scale
+    dn.apply(g)
+    got = np.stack([g.get(c)[G:-G, G:-G, G:-G] for c in (P.JX, P.JY, P.JZ)]).astype(np.float64)
+    err = rel_err(got, ref)
+    print("apply prec=%d: rel err vs forward %.3g" % (prec, err), flush=True)
+    assert err < 1e-6
+    g.close()
+
+
 @pytest.mark.parametrize("prec,tol", [(P.F64, 1e-11), (P.F32, 2e-4)])
 def test_projection_matches_oracle_and_restores_continuity(ctx, prec, tol):
     """direct (non-conserving) deposition, then the projection: J equals the oracle's projection
