@@ -136,9 +136,18 @@ OR_DEFINE_YEE(or_yee_update_e_f32, or_yee_update_b_f32, or_gridf, float)
 
 /* ------------------------------------------------------------------------------------ */
 /* Periodic ghosts, width G on both sides of every dimension.  Physical array indices     */
-/* are [G, G+N).  fill: ghost := its periodic image, dimension by dimension over the full  */
+/* are [G, G+N), N >= 1.  fill: ghost := its periodic image, dimension by dimension over the full */
 /* extent of the other two.  fold: physical += ghost image, low ghosts first, then high.   */
 /* ------------------------------------------------------------------------------------ */
+/* array index of the periodic image of index i; an edge of N < G cells wraps more than once */
+static int or_ghost_image(int i, int G, int N)
+{
+    if (i >= G && i < G + N) return i;
+    int r = (i - G) % N;
+    if (r < 0) r += N;
+    return G + r;
+}
+
 #define OR_DEFINE_GHOST(NAME_FILL, NAME_FOLD, T)                                            \
 void NAME_FILL(const int s[3], int G, T* f)                                                 \
 {                                                                                           \
@@ -149,7 +158,8 @@ void NAME_FILL(const int s[3], int G, T* f)                                     
         for (int p = 0; p < s[o1]; p++)                                                     \
         for (int q = 0; q < s[o2]; q++) {                                                   \
             int ilo[3], isrc[3], ihi[3], isrc2[3];                                          \
-            ilo[d] = a; isrc[d] = a + N; ihi[d] = G + N + a; isrc2[d] = G + a;              \
+            ilo[d] = a; isrc[d] = or_ghost_image(a, G, N);                                  \
+            ihi[d] = G + N + a; isrc2[d] = or_ghost_image(G + N + a, G, N);                 \
             ilo[o1] = isrc[o1] = ihi[o1] = isrc2[o1] = p;                                   \
             ilo[o2] = isrc[o2] = ihi[o2] = isrc2[o2] = q;                                   \
             f[IDX3(s, ilo[0], ilo[1], ilo[2])] = f[IDX3(s, isrc[0], isrc[1], isrc[2])];     \
@@ -166,13 +176,13 @@ void NAME_FOLD(const int s[3], int G, T* f)                                     
         for (int q = 0; q < s[o2]; q++) {                                                   \
             for (int a = 0; a < G; a++) {                                                   \
                 int it[3], ig[3];                                                           \
-                it[d] = a + N; ig[d] = a;                                                   \
+                it[d] = or_ghost_image(a, G, N); ig[d] = a;                                 \
                 it[o1] = ig[o1] = p; it[o2] = ig[o2] = q;                                   \
                 f[IDX3(s, it[0], it[1], it[2])] += f[IDX3(s, ig[0], ig[1], ig[2])];         \
             }                                                                               \
             for (int a = 0; a < G; a++) {                                                   \
                 int it[3], ig[3];                                                           \
-                it[d] = G + a; ig[d] = G + N + a;                                           \
+                it[d] = or_ghost_image(G + N + a, G, N); ig[d] = G + N + a;                 \
                 it[o1] = ig[o1] = p; it[o2] = ig[o2] = q;                                   \
                 f[IDX3(s, it[0], it[1], it[2])] += f[IDX3(s, ig[0], ig[1], ig[2])];         \
             }                                                                               \

@@ -211,7 +211,9 @@ int pk_grid_create(pk_ctx* c, int prec, const int size[3], const double origin[3
     for (int d = 0; d < 3; d++) {
         if (size[d] < 2) return pk_fail(c, PK_E_ARG, "pk_grid_create: size must be >= 2 in every dimension");
         if (!(step[d] > 0.0)) return pk_fail(c, PK_E_ARG, "pk_grid_create: step must be positive");
-        if (ghost < 0 || size[d] - 2 * ghost < ghost) return pk_fail(c, PK_E_ARG, "pk_grid_create: bad ghost width");
+        // at least one physical cell: a 1D / 2D grid is a 3D grid with edges of one cell, whose
+        // periodic ghosts all image that cell
+        if (ghost < 0 || size[d] - 2 * ghost < 1) return pk_fail(c, PK_E_ARG, "pk_grid_create: bad ghost width");
     }
     pk_grid* g = new pk_grid();
     g->ctx = c;

@@ -106,13 +106,12 @@ constexpr int BX = 2, BY = 8, BZ = 16, HX = BX + 2, HY = BY + 2, HZ = BZ + 2;
 constexpr int BRICK_TILES = BX * BY * (BZ / 16);
 static_assert(BRICK_TILES % 8 == 0, "4 wavefronts x 2 rows per pass");
 
-// periodic image of a in [-1, 3n)
+// periodic image of a >= -1 (a brick's halo can reach several periods past an edge shorter
+// than the brick, e.g. the one-cell edge of a 2D grid)
 __device__ __forceinline__ int wrap(int a, int n)
 {
     a += a < 0 ? n : 0;
-    a -= a >= n ? n : 0;
-    a -= a >= n ? n : 0;
-    return a;
+    return a < n ? a : a % n;
 }
 
 template <int CIN, int MODE>

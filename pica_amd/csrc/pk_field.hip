@@ -73,6 +73,16 @@ __global__ void __launch_bounds__(256) k_yee_b(GridView<T> g, YeeCoef<T> k, long
 // Periodic fill: every ghost cell takes the value of its wrapped physical image.  The ghost
 // cells are enumerated as x slabs (full y,z), then y slabs (physical x, full z), then z slabs
 // (physical x,y); the closed-form image equals the dimension-by-dimension copy.
+// array index of the periodic image of index i (physical indices [G, G + N)); an edge of N < G
+// cells (1D / 2D grids: N = 1) wraps more than once
+__device__ __forceinline__ int ghost_image(int i, int G, int N)
+{
+    if (i >= G && i < G + N) return i;
+    int r = (i - G) % N;
+    if (r < 0) r += N;
+    return G + r;
+}
+
 template <class T>
 __device__ __forceinline__ void ghost_fill_at(const GridView<T>& g, unsigned mask, long nx_, long ny_, long nz_, long t)
 {
@@ -99,9 +109,7 @@ __device__ __forceinline__ void ghost_fill_at(const GridView<T>& g, unsigned mas
     } else {
         return;
     }
-    const int si = i < G ? i + N0 : (i >= G + N0 ? i - N0 : i);
-    const int sj = j < G ? j + N1 : (j >= G + N1 ? j - N1 : j);
-    const int sk = k < G ? k + N2 : (k >= G + N2 ? k - N2 : k);
+    const int si = ghost_image(i, G, N0), sj = ghost_image(j, G, N1), sk = ghost_image(k, G, N2);
     const long dst = ((long)i * g.s1 + j) * g.s2 + k;
     const long src = ((long)si * g.s1 + sj) * g.s2 + sk;
     for (int c = 0; c < PK_NCOMP; c++)
@@ -133,8 +141,8 @@ __device__ __forceinline__ void ghost_fold_at(const GridView<T>& g, unsigned mas
     for (int c = 0; c < PK_NCOMP; c++) {
         if (!(mask & (1u << c))) continue;
         T* f = g.f[c];
-        for (int a = 0; a < G; a++) f[base + (a + N) * stride[d]] += f[base + a * stride[d]];
-        for (int a = 0; a < G; a++) f[base + (G + a) * stride[d]] += f[base + (G + N + a) * stride[d]];
+        for (int a = 0; a < G; a++) f[base + ghost_image(a, G, N) * stride[d]] += f[base + a * stride[d]];
+        for (int a = 0; a < G; a++) f[base + ghost_image(G + N + a, G, N) * stride[d]] += f[base + (G + N + a) * stride[d]];
     }
 }
 
@@ -194,8 +202,7 @@ int ghost(pk_grid* g, int ops)
     const int G = g->ghost;
     if (G <= 0) return pk_fail(g->ctx, PK_E_ARG, "pk_ghost: grid has no ghost layers");
     for (int d = 0; d < 3; d++)
-        if (g->s[d] - 2 * G < G)
-            return pk_fail(g->ctx, PK_E_ARG, "pk_ghost: physical extent smaller than the ghost width");
+        if (g->s[d] - 2 * G < 1) return pk_fail(g->ctx, PK_E_ARG, "pk_ghost: no physical cells");
     unsigned fold = 0, fill = 0;
     if (ops & PK_FOLD_J) fold |= (1u << PK_JX) | (1u << PK_JY) | (1u << PK_JZ);
     if (ops & PK_FOLD_RHO) fold |= (1u << PK_RHO);

@@ -195,8 +195,9 @@ int log2_exact(int n)
     return (1 << l) == n ? l : -1;
 }
 
-// work layout in g->proj (doubles): complex volume (2 nv), twiddles (n0 + n1 + n2), eigenvalues
-// (n0 + n1 + n2), divergence of the reference current (nv)
+// work layout in g->proj (doubles): complex volume (2 nv), twiddles (n_d / 2 complex values per
+// dimension), eigenvalues (n0 + n1 + n2), then, at 2 nv + 2 (n0 + n1 + n2), the divergence of the
+// reference current (nv)
 double* jref_of(pk_grid* g, const Phys& P) { return g->proj + 2 * P.nv + 2 * (P.n[0] + P.n[1] + P.n[2]); }
 
 int ensure_proj(pk_grid* g, const Phys& P)
@@ -230,10 +231,12 @@ int ensure_proj(pk_grid* g, const Phys& P)
     return PK_OK;
 }
 
-// forward: z, y, x passes; inverse: x (with the Poisson solve when lam != null), y, z
+// forward: z, y, x passes; inverse: x, y, z, the first of them with the Poisson solve when
+// lam != null.  Edges of one cell (1D / 2D grids) need no pass.
 template <class R>
 int fft3(pk_grid* g, const Phys& P, Cx<R>* w, const double2* tw[3], int inverse, const double* lam)
 {
+    bool first = true;
     hipStream_t s = g->ctx->stream;
     const long n12 = (long)P.n[1] * P.n[2];
     // pencils along x (stride n1 n2), y (stride n2), z (contiguous)
@@ -243,12 +246,14 @@ int fft3(pk_grid* g, const Phys& P, Cx<R>* w, const double2* tw[3], int inverse,
     for (int dd = 0; dd < 3; dd++) {
         const int d = inverse ? dd : 2 - dd;
         const int n = P.n[d];
+        if (n == 1) continue;
         const long npencil = P.nv / n;
         const int pb = n <= 256 ? 16 : 4096 / n;          // at most 74 KB of LDS (FP64, n = 1024)
         hipLaunchKernelGGL(k_fft<R>, dim3((unsigned)((npencil + pb - 1) / pb)), dim3(256),
                            sizeof(Cx<R>) * (pb * (n + 1) + n / 2), s, w, log2_exact(n), npencil, log2_exact(pb),
                            log2_exact((int)inner[d]), ostride[d], stride[d], d == 2 ? 1 : 0, tw[d], inverse,
-                           (inverse && dd == 0) ? lam : nullptr, P);
+                           (inverse && first) ? lam : nullptr, P);
+        first = false;
     }
     PK_HIP(g->ctx, hipGetLastError());
     return PK_OK;
@@ -266,7 +271,8 @@ int project(pk_grid* g, const Phys& P, const double* rho1, double dt)
     tw[0] = (const double2*)tab;
     tw[1] = tw[0] + P.n[0] / 2;
     tw[2] = tw[1] + P.n[1] / 2;
-    const double* lam = tab + (P.n[0] + P.n[1] + P.n[2]);
+    // the twiddle block holds n / 2 complex values per dimension (none for an edge of one cell)
+    const double* lam = tab + 2 * (P.n[0] / 2 + P.n[1] / 2 + P.n[2] / 2);
     if (rho1)
         hipLaunchKernelGGL(k_residual<T>, dim3(nblk(P.nv, 256)), dim3(256), 0, s, grid_view<T>(g), P, rho1,
                            g->rho_prev, 1.0 / dt, w);
@@ -285,10 +291,11 @@ int phys_of(pk_grid* g, Phys& P)
 {
     for (int d = 0; d < 3; d++) {
         P.n[d] = g->s[d] - 2 * g->ghost;
-        if (P.n[d] < 2 || P.n[d] > 1024 || log2_exact(P.n[d]) < 0)
+        if (P.n[d] < 1 || P.n[d] > 1024 || log2_exact(P.n[d]) < 0)
             return pk_fail(g->ctx, PK_E_UNSUPPORTED, "continuity projection: physical edges must be powers of two");
     }
     P.nv = (long)P.n[0] * P.n[1] * P.n[2];
+    if (P.nv < 2) return pk_fail(g->ctx, PK_E_UNSUPPORTED, "continuity projection: grid of a single cell");
     return PK_OK;
 }
 

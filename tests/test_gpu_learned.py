@@ -38,7 +38,7 @@ def dn(ctx):
 
 
 @pytest.mark.parametrize("layer", [1, 2, 3])
-@pytest.mark.parametrize("n", [(6, 5, 8), (5, 3, 7)])
+@pytest.mark.parametrize("n", [(6, 5, 8), (5, 3, 7), (4, 4, 1), (2, 1, 3)])
 def test_conv_layer_matches_oracle(dn, layer, n):
     _say("conv layer %d n=%s" % (layer, n))
     p = _params(1)

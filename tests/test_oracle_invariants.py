@@ -139,25 +139,31 @@ def test_yee_divergences():
     assert np.allclose(dive1 - dive0, -4 * O.PI * dt * divj, rtol=0, atol=1e-12 * np.abs(dive0).max())
 
 
-def test_ghost_fill_and_fold_are_periodic():
+@pytest.mark.parametrize("shape", [(10, 9, 8), (5, 9, 5), (6, 5, 8), (5, 5, 5)])
+def test_ghost_fill_and_fold_are_periodic(shape):
+    """fill copies the periodic image, fold adds every ghost into its image; edges shorter than
+    the ghost width (1D / 2D grids: one physical cell) wrap more than once"""
     rng = np.random.default_rng(4)
-    a = rng.standard_normal((10, 9, 8))
+    a = rng.standard_normal(shape)
     b = a.copy()
     O.ghost_fill(b)
     G = 2
     N = np.array(a.shape) - 2 * G
+    img = [(np.arange(n) - G) % m + G for n, m in zip(a.shape, N)]
     for i in range(a.shape[0]):
         for j in range(a.shape[1]):
             for k in range(a.shape[2]):
-                wi = (i - G) % N[0] + G
-                wj = (j - G) % N[1] + G
-                wk = (k - G) % N[2] + G
-                assert b[i, j, k] == a[wi, wj, wk]
-    # fold: the physical sum equals the sum of everything (nothing lost, nothing doubled)
-    c = rng.standard_normal((10, 9, 8))
-    tot = c.sum()
+                assert b[i, j, k] == a[img[0][i], img[1][j], img[2][k]]
+    # fold: each physical value is the sum of every array entry imaging to it (brute force)
+    c = rng.standard_normal(shape)
+    ref = np.zeros(shape)
+    for i in range(a.shape[0]):
+        for j in range(a.shape[1]):
+            for k in range(a.shape[2]):
+                ref[img[0][i], img[1][j], img[2][k]] += c[i, j, k]
     O.ghost_fold(c)
-    assert math.isclose(c[G:-G, G:-G, G:-G].sum(), tot, rel_tol=1e-13)
+    phys = (slice(G, -G),) * 3
+    np.testing.assert_allclose(c[phys], ref[phys], rtol=1e-13, atol=1e-13)
 
 
 # ------------------------------------------------------------------ TSC / rho / Esirkepov
