@@ -132,6 +132,19 @@ int pk_ens_load_thermal(pk_ens* e, int ppc, double uth, double drift_x, double w
 int pk_ens_load_modes(pk_ens* e, int ppc, double uth, double drift_x, double weight, uint64_t seed, int nmodes,
                       const double* modes);
 
+/* ---- snapshots (no counterpart in pica): raw files for restart, fixtures and training data.
+ * One 160-byte little-endian header ("PKSNAP1\0", kind 1 grid / 2 ensemble, version 1, prec,
+ * posmode, dims[3], ghost or species, ncps[3], nfield, origin[3], step[3], count, nsct, esize)
+ * followed by raw arrays: a grid's PK_NCOMP components over its whole storage (ghosts included);
+ * an ensemble's nsct int32 supercell counts, then its fields in supercell order (F64: x y z ux uy
+ * uz w as float64; F32 split: cell as int32, ox oy oz ux uy uz w as float32).  A load restores
+ * the bits and the order exactly; the target must have the same geometry, species and
+ * precision (else PK_E_ARG). */
+int pk_grid_save(pk_grid* g, const char* path);
+int pk_grid_load(pk_grid* g, const char* path);
+int pk_ens_save(pk_ens* e, const char* path);   /* re-sorts pending movers first */
+int pk_ens_load(pk_ens* e, const char* path);
+
 /* ---- many ensembles stepped together (no counterpart in pica, whose loops are the caller's) ----
  * A batch steps n (ensemble, grid) pairs of one precision and one supercell shape, with uneven
  * grid sizes and particle counts, doing each stage of pk_step for all of them in one launch

@@ -54,7 +54,8 @@ ABI_SYMBOLS = (
     "pk_batch_get", "pk_batch_run", "pk_batch_autotune", "pk_ens_load_modes", "pk_dn_create", "pk_dn_destroy",
     "pk_dn_num_params", "pk_dn_init", "pk_dn_set_params", "pk_dn_get_params", "pk_dn_set_scale", "pk_dn_forward_host",
     "pk_dn_layer_host", "pk_dn_grad_host", "pk_dn_add_pair", "pk_dn_num_pairs", "pk_dn_clear_pairs", "pk_dn_train",
-    "pk_dn_apply", "pk_rho_save", "pk_project_continuity", "pk_step_denoised",
+    "pk_dn_apply", "pk_rho_save", "pk_project_continuity", "pk_step_denoised", "pk_grid_save", "pk_grid_load",
+    "pk_ens_save", "pk_ens_load",
 )
 
 _LIB = None
@@ -138,6 +139,8 @@ def lib():
         L.pk_dn_train.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_uint64, vp]
         L.pk_dn_apply.argtypes = [vp, vp]
         L.pk_rho_save.argtypes = [vp, vp, ctypes.c_int]
+        for name in ("pk_grid_save", "pk_grid_load", "pk_ens_save", "pk_ens_load"):
+            getattr(L, name).argtypes = [vp, ctypes.c_char_p]
         L.pk_project_continuity.argtypes = [vp, vp, ctypes.c_int, ctypes.c_double]
         L.pk_step_denoised.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.c_int, ctypes.c_int]
         L.pk_batch_create.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.POINTER(vp)]
@@ -274,6 +277,14 @@ class YeeGrid:
     def ghost_ops(self, ops):
         self.ctx.check(lib().pk_ghost(self.h, ops))
 
+    def save(self, path):
+        """raw snapshot of every component (pica_amd.snapshot reads it without a GPU)"""
+        self.ctx.check(lib().pk_grid_save(self.h, os.fsencode(path)))
+
+    def load(self, path):
+        """restore a snapshot of a grid of the same geometry and precision, bit for bit"""
+        self.ctx.check(lib().pk_grid_load(self.h, os.fsencode(path)))
+
     def close(self):
         if getattr(self, "h", None):
             if self.ctx.h:
@@ -363,6 +374,15 @@ class EnsembleSupercells:
 
     def resort(self):
         self.ctx.check(lib().pk_ens_resort(self.h))
+
+    def save(self, path):
+        """raw snapshot of the particles in supercell order (pica_amd.snapshot reads it)"""
+        self.ctx.check(lib().pk_ens_save(self.h, os.fsencode(path)))
+
+    def load(self, path):
+        """restore a snapshot of an ensemble of the same geometry, species and precision: the
+        same particles, bits and order, so a continued run equals the uninterrupted one"""
+        self.ctx.check(lib().pk_ens_load(self.h, os.fsencode(path)))
 
     def close(self):
         if getattr(self, "h", None):

@@ -24,6 +24,7 @@ SOURCES = [
     ("pk_batch.hip", ["-ffp-contract=off"]),
     ("pk_denoise.hip", ["-ffp-contract=fast"]),
     ("pk_project.hip", ["-ffp-contract=off"]),
+    ("pk_snapshot.hip", ["-ffp-contract=off"]),
 ]
 
 COMMON = [
