@@ -220,6 +220,10 @@ int pk_ens_resort(pk_ens* e);
 /* a whole periodic PIC step: J = 0, particle step, fold J, B(dt/2), E(dt), B(dt/2) with ghost
  * fills, re-sort */
 int pk_step(pk_ens* e, pk_grid* g, double dt, int shape, int dep);
+/* pk_step for several species on one grid (pica's ParticleTypes table, Particle.h:26-31, with
+ * one ensemble per type set by pk_species_set): every ensemble's particle stage adds to J, then
+ * one field update; ensembles may be electrons and mobile ions alike */
+int pk_step_species(pk_ens* const* ens, int n, pk_grid* g, double dt, int shape, int dep);
 int pk_run(pk_ens* e, pk_grid* g, double dt, int shape, int dep, int nsteps);
 
 /* ---- diagnostics (FP64 reductions on the device) ----

@@ -241,6 +241,17 @@ void step(EnsembleSupercells<Real>& ens, YeeGrid<Real>& grid, double dt, FormFac
     grid.context().check(pk_run(ens.handle(), grid.handle(), dt, (int)ff, (int)dep, nsteps));
 }
 
+// One step of several species (electrons, mobile ions, ...) on one grid: every ensemble's
+// particles deposit into the same J before the field update.
+template <class Real>
+void step(std::vector<EnsembleSupercells<Real>*>& species, YeeGrid<Real>& grid, double dt,
+          FormFactor ff = FormFactor::CIC, Depositor dep = Depositor::Direct)
+{
+    std::vector<pk_ens*> h;
+    for (auto* e : species) h.push_back(e->handle());
+    grid.context().check(pk_step_species(h.data(), (int)h.size(), grid.handle(), dt, (int)ff, (int)dep));
+}
+
 struct Diagnostics {
     double fieldEnergy, kineticEnergy, charge, gaussMax, gaussDrift, rhoMax;
 };

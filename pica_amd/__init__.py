@@ -55,7 +55,7 @@ ABI_SYMBOLS = (
     "pk_dn_num_params", "pk_dn_init", "pk_dn_set_params", "pk_dn_get_params", "pk_dn_set_scale", "pk_dn_forward_host",
     "pk_dn_layer_host", "pk_dn_grad_host", "pk_dn_add_pair", "pk_dn_num_pairs", "pk_dn_clear_pairs", "pk_dn_train",
     "pk_dn_apply", "pk_rho_save", "pk_project_continuity", "pk_step_denoised", "pk_grid_save", "pk_grid_load",
-    "pk_ens_save", "pk_ens_load",
+    "pk_ens_save", "pk_ens_load", "pk_step_species",
 )
 
 _LIB = None
@@ -139,6 +139,7 @@ def lib():
         L.pk_dn_train.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_uint64, vp]
         L.pk_dn_apply.argtypes = [vp, vp]
         L.pk_rho_save.argtypes = [vp, vp, ctypes.c_int]
+        L.pk_step_species.argtypes = [vp, ctypes.c_int, vp, ctypes.c_double, ctypes.c_int, ctypes.c_int]
         for name in ("pk_grid_save", "pk_grid_load", "pk_ens_save", "pk_ens_load"):
             getattr(L, name).argtypes = [vp, ctypes.c_char_p]
         L.pk_project_continuity.argtypes = [vp, vp, ctypes.c_int, ctypes.c_double]
@@ -547,6 +548,12 @@ def particle_step(ens, grid, dt, shape=CIC, dep=DIRECT):
 
 def step(ens, grid, dt, shape=CIC, dep=DIRECT):
     ens.ctx.check(lib().pk_step(ens.h, grid.h, float(dt), shape, dep))
+
+
+def step_species(ensembles, grid, dt, shape=CIC, dep=DIRECT):
+    """one PIC step of several species (electrons, mobile ions, ...) sharing one grid"""
+    hs = (ctypes.c_void_p * len(ensembles))(*[e.h for e in ensembles])
+    grid.ctx.check(lib().pk_step_species(hs, len(ensembles), grid.h, float(dt), shape, dep))
 
 
 def run(ens, grid, dt, nsteps, shape=CIC, dep=DIRECT):

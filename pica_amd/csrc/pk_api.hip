@@ -476,6 +476,29 @@ int pk_step(pk_ens* e, pk_grid* g, double dt, int shape, int dep)
     return pk_ens_resort(e);
 }
 
+int pk_step_species(pk_ens* const* ens, int n, pk_grid* g, double dt, int shape, int dep)
+{
+    if (!ens || n < 1 || !g) return PK_E_ARG;
+    int r;
+    for (int k = 0; k < n; k++)
+        if ((r = pk_check_pair(ens[k], g))) return r;
+    for (int comp = PK_JX; comp <= PK_JZ; comp++)
+        if ((r = pk_field_zero(g, comp))) return r;
+    // every species deposits into the same J (the particle stage adds to it)
+    for (int k = 0; k < n; k++)
+        if ((r = pk_particle_step(ens[k], g, dt, shape, dep))) return r;
+    if ((r = pk_field_ghost(g, PK_FOLD_J | PK_FILL_J))) return r;
+    if ((r = pk_field_yee_b(g, dt * 0.5))) return r;
+    if ((r = pk_field_ghost(g, PK_FILL_B))) return r;
+    if ((r = pk_field_yee_e(g, dt))) return r;
+    if ((r = pk_field_ghost(g, PK_FILL_E))) return r;
+    if ((r = pk_field_yee_b(g, dt * 0.5))) return r;
+    if ((r = pk_field_ghost(g, PK_FILL_B))) return r;
+    for (int k = 0; k < n; k++)
+        if ((r = pk_ens_resort(ens[k]))) return r;
+    return PK_OK;
+}
+
 int pk_run(pk_ens* e, pk_grid* g, double dt, int shape, int dep, int nsteps)
 {
     for (int i = 0; i < nsteps; i++) {
