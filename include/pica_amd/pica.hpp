@@ -119,6 +119,9 @@ public:
     }
     void zero(Component c) { ctx_.check(pk_grid_zero(h_, c)); }
     void fillGhostsEB() { ctx_.check(pk_ghost(h_, PK_FILL_E | PK_FILL_B)); }
+    // snapshot files (restart, fixtures): every component, ghosts included, bit for bit
+    void save(const std::string& path) const { ctx_.check(pk_grid_save(h_, path.c_str())); }
+    void load(const std::string& path) { ctx_.check(pk_grid_load(h_, path.c_str())); }
     pk_grid* handle() const { return h_; }
     Context& context() const { return ctx_; }
 
@@ -209,6 +212,9 @@ public:
         if (n < 0) ctx_.check(PK_E_STATE);
         return n;
     }
+    // snapshot files: the particles, bits and order; a load needs the same geometry and species
+    void save(const std::string& path) { ctx_.check(pk_ens_save(h_, path.c_str())); }
+    void load(const std::string& path) { ctx_.check(pk_ens_load(h_, path.c_str())); }
     Int3 getNumCells() const { return numCells_; }
     Int3 getNumCellsPerSupercell() const { return ncps_; }
     Int3 getNumSupercells() const

@@ -1,5 +1,6 @@
 """The C++ surface (include/pica_amd/pica.hpp, pica's names over the C ABI) driven by a plain
-g++ program, checked against the oracle: particles and fields of a short TSC/Esirkepov run."""
+g++ program, checked against the oracle: particles and fields of a short TSC/Esirkepov run,
+restarted halfway from snapshot files and finished through the several-species step."""
 import os
 import subprocess
 
