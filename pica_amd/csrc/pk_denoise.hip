@@ -477,7 +477,7 @@ int forward(pk_dn* d, const Vol& V)
 int wgrad(pk_dn* d, const Vol& V, const uint16_t* in, int cin_pad, int cin, const uint16_t* g, int gpad, int cout,
           float* dW)
 {
-    const long chunk = 1024;
+    const long chunk = 256;          // voxels per wavefront (more wavefronts than taps x volume / 4096)
     const unsigned bx = (unsigned)((V.nv + 4 * chunk - 1) / (4 * chunk));
     if (cin_pad == 16)
         hipLaunchKernelGGL(k_wgrad<16>, dim3(bx, 27), dim3(256), 0, d->ctx->stream, V, in, g, gpad, cin, cout, chunk, dW);
