@@ -59,17 +59,28 @@ struct InsertArgs {
     int32_t* flags;
 };
 
-// one wavefront appends the incoming movers of supercell s, class by class; an overflow of the
-// slack leaves the supercell as it is, raises the ensemble's overflow flag (and *pending)
+// INSERT_SC supercells per workgroup, one per wavefront (XCD-aware over groups of them);
+// INSERT_LIST sources of incoming particles per wavefront in LDS
+constexpr int INSERT_SC = 4, INSERT_LIST = 256;
+
+// one wavefront appends the incoming movers of supercell s in class order (the class-d leavers
+// of the class-d neighbour, in that neighbour's outbox order); an overflow of the slack leaves
+// the supercell as it is, raises the ensemble's overflow flag (and *pending).
+// Three memory round trips: the 26 class counts and source outboxes; the first 64 direction
+// bytes of every source outbox; the copies, one lane per arrival.  Between them the arrivals'
+// sources are listed in LDS.  More than INSERT_LIST arrivals, or a source outbox above 64
+// entries, take the class-by-class path.
 template <class W, int NF>
 __device__ __forceinline__ void insert_sc(const InsertArgs<W>& A, long s, int* pending)
 {
+    __shared__ long list_all[INSERT_SC][INSERT_LIST];
     const EnsGeom& eg = A.eg;
     const int lane = threadIdx.x & 63;
+    long* list = list_all[(threadIdx.x >> 6) % INSERT_SC];
     const int nfinal = A.nstay[s];
     const long cap = A.off[s + 1] - A.off[s];
     // lane d < 27 (d != 13): the neighbour the class-d leavers come from, their count, and that
-    // neighbour's outbox (all loaded at once: one memory latency, not one per class)
+    // neighbour's outbox
     int m_d = 0, nm_d = 0;
     long ob_d = 0;
     if (lane < 27 && lane != 13) {
@@ -91,9 +102,45 @@ __device__ __forceinline__ void insert_sc(const InsertArgs<W>& A, long s, int* p
         return;
     }
     if (lane == 0) A.cnt[s] = nfinal + nin;
-    long p = A.off[s] + nfinal;
-    // the classes with leavers, in class order
-    for (uint64_t todo = __ballot(m_d > 0); todo; todo &= todo - 1ull) {
+    if (nin == 0) return;
+    const uint64_t todo0 = __ballot(m_d > 0);
+    const long base = A.off[s] + nfinal;
+    const bool small = nin <= INSERT_LIST && __ballot(nm_d > 64) == 0ull;
+    if (small) {
+        // every source outbox has at most 64 entries: lane k loads entry k of each at once
+        uint8_t od[27];
+#pragma unroll
+        for (int d = 0; d < 27; d++) {
+            const int nm = __shfl(nm_d, d);
+            const long ob = __shfl(ob_d, d);
+            od[d] = ((todo0 >> d) & 1ull) && lane < nm ? A.odir[ob + lane] : (uint8_t)255;
+        }
+        int p = 0, found = 0;
+#pragma unroll
+        for (int d = 0; d < 27; d++) {
+            if (!((todo0 >> d) & 1ull)) continue;
+            const bool f = od[d] == d;
+            const uint64_t bal = __ballot(f);
+            const int t = p + __popcll(bal & ((1ull << lane) - 1ull));
+            const long ob = __shfl(ob_d, d);       // by every lane: lane d may not take the branch
+            if (f && t < nin) list[t] = ob + lane;
+            p += __shfl(m_d, d);
+            found += __popcll(bal);
+        }
+        if (found != nin) {            // outbox and class counts disagree: report, copy nothing
+            if (lane == 0) atomicOr(&A.flags[PK_FLAG_ERR], 1);
+            return;
+        }
+        __builtin_amdgcn_wave_barrier();
+        for (int t = lane; t < nin; t += 64) {
+            const long src = list[t];
+#pragma unroll
+            for (int q = 0; q < NF; q++) A.pm.f[q][base + t] = A.po.f[q][src];
+        }
+        return;
+    }
+    long p = base;
+    for (uint64_t todo = todo0; todo; todo &= todo - 1ull) {
         const int d = __ffsll((long long)todo) - 1;
         const int m = __shfl(m_d, d), nm = __shfl(nm_d, d);
         const long ob = __shfl(ob_d, d);
@@ -113,8 +160,7 @@ __device__ __forceinline__ void insert_sc(const InsertArgs<W>& A, long s, int* p
     }
 }
 
-// INSERT_SC supercells per workgroup, one per wavefront (XCD-aware over groups of them)
-constexpr int INSERT_SC = 4;
+
 
 template <class W, int NF>
 __global__ void __launch_bounds__(64 * INSERT_SC) k_insert(InsertArgs<W> A, int gchunk)
