@@ -77,8 +77,13 @@ int pk_timer_reset(pk_ctx* ctx);
  * atomic) or PK_KERNEL_CELL_SORTED (particles sorted by cell in LDS, per-cell register sums;
  * 2^3 or 4^3 supercells).
  * PK_OPT_CHANGE_QUEUE caps the cell-sorted kernel's LDS queue of cell-changing particles
- * (0 = as many entries as the LDS budget allows; beyond the queue a lane deposits at once). */
-enum { PK_OPT_PARTICLE_KERNEL = 0, PK_OPT_CHANGE_QUEUE = 1, PK_OPT_N };
+ * (0 = as many entries as the LDS budget allows; beyond the queue a lane deposits at once).
+ * PK_OPT_INTERLEAVE (default 4): every that many re-sorts of an FP32 ensemble, pk_ens_resort
+ * reorders each supercell's particles by (rank among the particles of their cell, cell), so that
+ * consecutive particles -- the lanes of a wavefront -- sit in different cells and their LDS
+ * deposits spread over the banks.  0 turns it off.  Supercells of more than 64 cells or 2048
+ * particles are left as they are.  Batched stepping (pk_batch_run) does not reorder. */
+enum { PK_OPT_PARTICLE_KERNEL = 0, PK_OPT_CHANGE_QUEUE = 1, PK_OPT_INTERLEAVE = 2, PK_OPT_N };
 enum { PK_KERNEL_AUTO = 0, PK_KERNEL_PER_PARTICLE = 1, PK_KERNEL_CELL_SORTED = 2 };
 int pk_ctx_set_option(pk_ctx* ctx, int option, int value);
 int pk_ctx_get_option(pk_ctx* ctx, int option, int* value);
@@ -124,6 +129,9 @@ int64_t pk_ens_num_supercells(pk_ens* e);
 /* per-cell particle counts (physical cells, z fastest) and per-particle cell keys in
  * supercell order (linear physical cell index), either may be NULL */
 int pk_ens_cells(pk_ens* e, int32_t* cell_counts, int32_t* cell_of_particle);
+/* reorder every supercell of an FP32 ensemble now, as PK_OPT_INTERLEAVE does periodically
+ * (no counterpart in pica); a no-op for FP64 ensembles */
+int pk_ens_interleave(pk_ens* e);
 /* seeded thermal loader on the device: ppc particles per cell at uniform random positions,
  * u ~ N(0, uth) per component, plus +drift_x on even and -drift_x on odd particle ids. */
 int pk_ens_load_thermal(pk_ens* e, int ppc, double uth, double drift_x, double weight, uint64_t seed);
@@ -134,7 +142,8 @@ int pk_ens_load_modes(pk_ens* e, int ppc, double uth, double drift_x, double wei
 
 /* ---- snapshots (no counterpart in pica): raw files for restart, fixtures and training data.
  * One 160-byte little-endian header ("PKSNAP1\0", kind 1 grid / 2 ensemble, version 1, prec,
- * posmode, dims[3], ghost or species, ncps[3], nfield, origin[3], step[3], count, nsct, esize)
+ * posmode, dims[3], ghost or species, ncps[3], nfield, origin[3], step[3], count, nsct, esize,
+ * re-sort count)
  * followed by raw arrays: a grid's PK_NCOMP components over its whole storage (ghosts included);
  * an ensemble's nsct int32 supercell counts, then its fields in supercell order (F64: x y z ux uy
  * uz w as float64; F32 split: cell as int32, ox oy oz ux uy uz w as float32).  A load restores

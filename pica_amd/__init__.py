@@ -32,7 +32,7 @@ H2D, D2H = 0, 1
 FILL_E, FILL_B, FILL_J, FOLD_J, FOLD_RHO = 1, 2, 4, 8, 16
 DIAG_FIELD_ENERGY, DIAG_KINETIC_ENERGY, DIAG_CHARGE, DIAG_GAUSS_MAX, DIAG_GAUSS_DRIFT, DIAG_RHO_MAX = range(6)
 DIAG_N = 6
-OPT_PARTICLE_KERNEL, OPT_CHANGE_QUEUE = 0, 1
+OPT_PARTICLE_KERNEL, OPT_CHANGE_QUEUE, OPT_INTERLEAVE = 0, 1, 2
 BATCH_WG_PER_CU, BATCH_ORDER, BATCH_GRAPH = 0, 1, 2
 KERNEL_AUTO, KERNEL_PER_PARTICLE, KERNEL_CELL_SORTED = 0, 1, 2
 
@@ -55,7 +55,7 @@ ABI_SYMBOLS = (
     "pk_dn_num_params", "pk_dn_init", "pk_dn_set_params", "pk_dn_get_params", "pk_dn_set_scale", "pk_dn_forward_host",
     "pk_dn_layer_host", "pk_dn_grad_host", "pk_dn_add_pair", "pk_dn_num_pairs", "pk_dn_clear_pairs", "pk_dn_train",
     "pk_dn_apply", "pk_rho_save", "pk_project_continuity", "pk_step_denoised", "pk_grid_save", "pk_grid_load",
-    "pk_ens_save", "pk_ens_load", "pk_step_species",
+    "pk_ens_save", "pk_ens_load", "pk_step_species", "pk_ens_interleave",
 )
 
 _LIB = None
@@ -140,6 +140,7 @@ def lib():
         L.pk_dn_apply.argtypes = [vp, vp]
         L.pk_rho_save.argtypes = [vp, vp, ctypes.c_int]
         L.pk_step_species.argtypes = [vp, ctypes.c_int, vp, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+        L.pk_ens_interleave.argtypes = [vp]
         for name in ("pk_grid_save", "pk_grid_load", "pk_ens_save", "pk_ens_load"):
             getattr(L, name).argtypes = [vp, ctypes.c_char_p]
         L.pk_project_continuity.argtypes = [vp, vp, ctypes.c_int, ctypes.c_double]
@@ -375,6 +376,10 @@ class EnsembleSupercells:
 
     def resort(self):
         self.ctx.check(lib().pk_ens_resort(self.h))
+
+    def interleave(self):
+        """reorder each supercell (FP32) so that consecutive particles sit in different cells"""
+        self.ctx.check(lib().pk_ens_interleave(self.h))
 
     def save(self, path):
         """raw snapshot of the particles in supercell order (pica_amd.snapshot reads it)"""

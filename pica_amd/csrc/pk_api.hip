@@ -146,6 +146,11 @@ int pk_ctx_set_option(pk_ctx* c, int option, int value)
         c->opt[option] = value;
         return PK_OK;
     }
+    if (option == PK_OPT_INTERLEAVE) {
+        if (value < 0) return pk_fail(c, PK_E_ARG, "pk_ctx_set_option: negative interleave period");
+        c->opt[option] = value;
+        return PK_OK;
+    }
     return pk_fail(c, PK_E_ARG, "pk_ctx_set_option: unknown option");
 }
 
@@ -397,6 +402,9 @@ int pk_ens_resort(pk_ens* e)
         return pk_fail(e->ctx, PK_E_STATE,
                        "particle step: a particle was not finite or moved further than its tile allows "
                        "(it was left unchanged)");
+    e->resorts++;
+    const int period = e->ctx->opt[PK_OPT_INTERLEAVE];
+    if (e->posmode == PK_POS_SPLIT && period > 0 && e->resorts % period == 0) return pk_ens_interleave(e);
     return PK_OK;
 }
 

@@ -595,6 +595,164 @@ int pk_ens_alloc(pk_ens* e, size_t capacity)
     return PK_OK;
 }
 
+// ---- interleave (FP32 split ensembles) ----
+constexpr int IL_P = 8;          // particles per thread: supercells of up to 2048 particles
+constexpr int IL_RMAX = 64;      // most particles of one cell that are interleaved
+
+// One workgroup reorders supercell s in place: by (rank of the particle among those of its cell,
+// ranks in storage order; cell), so consecutive particles sit in different cells.  Supercells of
+// more than 64 cells or 256 IL_P particles, or with a cell of more than IL_RMAX particles, are
+// left as they are.
+//
+// Ranks come 256 particles at a time: each particle sets its lane bit in cm[cell][wavefront]
+// (one LDS atomic OR), and its rank within the chunk is the popcount of the lanes below it in
+// the same cell.  The per-rank cell masks M and their prefix counts come from wavefront 0 with
+// ballots and shuffles.  The eight 32-bit fields then pass one at a time through an LDS stage
+// (scattered LDS writes, coalesced global loads and stores), two fields ahead in flight.
+__global__ void __launch_bounds__(256) k_interleave_split(EnsGeom eg, PartSplit pm, const int64_t* __restrict__ off,
+                                                          const int32_t* __restrict__ cnt)
+{
+    const long s = pk_xcd_supercell(blockIdx.x, eg.xcd_chunk);
+    if (s >= eg.nsct) return;
+    const int n = cnt[s];
+    const int ncl = eg.ncps[0] * eg.ncps[1] * eg.ncps[2];
+    if (n < 2 || n > 256 * IL_P || ncl > 64) return;
+    __shared__ unsigned long long cm[64][4];
+    __shared__ int hist[64];
+    __shared__ unsigned long long M[IL_RMAX];
+    __shared__ int start[IL_RMAX];
+    __shared__ int maxc, bad;
+    __shared__ uint32_t stage[2][256 * IL_P];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint64_t below = (1ull << lane) - 1ull;
+    const long base = off[s];
+    const int m0[3] = {(int)(s / ((long)eg.nsc[1] * eg.nsc[2])) * eg.ncps[0],
+                       (int)((s / eg.nsc[2]) % eg.nsc[1]) * eg.ncps[1], (int)(s % eg.nsc[2]) * eg.ncps[2]};
+    uint32_t* const fld[8] = {(uint32_t*)pm.cell, (uint32_t*)pm.ox, (uint32_t*)pm.oy, (uint32_t*)pm.oz,
+                              (uint32_t*)pm.ux,   (uint32_t*)pm.uy, (uint32_t*)pm.uz, (uint32_t*)pm.w};
+    uint32_t v[8][IL_P];
+    int key[IL_P], pos[IL_P];
+#pragma unroll
+    for (int f = 0; f < 3; f++)
+#pragma unroll
+        for (int k = 0; k < IL_P; k++)
+            if (tid + 256 * k < n) v[f][k] = fld[f][base + tid + 256 * k];
+#pragma unroll
+    for (int k = 0; k < IL_P; k++) {
+        key[k] = -1;
+        pos[k] = 0;
+        if (tid + 256 * k < n) {
+            int cc[3];
+            cell_coords((int32_t)v[0][k], eg, cc);
+            int kk = 0;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                int l = cc[d] - m0[d];
+                l = l < 0 ? 0 : (l >= eg.ncps[d] ? eg.ncps[d] - 1 : l);   // a stray particle still gets a slot
+                kk = kk * eg.ncps[d] + l;
+            }
+            key[k] = kk;
+        }
+    }
+    cm[tid >> 2][tid & 3] = 0ull;
+    if (tid < 64) hist[tid] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < IL_P; k++) {
+        if (256 * k >= n) break;
+        const int kk = key[k];
+        if (kk >= 0) atomicOr(&cm[kk][wave], 1ull << lane);
+        __syncthreads();
+        if (kk >= 0) {
+            int r = hist[kk] + __popcll(cm[kk][wave] & below);
+            for (int w = 0; w < wave; w++) r += __popcll(cm[kk][w]);
+            pos[k] = r;                                   // the rank for now
+        }
+        __syncthreads();
+        // the thread that counts cell tid also clears its masks: no other thread touches them here
+        if (tid < 64) {
+            hist[tid] += __popcll(cm[tid][0]) + __popcll(cm[tid][1]) + __popcll(cm[tid][2]) + __popcll(cm[tid][3]);
+            cm[tid][0] = 0ull; cm[tid][1] = 0ull; cm[tid][2] = 0ull; cm[tid][3] = 0ull;
+        }
+        __syncthreads();
+    }
+    // M[r] bit c: cell c holds more than r particles; start[r]: particles of rank below r
+    if (wave == 0) {
+        const int h = lane < ncl ? hist[lane] : 0;
+        int mx = h;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const int t = __shfl_xor(mx, o);
+            mx = t > mx ? t : mx;
+        }
+        uint64_t mine = 0ull;
+        const int rmax = mx < IL_RMAX ? mx : IL_RMAX;
+        for (int r = 0; r < rmax; r++) {
+            const uint64_t b = __ballot(h > r);
+            if (lane == r) mine = b;
+        }
+        const int c = lane < rmax ? __popcll(mine) : 0;
+        int incl = c;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(incl, o);
+            if (lane >= o) incl += t;
+        }
+        if (lane < rmax) {
+            M[lane] = mine;
+            start[lane] = incl - c;
+        }
+        if (lane == 63) {
+            maxc = mx;
+            bad = mx > IL_RMAX || incl != n;
+        }
+    }
+    __syncthreads();
+    const int mx = maxc;
+    // every rank must name a set bit of M: otherwise the supercell is left as it is
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < IL_P; k++)
+        if (key[k] >= 0) {
+            const int r = pos[k];
+            if (r >= 0 && r < mx && r < IL_RMAX && ((M[r] >> key[k]) & 1ull))
+                pos[k] = start[r] + __popcll(M[r] & ((1ull << key[k]) - 1ull));
+            else
+                ok = false;
+        }
+    if (!ok) bad = 1;
+    __syncthreads();
+    if (bad) return;
+#pragma unroll
+    for (int f = 0; f < 8; f++) {
+        uint32_t* const st = stage[f & 1];
+#pragma unroll
+        for (int k = 0; k < IL_P; k++)
+            if (key[k] >= 0) st[pos[k]] = v[f][k];
+        __syncthreads();
+        if (f + 3 < 8)
+#pragma unroll
+            for (int k = 0; k < IL_P; k++)
+                if (tid + 256 * k < n) v[f + 3][k] = fld[f + 3][base + tid + 256 * k];
+#pragma unroll
+        for (int k = 0; k < IL_P; k++)
+            if (tid + 256 * k < n) fld[f][base + tid + 256 * k] = st[tid + 256 * k];
+    }
+}
+
+extern "C" int pk_ens_interleave(pk_ens* e)
+{
+    if (!e) return PK_E_ARG;
+    if (int r = pk_ens_resort(e)) return r;
+    if (e->posmode != PK_POS_SPLIT || e->np == 0) return PK_OK;
+    EnsGeom eg = ens_geom(e);
+    pk_ktimer tm(e->ctx, "interleave");
+    hipLaunchKernelGGL(k_interleave_split, dim3((unsigned)(PK_XCD * (long)eg.xcd_chunk)), dim3(256), 0, e->ctx->stream,
+                       eg, part_split(e, e->main), e->off, e->cnt);
+    PK_HIP(e->ctx, hipGetLastError());
+    return PK_OK;
+}
+
 int pk_ens_insert_launch(pk_ens* e)
 {
     EnsGeom eg = ens_geom(e);

@@ -39,7 +39,7 @@ struct pk_ctx {
     std::vector<hipEvent_t> evpool;
     std::map<std::string, pk_timer_rec> timers;
     int32_t* flags_host = nullptr;     // pinned, PK_NFLAGS
-    int opt[PK_OPT_N] = {PK_KERNEL_AUTO, 0};
+    int opt[PK_OPT_N] = {PK_KERNEL_AUTO, 0, 4};
 };
 
 enum { PK_FLAG_ERR = 0, PK_FLAG_OVERFLOW = 1, PK_NFLAGS = 4 };
@@ -88,6 +88,7 @@ struct pk_ens {
     int maxcap = 0;
     int64_t np = 0;
     bool pending_movers = false;  // particle step done, re-sort not yet done
+    long resorts = 0;             // re-sorts done (the interleave period counts them)
     int nfield() const { return prec == PK_F64 ? 7 : 8; }
     size_t esize() const { return prec == PK_F64 ? 8 : 4; }
     void* field(void* buf, int f) const { return (char*)buf + (size_t)f * capacity * esize(); }

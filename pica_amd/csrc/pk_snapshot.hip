@@ -34,7 +34,9 @@ struct PkSnapHeader {
     int64_t count;          // grid: elements per component; ensemble: particles
     int64_t nsct;           // ensemble: supercells
     uint32_t esize;         // bytes per element
-    uint32_t reserved[9];
+    uint32_t pad;
+    int64_t resorts;        // ensemble: re-sorts done (the interleave period counts them)
+    uint32_t reserved[6];
 };
 static_assert(sizeof(PkSnapHeader) == 160, "snapshot header is 160 bytes");
 
@@ -152,6 +154,7 @@ int pk_ens_save(pk_ens* e, const char* path)
     h.count = np;
     h.nsct = nsct;
     h.esize = (uint32_t)e->esize();
+    h.resorts = e->resorts;
     File out(path, "wb");
     if (!out.f) return pk_fail(c, PK_E_ARG, "pk_ens_save: cannot create the file");
     if (!out.write(&h, sizeof(h)) || !out.write(cnt.data(), sizeof(int32_t) * nsct))
@@ -220,6 +223,7 @@ int pk_ens_load(pk_ens* e, const char* path)
     e->maxcap = maxcap;
     e->np = np;
     e->pending_movers = false;
+    e->resorts = h.resorts;
     return PK_OK;
 }
 

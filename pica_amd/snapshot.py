@@ -4,7 +4,8 @@ analysis, fixtures and training data away from the device.
 File layout (little-endian): a 160-byte header, then raw arrays.
   header: magic "PKSNAP1\\0", kind (1 grid, 2 ensemble), version (1), prec (PK_F64 = 0,
           PK_F32 = 1), posmode, dims[3], ghost (grid) or species index (ensemble), ncps[3],
-          nfield, origin[3], step[3], count, nsct, esize, 9 reserved words;
+          nfield, origin[3], step[3], count, nsct, esize, a pad word, resorts (int64 at byte 128),
+          6 reserved words;
   grid:     nfield components over the whole storage dims[0] x dims[1] x dims[2], z fastest;
   ensemble: nsct int32 supercell counts, then the fields in supercell order
             (F64: x y z ux uy uz w; F32 split: cell int32, ox oy oz ux uy uz w float32).
@@ -15,7 +16,7 @@ HEADER = np.dtype([
     ("magic", "S8"), ("kind", "<u4"), ("version", "<u4"), ("prec", "<u4"), ("posmode", "<u4"),
     ("dims", "<i4", (3,)), ("ghost", "<i4"), ("ncps", "<i4", (3,)), ("nfield", "<i4"),
     ("origin", "<f8", (3,)), ("step", "<f8", (3,)), ("count", "<i8"), ("nsct", "<i8"), ("esize", "<u4"),
-    ("reserved", "<u4", (9,)),
+    ("pad", "<u4"), ("resorts", "<i8"), ("reserved", "<u4", (6,)),
 ])
 assert HEADER.itemsize == 160
 

@@ -132,3 +132,50 @@ def test_split_cells_and_counts(ctx):
     lin = (ci[:, 0] * ncell[1] + ci[:, 1]) * ncell[2] + ci[:, 2]
     assert np.array_equal(cells, lin)
     assert np.array_equal(counts.ravel(), np.bincount(lin, minlength=int(np.prod(ncell))))
+
+
+def _interleave_order(cells, off, ncell, ncps):
+    """the order pk_ens_interleave gives each supercell: by (rank of the particle among those of
+    its cell in the current order, local cell index)"""
+    perm = np.arange(cells.shape[0])
+    for s in range(len(off) - 1):
+        a, b = off[s], off[s + 1]
+        c = cells[a:b]
+        cz, cy, cx = c % ncell[2], (c // ncell[2]) % ncell[1], c // (ncell[1] * ncell[2])
+        loc = ((cx % ncps[0]) * ncps[1] + cy % ncps[1]) * ncps[2] + cz % ncps[2]
+        rank = np.zeros(b - a, np.int64)
+        seen = {}
+        for i, k in enumerate(loc):
+            rank[i] = seen.get(k, 0)
+            seen[k] = rank[i] + 1
+        perm[a:b] = a + np.lexsort((loc, rank))
+    return perm
+
+
+def test_interleave_order_and_set(ctx):
+    """pk_ens_interleave keeps every particle (bits included) and every supercell's membership,
+    and orders each supercell by (rank within its cell, cell)"""
+    _say("interleave")
+    ncell, ncps, ppc = (8, 12, 8), (4, 4, 4), 6
+    dt, x, u, w = _inputs(88, ncell, ppc)
+    ens = P.EnsembleSupercells(ctx, ncell, (1.0, 1.0, 1.0), ncps, prec=P.F32)
+    g = P.YeeGrid.periodic(ctx, ncell, prec=P.F32)
+    ens.add(x, u, w)
+    ctx.set_option(P.OPT_INTERLEAVE, 0)
+    try:
+        P.run(ens, g, dt, 6, P.TSC, P.ESIRKEPOV)          # let the order mix
+    finally:
+        ctx.set_option(P.OPT_INTERLEAVE, 4)
+    x0, u0, w0, off0 = ens.particles()
+    _, cells0 = ens.cells()
+    ens.interleave()
+    x1, u1, w1, off1 = ens.particles()
+    assert np.array_equal(off0, off1)
+    perm = _interleave_order(cells0, off0, ncell, ncps)
+    assert not np.array_equal(perm, np.arange(perm.shape[0]))
+    assert np.array_equal(x1, x0[perm]) and np.array_equal(u1, u0[perm]) and np.array_equal(w1, w0[perm])
+    _, cells1 = ens.cells()
+    # the first particles of each supercell now sit in distinct cells
+    for s in range(len(off1) - 1):
+        head = cells1[off1[s]:off1[s + 1]][:32]
+        assert len(np.unique(head)) == len(head)
