@@ -93,7 +93,7 @@ def run_gpu(args, prec):
     t1 = time.perf_counter()
     ctx.timer_enable(False)
     kernels = {}
-    for k in ("particles", "insert", "yee_e", "yee_b", "ghost", "zero", "rebuild"):
+    for k in ("particles", "insert", "interleave", "yee_e", "yee_b", "ghost", "zero", "rebuild"):
         ms, cnt = ctx.timer(k)
         if cnt:
             kernels[k] = {"ms_total": ms, "launches": cnt, "ms_mean": ms / cnt}
