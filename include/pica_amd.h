@@ -79,9 +79,10 @@ int pk_timer_reset(pk_ctx* ctx);
  * PK_OPT_CHANGE_QUEUE caps the cell-sorted kernel's LDS queue of cell-changing particles
  * (0 = as many entries as the LDS budget allows; beyond the queue a lane deposits at once).
  * PK_OPT_INTERLEAVE (default 4): every that many re-sorts of an FP32 ensemble, pk_ens_resort
- * reorders each supercell's particles by (rank among the particles of their cell, cell), so that
- * consecutive particles -- the lanes of a wavefront -- sit in different cells and their LDS
- * deposits spread over the banks.  0 turns it off.  Supercells of more than 64 cells or 2048
+ * reorders each supercell's particles: pairs of particles of one cell in adjacent slots, ordered
+ * by (pair rank within the cell, cell), then the odd particle of every odd-count cell.  A lane of
+ * the particle kernel takes two adjacent slots and deposits a same-cell pair with one set of LDS
+ * atomics, and the lanes of a wavefront sit in different cells.  0 turns it off.  Supercells of more than 64 cells or 2048
  * particles are left as they are.  Batched stepping (pk_batch_run) does not reorder. */
 enum { PK_OPT_PARTICLE_KERNEL = 0, PK_OPT_CHANGE_QUEUE = 1, PK_OPT_INTERLEAVE = 2, PK_OPT_N };
 enum { PK_KERNEL_AUTO = 0, PK_KERNEL_PER_PARTICLE = 1, PK_KERNEL_CELL_SORTED = 2 };

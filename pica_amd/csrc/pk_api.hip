@@ -404,7 +404,7 @@ int pk_ens_resort(pk_ens* e)
                        "(it was left unchanged)");
     e->resorts++;
     const int period = e->ctx->opt[PK_OPT_INTERLEAVE];
-    if (e->posmode == PK_POS_SPLIT && period > 0 && e->resorts % period == 0) return pk_ens_interleave(e);
+    if (e->posmode == PK_POS_SPLIT && period > 0 && e->resorts % period == 0) return pk_ens_interleave_launch(e);
     return PK_OK;
 }
 

@@ -82,7 +82,7 @@ int enqueue_step(pk_batch* b)
     if ((r = pk_batch_insert_launch(b))) return r;
     hipLaunchKernelGGL(k_batch_step_done, dim3(1), dim3(64), 0, b->ctx->stream, b->halt);
     PK_HIP(b->ctx, hipGetLastError());
-    return PK_OK;
+    return pk_batch_interleave_launch(b);
 }
 
 void drop_graph(pk_batch* b)
@@ -103,6 +103,7 @@ int prepare(pk_batch* b, double dt, int shape, int dep)
     r = b->prec == PK_F64 ? pk_batch_f64_prepare(b, dt, shape, dep) : pk_batch_split_prepare(b, dt, shape, dep);
     if (r) return r;
     if ((r = pk_batch_insert_prepare(b))) return r;
+    if ((r = pk_batch_interleave_prepare(b))) return r;
     b->dt = dt;
     b->shape = shape;
     b->dep = dep;
@@ -144,12 +145,24 @@ int run(pk_batch* b, double dt, int shape, int dep, int nsteps)
     int r;
     for (pk_ens* e : b->ens)
         if ((r = pk_ens_resort(e))) return r;
-    if (b->dirty || dt != b->dt || shape != b->shape || dep != b->dep)
+    const int period = b->ens[0]->posmode == PK_POS_SPLIT ? c->opt[PK_OPT_INTERLEAVE] : 0;
+    if (b->dirty || dt != b->dt || shape != b->shape || dep != b->dep || period != b->ilperiod)
         if ((r = prepare(b, dt, shape, dep))) return r;
+    // re-sort counts before this run; each pass tells the interleave kernel where it starts
+    std::vector<long> r0(b->n());
+    for (int j = 0; j < b->n(); j++) r0[j] = b->ens[j]->resorts;
+    auto set_base = [&](int done) -> int {
+        if (b->ilperiod <= 0) return PK_OK;
+        for (int j = 0; j < b->n(); j++) b->ilbase_host[j] = r0[j] + done;
+        PK_HIP(c, hipMemcpyAsync(b->ilbase, b->ilbase_host.data(), sizeof(int64_t) * b->n(), hipMemcpyHostToDevice,
+                                 c->stream));
+        return PK_OK;
+    };
     hipLaunchKernelGGL(k_batch_clear, dim3(1), dim3(64), 0, c->stream, b->halt);
     PK_HIP(c, hipGetLastError());
     bool state_err = false;
     int done = 0;
+    if ((r = set_base(0))) return r;
     while (done < nsteps) {
         {
             pk_ktimer tm(c, "batch_steps");
@@ -174,11 +187,16 @@ int run(pk_batch* b, double dt, int shape, int dep, int nsteps)
             if (fl[PK_FLAG_OVERFLOW] && (r = pk_ens_rebuild(e, 0))) return r;
             if ((r = pk_flags_clear(e))) return r;
         }
+        // the interleave the overflowed step skipped, where its count is due
+        for (int j = 0; j < b->n() && b->ilperiod > 0; j++)
+            if ((r0[j] + done) % b->ilperiod == 0 && (r = pk_ens_interleave_launch(b->ens[j]))) return r;
         b->dirty = true;
         if (done < nsteps && (r = prepare(b, dt, shape, dep))) return r;
         hipLaunchKernelGGL(k_batch_clear, dim3(1), dim3(64), 0, c->stream, b->halt);
         PK_HIP(c, hipGetLastError());
+        if ((r = set_base(done))) return r;
     }
+    for (int j = 0; j < b->n(); j++) b->ens[j]->resorts = r0[j] + nsteps;
     for (pk_ens* e : b->ens) {
         int32_t fl[PK_NFLAGS];
         if ((r = pk_flags_read(e, fl))) return r;
@@ -201,6 +219,7 @@ struct Snapshot {
         size_t capacity = 0;
         int maxcap = 0;
         int64_t np = 0;
+        long resorts = 0;
     };
     std::vector<Ens> ens;
     std::vector<void*> grids;
@@ -241,6 +260,7 @@ int snapshot_take(pk_batch* b, Snapshot& S)
         s.capacity = e->capacity;
         s.maxcap = e->maxcap;
         s.np = e->np;
+        s.resorts = e->resorts;
     }
     PK_HIP(c, hipStreamSynchronize(c->stream));
     return PK_OK;
@@ -263,6 +283,7 @@ int snapshot_restore(pk_batch* b, const Snapshot& S)
                                  c->stream));
         e->maxcap = s.maxcap;
         e->np = s.np;
+        e->resorts = s.resorts;
         e->pending_movers = false;
         if ((r = pk_flags_clear(e))) return r;
     }
@@ -325,7 +346,8 @@ void pk_batch_destroy(pk_batch* b)
     if (!b) return;
     (void)hipStreamSynchronize(b->ctx->stream);
     drop_graph(b);
-    for (void* p : {(void*)b->items, b->pjobs, b->ijobs, b->fjobs, (void*)b->fstart, (void*)b->halt})
+    for (void* p : {(void*)b->items, b->pjobs, b->ijobs, b->fjobs, (void*)b->fstart, (void*)b->halt, b->iljobs,
+                    (void*)b->ilbase})
         if (p) (void)hipFree(p);
     delete b;
 }

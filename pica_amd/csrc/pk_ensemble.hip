@@ -370,16 +370,29 @@ struct LoadParams {
     const double* modes;     // nmodes x (kx, ky, kz, ax, ay, az, phase), device
 };
 
-__device__ __forceinline__ void thermal_particle(const EnsGeom& eg, const LoadParams& lp, long s, int l, int gc[3],
-                                                 double U[3], double u[3])
+// Particle k of every cell has an id from its cell and k; the slot l it takes in its supercell
+// interleaves the cells, so that the lanes of a wavefront sit in different cells and their LDS
+// deposits do not collide on one address.  Unpaired: slot = k * cells + cell.  Paired (the FP32
+// split form, whose lanes take two adjacent slots): particles 2q and 2q + 1 of a cell in
+// adjacent slots, pairs by (q, cell), then particle ppc - 1 of every cell when ppc is odd --
+// the order k_interleave_split gives.
+__device__ __forceinline__ void thermal_particle(const EnsGeom& eg, const LoadParams& lp, long s, int l, bool paired,
+                                                 int gc[3], double U[3], double u[3])
 {
     const int sidx[3] = {(int)(s / ((long)eg.nsc[1] * eg.nsc[2])), (int)((s / eg.nsc[2]) % eg.nsc[1]),
                          (int)(s % eg.nsc[2])};
-    // slots interleave the cells of the supercell (slot = k * cells + cell): consecutive
-    // particles, hence the lanes of a wavefront, sit in different cells and their LDS
-    // deposits do not collide on one address
     const int ncsc = eg.ncps[0] * eg.ncps[1] * eg.ncps[2];
-    const int lc = l % ncsc, k = l / ncsc;
+    int lc = l % ncsc, k = l / ncsc;
+    if (paired) {
+        const int npair = (lp.ppc & ~1) * ncsc;
+        if (l < npair) {
+            lc = (l >> 1) % ncsc;
+            k = 2 * ((l >> 1) / ncsc) + (l & 1);
+        } else {
+            lc = l - npair;
+            k = lp.ppc - 1;
+        }
+    }
     const int a = lc / (eg.ncps[1] * eg.ncps[2]), b = (lc / eg.ncps[2]) % eg.ncps[1], c = lc % eg.ncps[2];
     gc[0] = sidx[0] * eg.ncps[0] + a;
     gc[1] = sidx[1] * eg.ncps[1] + b;
@@ -408,7 +421,7 @@ __global__ void __launch_bounds__(256) k_load_f64(EnsGeom eg, LoadParams lp, con
     for (int l = threadIdx.x; l < per_sc; l += blockDim.x) {
         int gc[3];
         double U[3], u[3];
-        thermal_particle(eg, lp, s, l, gc, U, u);
+        thermal_particle(eg, lp, s, l, false, gc, U, u);
         const long p = off[s] + l;
         pm.x[p] = ((double)gc[0] + U[0]) * lp.step[0];
         pm.y[p] = ((double)gc[1] + U[1]) * lp.step[1];
@@ -425,7 +438,7 @@ __global__ void __launch_bounds__(256) k_load_split(EnsGeom eg, LoadParams lp, c
     for (int l = threadIdx.x; l < per_sc; l += blockDim.x) {
         int gc[3];
         double U[3], u[3];
-        thermal_particle(eg, lp, s, l, gc, U, u);
+        thermal_particle(eg, lp, s, l, true, gc, U, u);
         const long p = off[s] + l;
         float o[3];
 #pragma unroll
@@ -599,29 +612,30 @@ int pk_ens_alloc(pk_ens* e, size_t capacity)
 constexpr int IL_P = 8;          // particles per thread: supercells of up to 2048 particles
 constexpr int IL_RMAX = 64;      // most particles of one cell that are interleaved
 
-// One workgroup reorders supercell s in place: by (rank of the particle among those of its cell,
-// ranks in storage order; cell), so consecutive particles sit in different cells.  Supercells of
-// more than 64 cells or 256 IL_P particles, or with a cell of more than IL_RMAX particles, are
-// left as they are.
+// One workgroup reorders supercell s in place.  With r the rank of a particle among those of its
+// cell (in storage order) and h the cell's count, the order is: first the pairs (ranks 2q and
+// 2q + 1 of one cell) by (q, cell), the two of a pair adjacent; then the last particle of every
+// cell with an odd count, by cell.  So a particle and its neighbour slot mostly share a cell, and
+// consecutive pairs sit in different cells.  Supercells of more than 64 cells or 256 IL_P
+// particles, or with a cell of more than IL_RMAX particles, are left as they are.
 //
 // Ranks come 256 particles at a time: each particle sets its lane bit in cm[cell][wavefront]
 // (one LDS atomic OR), and its rank within the chunk is the popcount of the lanes below it in
-// the same cell.  The per-rank cell masks M and their prefix counts come from wavefront 0 with
-// ballots and shuffles.  The eight 32-bit fields then pass one at a time through an LDS stage
-// (scattered LDS writes, coalesced global loads and stores), two fields ahead in flight.
-__global__ void __launch_bounds__(256) k_interleave_split(EnsGeom eg, PartSplit pm, const int64_t* __restrict__ off,
-                                                          const int32_t* __restrict__ cnt)
+// the same cell.  The per-pair-row cell masks M and their prefix counts come from wavefront 0
+// with ballots and shuffles.  The eight 32-bit fields then pass one at a time through an LDS
+// stage (scattered LDS writes, coalesced global loads and stores), two fields ahead in flight.
+__device__ __forceinline__ void interleave_sc(const EnsGeom& eg, const PartSplit& pm, const int64_t* __restrict__ off,
+                                              const int32_t* __restrict__ cnt, long s)
 {
-    const long s = pk_xcd_supercell(blockIdx.x, eg.xcd_chunk);
-    if (s >= eg.nsct) return;
     const int n = cnt[s];
     const int ncl = eg.ncps[0] * eg.ncps[1] * eg.ncps[2];
     if (n < 2 || n > 256 * IL_P || ncl > 64) return;
     __shared__ unsigned long long cm[64][4];
     __shared__ int hist[64];
-    __shared__ unsigned long long M[IL_RMAX];
-    __shared__ int start[IL_RMAX];
-    __shared__ int maxc, bad;
+    __shared__ unsigned long long M[IL_RMAX / 2];
+    __shared__ int start[IL_RMAX / 2];
+    __shared__ unsigned long long odd;
+    __shared__ int maxc, bad, npair;
     __shared__ uint32_t stage[2][256 * IL_P];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint64_t below = (1ull << lane) - 1ull;
@@ -676,7 +690,8 @@ __global__ void __launch_bounds__(256) k_interleave_split(EnsGeom eg, PartSplit 
         }
         __syncthreads();
     }
-    // M[r] bit c: cell c holds more than r particles; start[r]: particles of rank below r
+    // pair row q: M[q] bit c = cell c holds more than 2q + 1 particles, start[q] = slots of the
+    // rows below; odd bit c = cell c has an odd count; npair = slots of all pairs
     if (wave == 0) {
         const int h = lane < ncl ? hist[lane] : 0;
         int mx = h;
@@ -686,37 +701,45 @@ __global__ void __launch_bounds__(256) k_interleave_split(EnsGeom eg, PartSplit 
             mx = t > mx ? t : mx;
         }
         uint64_t mine = 0ull;
-        const int rmax = mx < IL_RMAX ? mx : IL_RMAX;
-        for (int r = 0; r < rmax; r++) {
-            const uint64_t b = __ballot(h > r);
-            if (lane == r) mine = b;
+        const int qmax = mx > IL_RMAX ? IL_RMAX / 2 : mx / 2;
+        for (int q = 0; q < qmax; q++) {
+            const uint64_t b = __ballot(h > 2 * q + 1);
+            if (lane == q) mine = b;
         }
-        const int c = lane < rmax ? __popcll(mine) : 0;
+        const uint64_t ob = __ballot(h & 1);
+        const int c = lane < qmax ? 2 * __popcll(mine) : 0;
         int incl = c;
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
             const int t = __shfl_up(incl, o);
             if (lane >= o) incl += t;
         }
-        if (lane < rmax) {
+        if (lane < qmax) {
             M[lane] = mine;
             start[lane] = incl - c;
         }
         if (lane == 63) {
             maxc = mx;
-            bad = mx > IL_RMAX || incl != n;
+            odd = ob;
+            npair = incl;
+            bad = mx > IL_RMAX || incl + __popcll(ob) != n;
         }
     }
     __syncthreads();
     const int mx = maxc;
-    // every rank must name a set bit of M: otherwise the supercell is left as it is
+    // every particle must name a set bit of M or of odd: otherwise the supercell is left as it is
     bool ok = true;
 #pragma unroll
     for (int k = 0; k < IL_P; k++)
         if (key[k] >= 0) {
-            const int r = pos[k];
-            if (r >= 0 && r < mx && r < IL_RMAX && ((M[r] >> key[k]) & 1ull))
-                pos[k] = start[r] + __popcll(M[r] & ((1ull << key[k]) - 1ull));
+            const int r = pos[k], h = hist[key[k]];
+            const uint64_t below_c = (1ull << key[k]) - 1ull;
+            if (r < 0 || r >= h || h > mx || mx > IL_RMAX)
+                ok = false;
+            else if ((h & 1) && r == h - 1)
+                pos[k] = npair + __popcll(odd & below_c);
+            else if ((M[r >> 1] >> key[k]) & 1ull)
+                pos[k] = start[r >> 1] + 2 * __popcll(M[r >> 1] & below_c) + (r & 1);
             else
                 ok = false;
         }
@@ -740,16 +763,78 @@ __global__ void __launch_bounds__(256) k_interleave_split(EnsGeom eg, PartSplit 
     }
 }
 
-extern "C" int pk_ens_interleave(pk_ens* e)
+__global__ void __launch_bounds__(256) k_interleave_split(EnsGeom eg, PartSplit pm, const int64_t* __restrict__ off,
+                                                          const int32_t* __restrict__ cnt)
 {
-    if (!e) return PK_E_ARG;
-    if (int r = pk_ens_resort(e)) return r;
+    const long s = pk_xcd_supercell(blockIdx.x, eg.xcd_chunk);
+    if (s >= eg.nsct) return;
+    interleave_sc(eg, pm, off, cnt, s);
+}
+
+struct InterleaveArgs {
+    EnsGeom eg;
+    PartSplit pm;
+    const int64_t* off;
+    const int32_t* cnt;
+};
+
+// batched form, last kernel of a batch step: ensemble j is reordered when its re-sort count
+// base[j] + (steps done in this pass) is a multiple of period; nothing after a halt
+__global__ void __launch_bounds__(256) k_interleave_split_batch(const InterleaveArgs* __restrict__ jobs,
+                                                                const int2* __restrict__ items, long nitems,
+                                                                const int* __restrict__ halt,
+                                                                const int64_t* __restrict__ base, int period)
+{
+    if (halt[PK_HALT_STOP]) return;
+    for (long w = blockIdx.x; w < nitems; w += gridDim.x) {
+        __syncthreads();
+        const int2 it = items[w];
+        if ((base[it.x] + halt[PK_HALT_DONE]) % period != 0) continue;
+        const InterleaveArgs& a = jobs[it.x];
+        interleave_sc(a.eg, a.pm, a.off, a.cnt, it.y);
+    }
+}
+
+int pk_ens_interleave_launch(pk_ens* e)
+{
     if (e->posmode != PK_POS_SPLIT || e->np == 0) return PK_OK;
     EnsGeom eg = ens_geom(e);
     pk_ktimer tm(e->ctx, "interleave");
     hipLaunchKernelGGL(k_interleave_split, dim3((unsigned)(PK_XCD * (long)eg.xcd_chunk)), dim3(256), 0, e->ctx->stream,
                        eg, part_split(e, e->main), e->off, e->cnt);
     PK_HIP(e->ctx, hipGetLastError());
+    return PK_OK;
+}
+
+extern "C" int pk_ens_interleave(pk_ens* e)
+{
+    if (!e) return PK_E_ARG;
+    if (int r = pk_ens_resort(e)) return r;
+    return pk_ens_interleave_launch(e);
+}
+
+int pk_batch_interleave_prepare(pk_batch* b)
+{
+    const int n = b->n();
+    b->ilperiod = b->ens[0]->posmode == PK_POS_SPLIT ? b->ctx->opt[PK_OPT_INTERLEAVE] : 0;
+    if (b->ilperiod <= 0) return PK_OK;
+    std::vector<InterleaveArgs> jobs(n);
+    for (int i = 0; i < n; i++) {
+        pk_ens* e = b->ens[i];
+        jobs[i] = InterleaveArgs{ens_geom(e), part_split(e, e->main), e->off, e->cnt};
+    }
+    b->ilbase_host.assign(n, 0);
+    int r = pk_batch_upload(b, &b->iljobs, jobs.data(), sizeof(jobs[0]) * n);
+    if (r) return r;
+    return pk_batch_upload(b, (void**)&b->ilbase, b->ilbase_host.data(), sizeof(int64_t) * n);
+}
+
+int pk_batch_interleave_launch(pk_batch* b)
+{
+    if (b->ilperiod <= 0 || b->nitems == 0) return PK_OK;
+    hipLaunchKernelGGL(k_interleave_split_batch, dim3((unsigned)b->nitems), dim3(256), 0, b->ctx->stream,
+                       (const InterleaveArgs*)b->iljobs, b->items, b->nitems, b->halt, b->ilbase, b->ilperiod);
+    PK_HIP(b->ctx, hipGetLastError());
     return PK_OK;
 }
 

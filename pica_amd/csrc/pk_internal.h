@@ -214,6 +214,10 @@ struct pk_batch {
     long fblocks[PK_BSTAGES] = {0, 0, 0, 0, 0};
     size_t plds = 0;           // dynamic LDS of the particle kernel (largest ensemble)
     int* halt = nullptr;       // device halt words (PK_HALT_*)
+    void* iljobs = nullptr;    // interleave-kernel arguments (FP32 split ensembles)
+    int64_t* ilbase = nullptr; // per ensemble: re-sort count at the start of the current pass
+    std::vector<int64_t> ilbase_host;
+    int ilperiod = 0;          // interleave period captured by the prepared step (0: none)
     hipGraphExec_t exec = nullptr;
     int n() const { return (int)ens.size(); }
 };
@@ -227,6 +231,9 @@ int pk_batch_f64_prepare(pk_batch* b, double dt, int shape, int dep);
 int pk_batch_f64_launch(pk_batch* b, int shape, int dep);
 int pk_batch_insert_prepare(pk_batch* b);
 int pk_batch_insert_launch(pk_batch* b);
+int pk_batch_interleave_prepare(pk_batch* b);
+int pk_batch_interleave_launch(pk_batch* b);
+int pk_ens_interleave_launch(pk_ens* e);   // reorder now, no re-sort first
 int pk_check_pair(pk_ens* e, pk_grid* g);   // the periodic step's grid/ensemble requirements
 int pk_flags_read(pk_ens* e, int32_t out[PK_NFLAGS]);
 int pk_flags_clear(pk_ens* e);

@@ -40,6 +40,22 @@ __device__ __forceinline__ bool colloc(int comp, int d)
     return (m[comp] >> d) & 1u;
 }
 
+// position of the k-th (from 0) set bit of m, k < popcount(m): popcount halving, 6 steps
+__device__ __forceinline__ int select_bit(uint64_t m, int k)
+{
+    int pos = 0;
+#pragma unroll
+    for (int w = 32; w >= 1; w >>= 1) {
+        const int c = __popcll(m & ((1ull << w) - 1ull));
+        if (k >= c) {
+            k -= c;
+            m >>= w;
+            pos += w;
+        }
+    }
+    return pos;
+}
+
 __device__ __forceinline__ void tsc_w(float d, float w[3])
 {
     const float a = 0.5f - d, b = 0.5f + d;
@@ -227,15 +243,12 @@ __device__ __forceinline__ bool node_change(float o, float on, int sh)
     return lo0 != lo1;
 }
 
-// Esirkepov deposit of one particle (local cell l, shift sh, offsets o -> on) on the base-m
-// deposition tile; big[d] (uniform over the wavefront) selects the window size per dimension
+// window base wb and the old shape S0 / shape change DS of one particle (local cell l, shift sh,
+// offsets o -> on) on the base-m deposition tile; big[d] selects the window size per dimension
 template <int SHAPE>
-__device__ __forceinline__ void esirkepov_window(double* jt, int dtn, int dsx, int dsy, const int l[3],
-                                                 const int sh[3], const float o[3], const float on[3],
-                                                 const bool big[3], float qw, const float hdt[3])
+__device__ __forceinline__ void esk_shapes(const int l[3], const int sh[3], const float o[3], const float on[3],
+                                           const bool big[3], int wb[3], float S0[3][4], float DS[3][4])
 {
-    float S0[3][4], DS[3][4];
-    int wb[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         const int nc0 = l[d] + 2, nc1 = l[d] + sh[d] + 2;
@@ -252,6 +265,18 @@ __device__ __forceinline__ void esirkepov_window(double* jt, int dtn, int dsx, i
 #pragma unroll
         for (int k = 0; k < 4; k++) DS[d][k] = S1[k] - S0[d][k];
     }
+}
+
+// Esirkepov deposit of one particle on the deposition tile; big[d] (uniform over the wavefront)
+// selects the window size per dimension
+template <int SHAPE>
+__device__ __forceinline__ void esirkepov_window(double* jt, int dtn, int dsx, int dsy, const int l[3],
+                                                 const int sh[3], const float o[3], const float on[3],
+                                                 const bool big[3], float qw, const float hdt[3])
+{
+    float S0[3][4], DS[3][4];
+    int wb[3];
+    esk_shapes<SHAPE>(l, sh, o, on, big, wb, S0, DS);
     const float kq[3] = {-qw * hdt[0], -qw * hdt[1], -qw * hdt[2]};
     constexpr int KS = SHAPE == PK_TSC ? 3 : 2;    // window without a node change
     const int sel = (big[0] ? 4 : 0) | (big[1] ? 2 : 0) | (big[2] ? 1 : 0);
@@ -265,6 +290,57 @@ __device__ __forceinline__ void esirkepov_window(double* jt, int dtn, int dsx, i
     case 6: esirkepov<KS + 1, KS + 1, KS>(jt, dtn, dsx, dsy, wb, S0, DS, kq); break;
     default: esirkepov<KS + 1, KS + 1, KS + 1>(jt, dtn, dsx, dsy, wb, S0, DS, kq); break;
     }
+}
+
+// Esirkepov deposit of two particles P and Q on one K x K x K window (neither changes its node):
+// every tile node receives the sum of both particles' terms with one atomic.  kqQ = 0 (and
+// finite Q shapes) deposits P alone.
+template <int K>
+__device__ __forceinline__ void esirkepov_pair(double* jt, int dtn, int sx, int sy, const int wb[3], const float S0P[3][4],
+                                               const float DSP[3][4], const float kqP[3], const float S0Q[3][4],
+                                               const float DSQ[3][4], const float kqQ[3])
+{
+    const int st[3] = {sx, sy, 1};
+    const int orig = wb[0] * sx + wb[1] * sy + wb[2];
+    const float third = 1.0f / 3.0f;
+#pragma unroll
+    for (int comp = 0; comp < 3; comp++) {
+        const int d1 = (comp + 1) % 3, d2 = (comp + 2) % 3;
+        float AP[K], BP[K], AQ[K], BQ[K];
+#pragma unroll
+        for (int p = 0; p < K; p++) {
+            AP[p] = kqP[comp] * (S0P[d1][p] + 0.5f * DSP[d1][p]);
+            BP[p] = kqP[comp] * (0.5f * S0P[d1][p] + third * DSP[d1][p]);
+            AQ[p] = kqQ[comp] * (S0Q[d1][p] + 0.5f * DSQ[d1][p]);
+            BQ[p] = kqQ[comp] * (0.5f * S0Q[d1][p] + third * DSQ[d1][p]);
+        }
+        double* t = jt + comp * dtn + orig + st[comp];
+#pragma unroll
+        for (int p = 0; p < K; p++)
+#pragma unroll
+            for (int r = 0; r < K; r++) {
+                const float cp = AP[p] * S0P[d2][r] + BP[p] * DSP[d2][r];
+                const float cq = AQ[p] * S0Q[d2][r] + BQ[p] * DSQ[d2][r];
+                float cum = 0.0f;
+#pragma unroll
+                for (int a = 0; a < K - 1; a++) {
+                    cum += DSP[comp][a] * cp + DSQ[comp][a] * cq;
+                    atomicAdd(&t[a * st[comp] + p * st[d1] + r * st[d2]], (double)cum);
+                }
+            }
+    }
+}
+
+// bit k of x moved to bit 2k
+__device__ __forceinline__ uint64_t spread_bits(uint32_t x)
+{
+    uint64_t v = x;
+    v = (v | (v << 16)) & 0x0000FFFF0000FFFFull;
+    v = (v | (v << 8)) & 0x00FF00FF00FF00FFull;
+    v = (v | (v << 4)) & 0x0F0F0F0F0F0F0F0Full;
+    v = (v | (v << 2)) & 0x3333333333333333ull;
+    v = (v | (v << 1)) & 0x5555555555555555ull;
+    return v;
 }
 
 struct SplitParams {
@@ -339,11 +415,18 @@ __device__ __forceinline__ void particles_split_sc(const SplitArgs& A, long s)
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const uint64_t below = (1ull << lane) - 1ull;
-    for (int t = tid; t < 6 * itn; t += PK_BLOCK) {
-        const int comp = t / itn, r = t % itn;
-        const int i = r / isx, rr = r % isx, j = rr / isy, k = rr % isy;
-        if (j < its[1])
-            itile[t] = g.f[PK_EX + comp][((long)(ibase[0] + i) * g.s1 + (ibase[1] + j)) * g.s2 + (ibase[2] + k)];
+    // field tile: a lane holds one (j, k) node of a slice, a wavefront walks the slices (comp, i)
+    const int uwave = __builtin_amdgcn_readfirstlane(wave);
+    for (int jk0 = 0; jk0 < its[1] * its[2]; jk0 += 64) {
+        const int jk = jk0 + lane, j = jk / its[2], k = jk - j * its[2];
+        if (jk < its[1] * its[2]) {
+            const long gofs = (long)(ibase[1] + j) * g.s2 + (ibase[2] + k);
+            float* const lt = itile + j * isy + k;
+            for (int sl = uwave; sl < 6 * its[0]; sl += WAVES) {
+                const int comp = sl / its[0], i = sl - comp * its[0];
+                lt[comp * itn + i * isx] = g.f[PK_EX + comp][(long)(ibase[0] + i) * g.s1 * g.s2 + gofs];
+            }
+        }
     }
     for (int t = tid; t < 3 * dtn; t += PK_BLOCK) jtile[t] = 0.0;
     for (int t = tid; t < 27; t += PK_BLOCK) dircnt[t] = 0;
@@ -356,141 +439,212 @@ __device__ __forceinline__ void particles_split_sc(const SplitArgs& A, long s)
     const long base = off[s];
     const int n = cnt[s];
     const int N1N2 = eg.ncell[1] * eg.ncell[2];
-    // Each wavefront owns the 64-particle groups wave, wave + WAVES, ...  No workgroup barrier
-    // until every particle is pushed, so the wavefronts drift apart and one's LDS atomics overlap
-    // another's arithmetic.  Every pushed particle is written back in place; movers are copied to
-    // the outbox afterwards, in index order.
-    for (int c0 = 64 * wave; c0 < n; c0 += PK_BLOCK) {
-        const int i = c0 + lane;
-        const bool act = i < n;
-        int cell = 0;
-        int c[3] = {0, 0, 0}, l[3] = {0, 0, 0}, sh[3] = {0, 0, 0};
-        float o[3] = {0, 0, 0}, on[3] = {0, 0, 0}, u[3] = {0, 0, 0}, v[3] = {0, 0, 0}, w = 0.0f;
-        bool ok = act;
-        // ---- load, interpolate, push, move (lane-divergent)
-        if (act) {
-            const long p = base + i;
-            cell = pm.cell[p];
-            o[0] = pm.ox[p]; o[1] = pm.oy[p]; o[2] = pm.oz[p];
-            u[0] = pm.ux[p]; u[1] = pm.uy[p]; u[2] = pm.uz[p];
-            w = pm.w[p];
-            cell_coords(cell, eg, c);
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                l[d] = c[d] - m0[d];
-                if (l[d] < 0 || l[d] >= nn[d] || !(o[d] >= 0.0f && o[d] < 1.0f)) ok = false;
+    // Each wavefront owns the 128-slot groups wave, wave + WAVES, ...; a lane pushes the two
+    // adjacent slots c0 + 2 lane and c0 + 2 lane + 1.  No workgroup barrier until every particle
+    // is pushed, so the wavefronts drift apart and one's LDS atomics overlap another's
+    // arithmetic.  Every pushed particle is written back in place; movers are copied to the
+    // outbox afterwards, in index order.  Esirkepov: when both particles of a lane keep their
+    // node in every dimension and share a deposition window (the interleaved order puts two
+    // particles of one cell in adjacent slots), their terms are summed and deposited together.
+    struct PState {
+        bool ok, queued, mover;
+        int cls, l[3], sh[3];
+        float o[3], on[3], qw;
+    };
+    auto push = [&](int i, PState& P) {
+        P.cls = 0;
+        P.queued = false;
+            const bool act = i < n;
+            int cell = 0;
+            int c[3] = {0, 0, 0}, l[3] = {0, 0, 0}, sh[3] = {0, 0, 0};
+            float o[3] = {0, 0, 0}, on[3] = {0, 0, 0}, u[3] = {0, 0, 0}, v[3] = {0, 0, 0}, w = 0.0f;
+            bool ok = act;
+            // ---- load, interpolate, push, move (lane-divergent)
+            if (act) {
+                const long p = base + i;
+                cell = pm.cell[p];
+                o[0] = pm.ox[p]; o[1] = pm.oy[p]; o[2] = pm.oz[p];
+                u[0] = pm.ux[p]; u[1] = pm.uy[p]; u[2] = pm.uz[p];
+                w = pm.w[p];
+                cell_coords(cell, eg, c);
+    #pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    l[d] = c[d] - m0[d];
+                    if (l[d] < 0 || l[d] >= nn[d] || !(o[d] >= 0.0f && o[d] < 1.0f)) ok = false;
+                }
+                ok = ok && (unsigned)cell < (unsigned)(eg.ncell[0] * N1N2);
             }
-            ok = ok && (unsigned)cell < (unsigned)(eg.ncell[0] * N1N2);
-        }
-        if (ok) {
-            float e[3], b[3];
-            interp_fields<SHAPE>(itile, itn, isx, isy, l, o, e, b);
-            boris_f32(u, e, b, sp.coeff);
-            ok = isfinite(u[0]) && isfinite(u[1]) && isfinite(u[2]);
-            const float invGamma = rsqrtf(1.0f + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
-#pragma unroll
-            for (int d = 0; d < 3; d++) v[d] = u[d] * invGamma;    // in units of c
-            move_split(o, v, sp.cdt_h, on, sh);
-        }
-        if (act && !ok) bad[0] = 1;                     // left unchanged and reported
-        // ---- deposition
-        const float qw = sp.q * w;
-        if (DEP == PK_ESIRKEPOV) {
-            // A wavefront deposits on the window its widest lane needs.  Particles that change
-            // their node in some dimension (the larger window there) go to the workgroup's change
-            // queue, deposited after the loop by full wavefronts; the rest deposit now on the
-            // small window.  A full queue sends the overflow back to its own wavefront.
-            int cls = 0;                                  // bit d: node change in dimension d
-            if (ok)
-                cls = (node_change<SHAPE>(o[0], on[0], sh[0]) ? 4 : 0) |
-                      (node_change<SHAPE>(o[1], on[1], sh[1]) ? 2 : 0) | (node_change<SHAPE>(o[2], on[2], sh[2]) ? 1 : 0);
-            const uint64_t cb = __ballot(cls != 0);
-            bool queued = false;
-            if (cb) {
-                int q0 = 0;
-                if (lane == 0) q0 = atomicAdd(qlen, __popcll(cb));
-                q0 = __shfl(q0, 0);
-                const int slot = q0 + __popcll(cb & below);
-                if (cls != 0 && slot < QCAP) {
-                    queued = true;
-                    rec[0 * QCAP + slot] = l[0] | (l[1] << 4) | (l[2] << 8) | ((sh[0] + 1) << 12) |
-                                           ((sh[1] + 1) << 14) | ((sh[2] + 1) << 16) | (cls << 18);
-                    rec[1 * QCAP + slot] = __float_as_int(o[0]);
-                    rec[2 * QCAP + slot] = __float_as_int(o[1]);
-                    rec[3 * QCAP + slot] = __float_as_int(o[2]);
-                    rec[4 * QCAP + slot] = __float_as_int(on[0]);
-                    rec[5 * QCAP + slot] = __float_as_int(on[1]);
-                    rec[6 * QCAP + slot] = __float_as_int(on[2]);
-                    rec[7 * QCAP + slot] = __float_as_int(qw);
+            if (ok) {
+                float e[3], b[3];
+                interp_fields<SHAPE>(itile, itn, isx, isy, l, o, e, b);
+                boris_f32(u, e, b, sp.coeff);
+                ok = isfinite(u[0]) && isfinite(u[1]) && isfinite(u[2]);
+                const float invGamma = rsqrtf(1.0f + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    #pragma unroll
+                for (int d = 0; d < 3; d++) v[d] = u[d] * invGamma;    // in units of c
+                move_split(o, v, sp.cdt_h, on, sh);
+            }
+            if (act && !ok) bad[0] = 1;                     // left unchanged and reported
+            // ---- deposition
+            const float qw = sp.q * w;
+            if (DEP == PK_ESIRKEPOV) {
+                // A wavefront deposits on the window its widest lane needs.  Particles that change
+                // their node in some dimension (the larger window there) go to the workgroup's change
+                // queue, deposited after the loop by full wavefronts; the rest deposit now on the
+                // small window.  A full queue sends the overflow back to its own wavefront.
+                int cls = 0;                                  // bit d: node change in dimension d
+                if (ok)
+                    cls = (node_change<SHAPE>(o[0], on[0], sh[0]) ? 4 : 0) |
+                          (node_change<SHAPE>(o[1], on[1], sh[1]) ? 2 : 0) | (node_change<SHAPE>(o[2], on[2], sh[2]) ? 1 : 0);
+                const uint64_t cb = __ballot(cls != 0);
+                bool queued = false;
+                if (cb) {
+                    int q0 = 0;
+                    if (lane == 0) q0 = atomicAdd(qlen, __popcll(cb));
+                    q0 = __shfl(q0, 0);
+                    const int slot = q0 + __popcll(cb & below);
+                    if (cls != 0 && slot < QCAP) {
+                        queued = true;
+                        rec[0 * QCAP + slot] = l[0] | (l[1] << 4) | (l[2] << 8) | ((sh[0] + 1) << 12) |
+                                               ((sh[1] + 1) << 14) | ((sh[2] + 1) << 16) | (cls << 18);
+                        rec[1 * QCAP + slot] = __float_as_int(o[0]);
+                        rec[2 * QCAP + slot] = __float_as_int(o[1]);
+                        rec[3 * QCAP + slot] = __float_as_int(o[2]);
+                        rec[4 * QCAP + slot] = __float_as_int(on[0]);
+                        rec[5 * QCAP + slot] = __float_as_int(on[1]);
+                        rec[6 * QCAP + slot] = __float_as_int(on[2]);
+                        rec[7 * QCAP + slot] = __float_as_int(qw);
+                    }
+                }
+                P.cls = cls;
+                P.queued = queued;
+            } else if (ok) {
+                // direct deposition of q w v at the new position: the CIC tile is pica's (base m+1,
+                // the interpolation indices), the TSC tile has base m (indices + 1)
+                constexpr int KK = SHAPE == PK_TSC ? 3 : 2;
+                int bc[3], bs[3];
+                float wc[3][3], ws[3][3];
+                bool fits = true;
+    #pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    interp_stencil<SHAPE>(l[d] + sh[d], on[d], bc[d], wc[d], bs[d], ws[d]);
+                    if (SHAPE == PK_TSC) {
+                        bc[d] += 1;
+                        bs[d] += 1;
+                    }
+                    if (bc[d] < 0 || bc[d] + KK > dts[d] || bs[d] < 0 || bs[d] + KK > dts[d]) fits = false;
+                }
+                if (fits) {
+    #pragma unroll
+                    for (int comp = 0; comp < 3; comp++) {
+                        const float val = v[comp] * (float)PK_C * qw;
+                        double* t = jtile + comp * dtn;
+                        int bi[3];
+                        float wt[3][3];
+    #pragma unroll
+                        for (int d = 0; d < 3; d++) {
+                            const bool col = colloc(comp, d);
+                            bi[d] = col ? bc[d] : bs[d];
+    #pragma unroll
+                            for (int k = 0; k < 3; k++) wt[d][k] = col ? wc[d][k] : ws[d][k];
+                        }
+                        double* tb = t + bi[0] * dsx + bi[1] * dsy + bi[2];
+    #pragma unroll
+                        for (int a = 0; a < KK; a++)
+    #pragma unroll
+                            for (int bb = 0; bb < KK; bb++)
+    #pragma unroll
+                                for (int cc = 0; cc < KK; cc++)
+                                    atomicAdd(&tb[a * dsx + bb * dsy + cc], (double)(wt[0][a] * wt[1][bb] * wt[2][cc] * val));
+                    }
+                } else {
+                    ok = false;
+                    bad[0] = 1;
                 }
             }
-            const bool now = ok && !queued;
+            // ---- periodic wrap of the cell; a mover leaves the supercell
+            bool mover = false;
+            if (ok) {
+                int cn[3];
+    #pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    cn[d] = c[d] + sh[d];
+                    if (cn[d] < 0) cn[d] += eg.ncell[d];
+                    if (cn[d] >= eg.ncell[d]) cn[d] -= eg.ncell[d];
+                    mover = mover || cn[d] / nn[d] != sidx[d];
+                }
+                const long p = base + i;
+                pm.cell[p] = (cn[0] * eg.ncell[1] + cn[1]) * eg.ncell[2] + cn[2];
+                pm.ox[p] = on[0]; pm.oy[p] = on[1]; pm.oz[p] = on[2];
+                pm.ux[p] = u[0]; pm.uy[p] = u[1]; pm.uz[p] = u[2];
+            }
+        P.ok = ok;
+        P.mover = mover;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            P.l[d] = l[d];
+            P.sh[d] = sh[d];
+            P.o[d] = o[d];
+            P.on[d] = on[d];
+        }
+        P.qw = qw;
+    };
+    // a particle left over by a full change queue deposits alone, on the window the widest such
+    // lane of the wavefront needs
+    auto overflow = [&](const PState& P) {
+        const bool direct = P.ok && !P.queued && P.cls != 0;
+        if (__ballot(direct)) {
             bool big[3];
 #pragma unroll
-            for (int d = 0; d < 3; d++) big[d] = __ballot(now && ((cls >> (2 - d)) & 1)) != 0ull;
-            if (now) esirkepov_window<SHAPE>(jtile, dtn, dsx, dsy, l, sh, o, on, big, qw, sp.hdt);
-        } else if (ok) {
-            // direct deposition of q w v at the new position: the CIC tile is pica's (base m+1,
-            // the interpolation indices), the TSC tile has base m (indices + 1)
-            constexpr int KK = SHAPE == PK_TSC ? 3 : 2;
-            int bc[3], bs[3];
-            float wc[3][3], ws[3][3];
-            bool fits = true;
+            for (int d = 0; d < 3; d++) big[d] = __ballot(direct && ((P.cls >> (2 - d)) & 1)) != 0ull;
+            if (direct) esirkepov_window<SHAPE>(jtile, dtn, dsx, dsy, P.l, P.sh, P.o, P.on, big, P.qw, sp.hdt);
+        }
+    };
+    for (int c0 = 128 * wave; c0 < n; c0 += 128 * WAVES) {
+        PState A, B;
+        push(c0 + 2 * lane, A);
+        push(c0 + 2 * lane + 1, B);
+        const uint64_t ma = __ballot(A.mover), mbm = __ballot(B.mover);
+        if (lane == 0) {
+            mask[c0 >> 6] = spread_bits((uint32_t)ma) | (spread_bits((uint32_t)mbm) << 1);
+            if (c0 + 64 < n) mask[(c0 >> 6) + 1] = spread_bits((uint32_t)(ma >> 32)) | (spread_bits((uint32_t)(mbm >> 32)) << 1);
+        }
+        if (DEP == PK_ESIRKEPOV) {
+            constexpr int KS = SHAPE == PK_TSC ? 3 : 2;    // window without a node change
+            overflow(A);
+            overflow(B);
+            const bool sa = A.ok && A.cls == 0, sb = B.ok && B.cls == 0;
+            const bool small[3] = {false, false, false};
+            float S0a[3][4] = {}, DSa[3][4] = {}, S0b[3][4] = {}, DSb[3][4] = {};
+            int wa[3] = {0, 0, 0}, wbb[3] = {0, 0, 0};
+            if (sa) esk_shapes<SHAPE>(A.l, A.sh, A.o, A.on, small, wa, S0a, DSa);
+            if (sb) esk_shapes<SHAPE>(B.l, B.sh, B.o, B.on, small, wbb, S0b, DSb);
+            const bool merge = sa && sb && wa[0] == wbb[0] && wa[1] == wbb[1] && wa[2] == wbb[2];
+            if (sa || sb) {
+                // A (with B when they share the window), or B alone when A has no small window
+                int wp[3];
+                float S0p[3][4], DSp[3][4], kp[3], kqq[3];
 #pragma unroll
-            for (int d = 0; d < 3; d++) {
-                interp_stencil<SHAPE>(l[d] + sh[d], on[d], bc[d], wc[d], bs[d], ws[d]);
-                if (SHAPE == PK_TSC) {
-                    bc[d] += 1;
-                    bs[d] += 1;
-                }
-                if (bc[d] < 0 || bc[d] + KK > dts[d] || bs[d] < 0 || bs[d] + KK > dts[d]) fits = false;
-            }
-            if (fits) {
+                for (int d = 0; d < 3; d++) {
+                    wp[d] = sa ? wa[d] : wbb[d];
 #pragma unroll
-                for (int comp = 0; comp < 3; comp++) {
-                    const float val = v[comp] * (float)PK_C * qw;
-                    double* t = jtile + comp * dtn;
-                    int bi[3];
-                    float wt[3][3];
-#pragma unroll
-                    for (int d = 0; d < 3; d++) {
-                        const bool col = colloc(comp, d);
-                        bi[d] = col ? bc[d] : bs[d];
-#pragma unroll
-                        for (int k = 0; k < 3; k++) wt[d][k] = col ? wc[d][k] : ws[d][k];
+                    for (int k = 0; k < 4; k++) {
+                        S0p[d][k] = sa ? S0a[d][k] : S0b[d][k];
+                        DSp[d][k] = sa ? DSa[d][k] : DSb[d][k];
                     }
-                    double* tb = t + bi[0] * dsx + bi[1] * dsy + bi[2];
-#pragma unroll
-                    for (int a = 0; a < KK; a++)
-#pragma unroll
-                        for (int bb = 0; bb < KK; bb++)
-#pragma unroll
-                            for (int cc = 0; cc < KK; cc++)
-                                atomicAdd(&tb[a * dsx + bb * dsy + cc], (double)(wt[0][a] * wt[1][bb] * wt[2][cc] * val));
+                    kp[d] = -(sa ? A.qw : B.qw) * sp.hdt[d];
+                    kqq[d] = merge ? -B.qw * sp.hdt[d] : 0.0f;
                 }
-            } else {
-                ok = false;
-                bad[0] = 1;
+                esirkepov_pair<KS>(jtile, dtn, dsx, dsy, wp, S0p, DSp, kp, S0b, DSb, kqq);
+            }
+            const bool second = sa && sb && !merge;
+            if (__ballot(second)) {
+                if (second) {
+                    const float kb[3] = {-B.qw * sp.hdt[0], -B.qw * sp.hdt[1], -B.qw * sp.hdt[2]};
+                    esirkepov<KS, KS, KS>(jtile, dtn, dsx, dsy, wbb, S0b, DSb, kb);
+                }
             }
         }
-        // ---- periodic wrap of the cell; a mover leaves the supercell
-        bool mover = false;
-        if (ok) {
-            int cn[3];
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                cn[d] = c[d] + sh[d];
-                if (cn[d] < 0) cn[d] += eg.ncell[d];
-                if (cn[d] >= eg.ncell[d]) cn[d] -= eg.ncell[d];
-                mover = mover || cn[d] / nn[d] != sidx[d];
-            }
-            const long p = base + i;
-            pm.cell[p] = (cn[0] * eg.ncell[1] + cn[1]) * eg.ncell[2] + cn[2];
-            pm.ox[p] = on[0]; pm.oy[p] = on[1]; pm.oz[p] = on[2];
-            pm.ux[p] = u[0]; pm.uy[p] = u[1]; pm.uz[p] = u[2];
-        }
-        const uint64_t mb = __ballot(mover);
-        if (lane == 0) mask[c0 >> 6] = mb;
     }
     __syncthreads();
     if (DEP == PK_ESIRKEPOV) {
@@ -537,9 +691,19 @@ __device__ __forceinline__ void particles_split_sc(const SplitArgs& A, long s)
             const int wd = y >> 6, bt = y & 63;
             return wpre[wd] + (bt ? __popcll(mask[wd] & ((1ull << bt) - 1ull)) : 0);
         };
-        for (int i = tid; i < n; i += PK_BLOCK) {
-            if (!((mask[i >> 6] >> (i & 63)) & 1ull)) continue;
-            const long src = base + i, q = base + movers_below(i);
+        // slot of the r-th mover (r < nmv): its mask word is the last one with wpre <= r
+        auto mover_slot = [&](long r) -> int {
+            int lo = 0, hi = words - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (wpre[mid] <= r) lo = mid; else hi = mid - 1;
+            }
+            return (lo << 6) + select_bit(mask[lo], (int)(r - wpre[lo]));
+        };
+        // one lane per mover (not per slot): the t-th mover goes to outbox slot t
+        for (int t = tid; t < nmv; t += PK_BLOCK) {
+            const int i = mover_slot(t);
+            const long src = base + i, q = base + t;
             const int cl = pm.cell[src];
             po.cell[q] = cl;
             po.ox[q] = pm.ox[src]; po.oy[q] = pm.oy[src]; po.oz[q] = pm.oz[src];
@@ -563,20 +727,12 @@ __device__ __forceinline__ void particles_split_sc(const SplitArgs& A, long s)
         for (int t = tid; t < nmv; t += PK_BLOCK) {
             const int j = nfinal + t;
             if ((mask[j >> 6] >> (j & 63)) & 1ull) continue;
-            const long r = (long)t - (movers_below(j) - below_final);
-            int lo = 0, hi = words - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (wpre[mid] <= r) lo = mid; else hi = mid - 1;
-            }
-            uint64_t m = mask[lo];
-            for (long z = r - wpre[lo]; z > 0; z--) m &= m - 1ull;
-            if (m == 0ull) {
+            const long r = (long)t - (movers_below(j) - below_final);   // its rank among the tail stayers
+            if (r < 0 || r >= below_final) {                             // the r-th hole is the r-th mover
                 bad[0] = 1;
                 continue;
             }
-            const int h = (lo << 6) + (__ffsll((long long)m) - 1);
-            const long src = base + j, dst = base + h;
+            const long src = base + j, dst = base + mover_slot(r);
             pm.cell[dst] = pm.cell[src];
             pm.ox[dst] = pm.ox[src]; pm.oy[dst] = pm.oy[src]; pm.oz[dst] = pm.oz[src];
             pm.ux[dst] = pm.ux[src]; pm.uy[dst] = pm.uy[src]; pm.uz[dst] = pm.uz[src];
@@ -584,14 +740,19 @@ __device__ __forceinline__ void particles_split_sc(const SplitArgs& A, long s)
         }
     }
     __syncthreads();
-    for (int t = tid; t < 3 * dtn; t += PK_BLOCK) {
-        const double v = jtile[t];
-        if (v == 0.0) continue;
-        const int comp = t / dtn, r = t % dtn;
-        const int i = r / dsx, rr = r % dsx, j = rr / dsy, k = rr % dsy;
-        if (j >= dts[1]) continue;                     // padding
-        atomicAdd(&g.f[PK_JX + comp][((long)(dbase[0] + i) * g.s1 + (dbase[1] + j)) * g.s2 + (dbase[2] + k)],
-                  (float)(v * (double)sp.norm));
+    // J flush: a lane holds one (j, k) node of a slice, a wavefront walks the slices (comp, i)
+    for (int jk0 = 0; jk0 < dts[1] * dts[2]; jk0 += 64) {
+        const int jk = jk0 + lane, j = jk / dts[2], k = jk - j * dts[2];
+        if (jk < dts[1] * dts[2]) {
+            const long gofs = (long)(dbase[1] + j) * g.s2 + (dbase[2] + k);
+            const double* const lt = jtile + j * dsy + k;
+            for (int sl = uwave; sl < 3 * dts[0]; sl += WAVES) {
+                const int comp = sl / dts[0], i = sl - comp * dts[0];
+                const double v = lt[comp * dtn + i * dsx];
+                if (v != 0.0)
+                    atomicAdd(&g.f[PK_JX + comp][(long)(dbase[0] + i) * g.s1 * g.s2 + gofs], (float)(v * (double)sp.norm));
+            }
+        }
     }
     if (tid < 27) mvc[s * 27 + tid] = dircnt[tid];
     if (tid == 0) {
@@ -601,8 +762,11 @@ __device__ __forceinline__ void particles_split_sc(const SplitArgs& A, long s)
     }
 }
 
+// At least 5 wavefronts per SIMD: the LDS tiles allow 5 workgroups per CU, and the TSC
+// Esirkepov body (two particles per lane) needs ~111 VGPRs, i.e. 4 wavefronts, unless the
+// compiler keeps it within 96 (a few spilled dwords; 14.5 -> 14.1 ms per step at 256^3 x 16 ppc).
 template <int SHAPE, int DEP, int NC>
-__global__ void __launch_bounds__(PK_BLOCK) k_particles_split(SplitArgs A)
+__global__ void __launch_bounds__(PK_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) k_particles_split(SplitArgs A)
 {
     const long s = pk_xcd_supercell(blockIdx.x, A.eg.xcd_chunk);
     if (s >= A.eg.nsct) return;
@@ -611,7 +775,7 @@ __global__ void __launch_bounds__(PK_BLOCK) k_particles_split(SplitArgs A)
 
 // many ensembles: a list of (ensemble, supercell) items; gridDim.x < nitems makes it persistent
 template <int SHAPE, int DEP, int NC>
-__global__ void __launch_bounds__(PK_BLOCK) k_particles_split_batch(const SplitArgs* __restrict__ jobs,
+__global__ void __launch_bounds__(PK_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) k_particles_split_batch(const SplitArgs* __restrict__ jobs,
                                                                     const int2* __restrict__ items, long nitems,
                                                                     const int* __restrict__ halt)
 {

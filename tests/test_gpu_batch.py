@@ -158,3 +158,38 @@ def test_batch_autotune_restores_state(ctx):
     for i, ((e, g), (re_, rg)) in enumerate(zip(mine, ref)):
         _compare(P.F32, _state(e, g), _state(re_, rg), "ensemble %d" % i)
     b.close()
+
+
+@pytest.mark.parametrize("period,graph,uth,slack", [(2, 1, 0.05, None), (3, 0, 0.05, None), (1, 1, 0.2, (0.0, 0))])
+def test_batch_interleave_matches_one_by_one(ctx, period, graph, uth, slack):
+    """with the periodic interleave on, the batch reorders each FP32 ensemble after the same
+    re-sorts as one-by-one stepping -- counted per ensemble (one ensemble starts a step ahead),
+    and also after an overflow rebuild (no slack, hot plasma) -- and ends in the same state,
+    particle order included"""
+    _say("batch interleave period=%d graph=%d uth=%g" % (period, graph, uth))
+    ctx.set_option(P.OPT_INTERLEAVE, period)
+    dt = courant_dt()
+    mine = _pairs(ctx, P.F32, uth=uth, slack=slack)
+    ref = _pairs(ctx, P.F32, uth=uth, slack=slack)
+    for e, g in (mine[0], ref[0]):
+        P.step(e, g, dt, P.TSC, P.ESIRKEPOV)
+    b = P.Batch(ctx, mine)
+    b.set(P.BATCH_GRAPH, graph)
+    ctx.timer_reset()
+    ctx.timer_enable(True)
+    try:
+        b.run(dt, 7, P.TSC, P.ESIRKEPOV)
+        for e, g in ref:
+            P.run(e, g, dt, 7, P.TSC, P.ESIRKEPOV)
+        ctx.sync()
+        _, rebuilds = ctx.timer("rebuild")
+    finally:
+        ctx.timer_enable(False)
+    worst = {}
+    for i, ((e, g), (re_, rg)) in enumerate(zip(mine, ref)):
+        errs = _compare(P.F32, _state(e, g), _state(re_, rg), "ensemble %d" % i)
+        worst = {k: max(v, worst.get(k, 0.0)) for k, v in errs.items()}
+    print("batch vs one-by-one with interleave period %d: %s, rebuilds %d" % (period, worst, rebuilds), flush=True)
+    if slack is not None:
+        assert rebuilds > 0
+    b.close()

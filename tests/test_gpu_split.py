@@ -135,8 +135,9 @@ def test_split_cells_and_counts(ctx):
 
 
 def _interleave_order(cells, off, ncell, ncps):
-    """the order pk_ens_interleave gives each supercell: by (rank of the particle among those of
-    its cell in the current order, local cell index)"""
+    """the order pk_ens_interleave gives each supercell, with r the rank of a particle among those
+    of its cell in the current order and h the cell's count: the pairs (ranks 2q, 2q + 1) by
+    (q, local cell, r mod 2), then the last particle of each odd-count cell by local cell"""
     perm = np.arange(cells.shape[0])
     for s in range(len(off) - 1):
         a, b = off[s], off[s + 1]
@@ -148,13 +149,15 @@ def _interleave_order(cells, off, ncell, ncps):
         for i, k in enumerate(loc):
             rank[i] = seen.get(k, 0)
             seen[k] = rank[i] + 1
-        perm[a:b] = a + np.lexsort((loc, rank))
+        h = np.array([seen[k] for k in loc], np.int64)
+        single = (h % 2 == 1) & (rank == h - 1)
+        perm[a:b] = a + np.lexsort((rank % 2, loc, np.where(single, 0, rank // 2), single))
     return perm
 
 
 def test_interleave_order_and_set(ctx):
     """pk_ens_interleave keeps every particle (bits included) and every supercell's membership,
-    and orders each supercell by (rank within its cell, cell)"""
+    and orders each supercell as pairs of one cell by (pair rank, cell), odd leftovers last"""
     _say("interleave")
     ncell, ncps, ppc = (8, 12, 8), (4, 4, 4), 6
     dt, x, u, w = _inputs(88, ncell, ppc)
@@ -175,7 +178,46 @@ def test_interleave_order_and_set(ctx):
     assert not np.array_equal(perm, np.arange(perm.shape[0]))
     assert np.array_equal(x1, x0[perm]) and np.array_equal(u1, u0[perm]) and np.array_equal(w1, w0[perm])
     _, cells1 = ens.cells()
-    # the first particles of each supercell now sit in distinct cells
+    # the first pairs of each supercell now sit in distinct cells, two particles of one cell each
     for s in range(len(off1) - 1):
         head = cells1[off1[s]:off1[s + 1]][:32]
-        assert len(np.unique(head)) == len(head)
+        assert np.array_equal(head[0::2], head[1::2])
+        assert len(np.unique(head[0::2])) == len(head) // 2
+
+
+@pytest.mark.parametrize("ppc", [4, 5])
+def test_loader_pairs_hold_the_fp64_particles(ctx, ppc):
+    """the FP32 loader places particle k of every cell in pairs (k = 2q, 2q + 1 adjacent, pairs
+    by (q, cell), particle ppc - 1 of every cell last when ppc is odd); the FP64 loader places it
+    at k * cells + cell; both hold the same particles (positions to FP32 rounding, momenta and
+    weights exactly rounded)"""
+    _say("loader pairs ppc=%d" % ppc)
+    ncell, ncps = (8, 8, 12), (4, 4, 4)
+    ens = {}
+    for prec in (P.F64, P.F32):
+        e = P.EnsembleSupercells(ctx, ncell, (1.0, 1.0, 1.0), ncps, prec=prec)
+        e.load_thermal(ppc, 0.05, 0.1, 1.0, 9)
+        ens[prec] = e
+    x64, u64, w64, off = ens[P.F64].particles()
+    x32, u32, w32, off32 = ens[P.F32].particles()
+    assert np.array_equal(off, off32)
+    ncsc = int(np.prod(ncps))
+    npair = (ppc & ~1) * ncsc
+    perm = []
+    for s in range(len(off) - 1):
+        for l in range(ppc * ncsc):
+            if l < npair:
+                lc, k = (l >> 1) % ncsc, 2 * ((l >> 1) // ncsc) + (l & 1)
+            else:
+                lc, k = l - npair, ppc - 1
+            perm.append(off[s] + k * ncsc + lc)
+    perm = np.array(perm)
+    assert np.abs(x32 - x64[perm]).max() < 1e-5
+    assert np.array_equal(u32, u64[perm].astype(np.float32).astype(np.float64))
+    assert np.array_equal(w32, w64[perm].astype(np.float32).astype(np.float64))
+    _, cells = ens[P.F32].cells()
+    for s in range(len(off) - 1):
+        c = cells[off[s]:off[s] + npair]
+        assert np.array_equal(c[0::2], c[1::2])
+    for e in ens.values():
+        e.close()
