@@ -14,8 +14,11 @@
 // dimension only when some lane of the wavefront changes its nearest node there, else 3 / 2.
 // Particles that change node go to a workgroup change queue deposited by full wavefronts after
 // the push, so most wavefronts issue only the small window.  The push loop has no workgroup
-// barrier: each wavefront owns every fourth 64-particle group and the four drift apart, which
-// lets one wavefront's LDS atomics overlap another's arithmetic.
+// barrier: each wavefront owns every fourth 128-slot group and the four drift apart, which
+// lets one wavefront's LDS atomics overlap another's arithmetic.  A lane pushes two adjacent
+// slots; when both particles keep their node and share a window (the loader and the periodic
+// interleave place two particles of one cell side by side) their Esirkepov terms are summed
+// before the LDS atomics, which halves the atomics of such a pair.
 //
 // Stencil conventions (grid origin -2h, two ghost layers; l = cell - supercell start):
 //   interpolation tile base = array index m+1 (pica's baseIdx), CIC n+2 / TSC n+3 wide;
