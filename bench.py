@@ -7,7 +7,7 @@ with Maxwellian u (sigma 0.01) plus a +-0.1c two-stream drift in x on alternate 
 omega_p dt = 0.1, TSC form factor, Boris push, Esirkepov (charge-conserving) deposition,
 4^3 supercells, re-sorted every step.  A "step" is the whole PIC step: J = 0, fused
 interpolation + push + deposition, J fold, B/2-E-B/2 Yee with ghost fills, re-sort (FP32: with
-the supercell interleave every 4th re-sort, the library's default).
+the supercell interleave every 3rd re-sort, the library's default).
 
 value = particles x steps / wall time of the timed steps (inputs resident in HBM).
 roofline: the fused particle kernel, algorithmic bytes per particle-step (FP64:
@@ -324,7 +324,7 @@ def main():
                                "%d^3 supercells re-sorted every step, %s; whole PIC step timed"
                                % (args.n, args.ppc, res["particles"], args.shape.upper(), args.dep, args.sc,
                                   "FP32 split positions (int32 cell + FP32 offset), supercells interleaved "
-                                  "every 4th re-sort" if head == "f32" else "FP64"),
+                                  "every 3rd re-sort" if head == "f32" else "FP64"),
                    "grid": [args.n] * 3, "ppc": args.ppc, "particles": res["particles"],
                    "particle_kernel": args.kernel},
         "roofline": res.get("roofline"),

@@ -78,7 +78,7 @@ int pk_timer_reset(pk_ctx* ctx);
  * 2^3 or 4^3 supercells).
  * PK_OPT_CHANGE_QUEUE caps the cell-sorted kernel's LDS queue of cell-changing particles
  * (0 = as many entries as the LDS budget allows; beyond the queue a lane deposits at once).
- * PK_OPT_INTERLEAVE (default 4): every that many re-sorts of an FP32 ensemble, pk_ens_resort
+ * PK_OPT_INTERLEAVE (default 3): every that many re-sorts of an FP32 ensemble, pk_ens_resort
  * reorders each supercell's particles: pairs of particles of one cell in adjacent slots, ordered
  * by (pair rank within the cell, cell), then the odd particle of every odd-count cell.  A lane of
  * the particle kernel takes two adjacent slots and deposits a same-cell pair with one set of LDS

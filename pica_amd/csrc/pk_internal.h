@@ -39,7 +39,7 @@ struct pk_ctx {
     std::vector<hipEvent_t> evpool;
     std::map<std::string, pk_timer_rec> timers;
     int32_t* flags_host = nullptr;     // pinned, PK_NFLAGS
-    int opt[PK_OPT_N] = {PK_KERNEL_AUTO, 0, 4};
+    int opt[PK_OPT_N] = {PK_KERNEL_AUTO, 0, 3};
 };
 
 enum { PK_FLAG_ERR = 0, PK_FLAG_OVERFLOW = 1, PK_NFLAGS = 4 };

@@ -22,10 +22,12 @@ def _say(msg):
 
 @pytest.fixture(autouse=True)
 def _no_interleave(ctx):
-    """batched stepping does not reorder supercells, so the one-by-one reference must not either"""
+    """these tests compare batched with one-by-one stepping without the periodic interleave (its
+    own test below turns it on for both)"""
+    period = ctx.get_option(P.OPT_INTERLEAVE)
     ctx.set_option(P.OPT_INTERLEAVE, 0)
     yield
-    ctx.set_option(P.OPT_INTERLEAVE, 4)
+    ctx.set_option(P.OPT_INTERLEAVE, period)
 
 
 def _pair(ctx, prec, spec, uth=0.05, slack=None):

@@ -26,13 +26,21 @@ def _make(ctx, prec, ncell=(16, 8, 8), ppc=8):
     return dt, w0, g, e
 
 
+@pytest.fixture
+def period4(ctx):
+    old = ctx.get_option(P.OPT_INTERLEAVE)
+    ctx.set_option(P.OPT_INTERLEAVE, 4)
+    yield
+    ctx.set_option(P.OPT_INTERLEAVE, old)
+
+
 @pytest.mark.parametrize("prec,tol", [(P.F64, 1e-12), (P.F32, 1e-5)])
-def test_restart_from_snapshot(ctx, tmp_path, prec, tol):
+def test_restart_from_snapshot(ctx, tmp_path, prec, tol, period4):
     _say("snapshot restart prec=%d" % prec)
     dt, w0, ga, ea = _make(ctx, prec)
     ea.load_thermal(8, 0.05, 0.1 / math.sqrt(0.99), w0, 77)
-    # 3 + 5 steps: the FP32 ensemble is reordered at re-sorts 4 and 8 (the default period), which
-    # the continued run only matches if the load restores the re-sort count
+    # 3 + 5 steps with the interleave period at 4: the FP32 ensemble is reordered at re-sorts 4
+    # and 8, which the continued run only matches if the load restores the re-sort count
     P.run(ea, ga, dt, 3, P.TSC, P.ESIRKEPOV)
     ea.save(tmp_path / "e.snap")
     ga.save(tmp_path / "g.snap")

@@ -164,11 +164,12 @@ def test_interleave_order_and_set(ctx):
     ens = P.EnsembleSupercells(ctx, ncell, (1.0, 1.0, 1.0), ncps, prec=P.F32)
     g = P.YeeGrid.periodic(ctx, ncell, prec=P.F32)
     ens.add(x, u, w)
+    period = ctx.get_option(P.OPT_INTERLEAVE)
     ctx.set_option(P.OPT_INTERLEAVE, 0)
     try:
         P.run(ens, g, dt, 6, P.TSC, P.ESIRKEPOV)          # let the order mix
     finally:
-        ctx.set_option(P.OPT_INTERLEAVE, 4)
+        ctx.set_option(P.OPT_INTERLEAVE, period)
     x0, u0, w0, off0 = ens.particles()
     _, cells0 = ens.cells()
     ens.interleave()
