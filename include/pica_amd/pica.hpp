@@ -62,6 +62,14 @@ public:
         if (r) throw Error(r, pk_last_error(h_));
     }
     void sync() const { check(pk_sync(h_)); }
+    // library options (PK_OPT_PARTICLE_KERNEL, PK_OPT_CHANGE_QUEUE, PK_OPT_INTERLEAVE)
+    void setOption(int option, int value) const { check(pk_ctx_set_option(h_, option, value)); }
+    int getOption(int option) const
+    {
+        int v = 0;
+        check(pk_ctx_get_option(h_, option, &v));
+        return v;
+    }
     // ParticleTypes::typesVector (Particle.h:26-31)
     void setSpecies(const std::vector<double>& mass, const std::vector<double>& charge) const
     {
@@ -212,6 +220,9 @@ public:
         if (n < 0) ctx_.check(PK_E_STATE);
         return n;
     }
+    // reorder every supercell now (FP32: same-cell pairs in adjacent slots; FP64: no-op), as the
+    // re-sort does every PK_OPT_INTERLEAVE re-sorts
+    void interleave() { ctx_.check(pk_ens_interleave(h_)); }
     // snapshot files: the particles, bits and order; a load needs the same geometry and species
     void save(const std::string& path) { ctx_.check(pk_ens_save(h_, path.c_str())); }
     void load(const std::string& path) { ctx_.check(pk_ens_load(h_, path.c_str())); }

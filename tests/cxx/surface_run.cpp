@@ -51,6 +51,9 @@ int main(int argc, char** argv)
         EnsembleSupercells<double> ens2(ctx, FP3(0, 0, 0), FP3(nc[0], nc[1], nc[2]), ncell, Int3(np[0], np[1], np[2]));
         grid2.load(snap + ".grid");
         ens2.load(snap + ".ens");
+        // options and the supercell interleave (a no-op for FP64 ensembles: the results stay the oracle's)
+        ctx.setOption(PK_OPT_INTERLEAVE, ctx.getOption(PK_OPT_INTERLEAVE));
+        ens2.interleave();
         std::vector<EnsembleSupercells<double>*> species{&ens2};
         for (int s = first; s < nsteps; s++) step(species, grid2, dt, ff, dep);
         std::vector<int64_t> off;
