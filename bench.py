@@ -9,7 +9,9 @@ omega_p dt = 0.1, TSC form factor, Boris push, Esirkepov (charge-conserving) dep
 interpolation + push + deposition, J fold, B/2-E-B/2 Yee with ghost fills, re-sort (FP32: with
 the supercell interleave every 3rd re-sort, the library's default).
 
-value = particles x steps / wall time of the timed steps (inputs resident in HBM).
+value = particles x steps / wall time of the timed steps (inputs resident in HBM).  Defaults: 5
+warm-up steps, then the workload's 200 steps timed (the plasma heats as the two-stream
+instability grows, so later steps cost more than the first ones).
 roofline: the fused particle kernel, algorithmic bytes per particle-step (FP64:
 104 + 72/ppc) x particles / its mean launch time from HIP events on the library's stream.
 cpu_baseline: the CPU oracle (OpenMP restatement of pica's path) on a bounded sample of the
@@ -88,7 +90,7 @@ def run_gpu(args, prec):
     t0 = time.perf_counter()
     for i in range(args.steps):
         P.step(ens, grid, dt, shape, dep)
-        if (i + 1) % 20 == 0:
+        if (i + 1) % 50 == 0:
             print("%s step %d/%d done" % (prec, i + 1, args.steps), file=sys.stderr, flush=True)
     ctx.sync()
     t1 = time.perf_counter()
@@ -266,7 +268,7 @@ def cpu_baseline(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--ppc", type=int, default=16)

@@ -260,22 +260,44 @@ __global__ void k_keys_stage(EnsGeom eg, StageGeom sg, int split, const double* 
     srcs[i] = SRC_STAGE | (uint64_t)i;
 }
 
+// per-destination counts of the sorted keys: the keys of one destination are consecutive, so the
+// first lane of each run within a wavefront adds the run's length (one atomic per run, not per key)
 __global__ void k_count_keys(long n, const uint32_t* keys, int32_t* cnt)
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) atomicAdd(&cnt[keys[i] / 28], 1);
+    const int lane = threadIdx.x & 63;
+    const bool act = i < n;
+    const uint32_t d = act ? keys[i] / 28 : 0xffffffffu;
+    const uint32_t dprev = __shfl_up(d, 1);
+    const bool head = act && (lane == 0 || d != dprev);
+    const uint64_t heads = __ballot(head);
+    const int nact = __popcll(__ballot(act));          // the active lanes are a prefix of the wavefront
+    if (head) {
+        const uint64_t later = lane == 63 ? 0ull : heads & (~0ull << (lane + 1));
+        const int next = later ? __ffsll((long long)later) - 1 : nact;
+        atomicAdd(&cnt[d], next - lane);
+    }
 }
 
 __global__ void k_caps(long nsct, const int32_t* cnt, double frac, int add, int64_t* cap64, int64_t* cnt64,
                        int32_t* maxcap)
 {
     const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nsct) return;
-    const int c = cnt[s];
-    const long cap = (long)c + (long)(c * frac) + add;
-    cap64[s] = cap;
-    cnt64[s] = c;
-    atomicMax(maxcap, (int)cap);
+    int capi = 0;
+    if (s < nsct) {
+        const int c = cnt[s];
+        const long cap = (long)c + (long)(c * frac) + add;
+        cap64[s] = cap;
+        cnt64[s] = c;
+        capi = (int)cap;
+    }
+    // the wavefront's largest capacity, then one atomic per wavefront
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const int t = __shfl_xor(capi, o);
+        capi = t > capi ? t : capi;
+    }
+    if ((threadIdx.x & 63) == 0 && capi > 0) atomicMax(maxcap, capi);
 }
 
 template <class W, int NF>
