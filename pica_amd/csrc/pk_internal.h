@@ -170,6 +170,22 @@ struct EnsGeom {
 };
 
 // 3D cell of a linear physical cell index 0 <= c < 2^31 (z fastest)
+// position of the k-th (from 0) set bit of m, k < popcount(m): popcount halving, 6 steps
+__device__ __forceinline__ int pk_select_bit(uint64_t m, int k)
+{
+    int pos = 0;
+#pragma unroll
+    for (int w = 32; w >= 1; w >>= 1) {
+        const int c = __popcll(m & ((1ull << w) - 1ull));
+        if (k >= c) {
+            k -= c;
+            m >>= w;
+            pos += w;
+        }
+    }
+    return pos;
+}
+
 __device__ __forceinline__ void cell_coords(int c, const EnsGeom& eg, int cc[3])
 {
     const int q12 = (int)fastdiv((uint32_t)c, eg.div12), q2 = (int)fastdiv((uint32_t)c, eg.div2);

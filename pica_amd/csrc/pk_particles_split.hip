@@ -43,22 +43,6 @@ __device__ __forceinline__ bool colloc(int comp, int d)
     return (m[comp] >> d) & 1u;
 }
 
-// position of the k-th (from 0) set bit of m, k < popcount(m): popcount halving, 6 steps
-__device__ __forceinline__ int select_bit(uint64_t m, int k)
-{
-    int pos = 0;
-#pragma unroll
-    for (int w = 32; w >= 1; w >>= 1) {
-        const int c = __popcll(m & ((1ull << w) - 1ull));
-        if (k >= c) {
-            k -= c;
-            m >>= w;
-            pos += w;
-        }
-    }
-    return pos;
-}
-
 __device__ __forceinline__ void tsc_w(float d, float w[3])
 {
     const float a = 0.5f - d, b = 0.5f + d;
@@ -701,7 +685,7 @@ __device__ __forceinline__ void particles_split_sc(const SplitArgs& A, long s)
                 const int mid = (lo + hi + 1) >> 1;
                 if (wpre[mid] <= r) lo = mid; else hi = mid - 1;
             }
-            return (lo << 6) + select_bit(mask[lo], (int)(r - wpre[lo]));
+            return (lo << 6) + pk_select_bit(mask[lo], (int)(r - wpre[lo]));
         };
         // one lane per mover (not per slot): the t-th mover goes to outbox slot t
         for (int t = tid; t < nmv; t += PK_BLOCK) {
