@@ -254,8 +254,8 @@ __device__ __forceinline__ void esk_shapes(const int l[3], const int sh[3], cons
     }
 }
 
-// Esirkepov deposit of one particle on the deposition tile; big[d] (uniform over the wavefront)
-// selects the window size per dimension
+// Esirkepov deposit of one particle on the deposition tile; big[d] selects the window size per
+// dimension (lanes with different choices take different branches of the switch)
 template <int SHAPE>
 __device__ __forceinline__ void esirkepov_window(double* jt, int dtn, int dsx, int dsy, const int l[3],
                                                  const int sh[3], const float o[3], const float on[3],
@@ -654,9 +654,11 @@ __device__ __forceinline__ void particles_split_sc(const SplitArgs& A, long s)
                 d_on[2] = __int_as_float(rec[6 * QCAP + t]);
                 dqw = __int_as_float(rec[7 * QCAP + t]);
             }
+            // each record on its own window: the lanes of one window shape run together (a union
+            // window per wavefront made most x-crossers pay for the few y/z-crossers' window)
             bool big[3];
 #pragma unroll
-            for (int d = 0; d < 3; d++) big[d] = __ballot(has && ((dcls >> (2 - d)) & 1)) != 0ull;
+            for (int d = 0; d < 3; d++) big[d] = ((dcls >> (2 - d)) & 1) != 0;
             if (has) esirkepov_window<SHAPE>(jtile, dtn, dsx, dsy, dl, dsh, d_o, d_on, big, dqw, sp.hdt);
         }
     }
