@@ -82,8 +82,9 @@ int pk_timer_reset(pk_ctx* ctx);
  * reorders each supercell's particles: pairs of particles of one cell in adjacent slots, ordered
  * by (pair rank within the cell, cell), then the odd particle of every odd-count cell.  A lane of
  * the particle kernel takes two adjacent slots and deposits a same-cell pair with one set of LDS
- * atomics, and the lanes of a wavefront sit in different cells.  0 turns it off.  Supercells of more than 64 cells or 2048
- * particles are left as they are.  Batched stepping (pk_batch_run) does not reorder. */
+ * atomics, and the lanes of a wavefront sit in different cells.  0 turns it off.  Supercells of
+ * more than 64 cells or 4096 particles, or with a cell of more than 128 particles, are left as
+ * they are.  Batched stepping (pk_batch_run) reorders at the same re-sort counts. */
 enum { PK_OPT_PARTICLE_KERNEL = 0, PK_OPT_CHANGE_QUEUE = 1, PK_OPT_INTERLEAVE = 2, PK_OPT_N };
 enum { PK_KERNEL_AUTO = 0, PK_KERNEL_PER_PARTICLE = 1, PK_KERNEL_CELL_SORTED = 2 };
 int pk_ctx_set_option(pk_ctx* ctx, int option, int value);

@@ -631,8 +631,10 @@ int pk_ens_alloc(pk_ens* e, size_t capacity)
 }
 
 // ---- interleave (FP32 split ensembles) ----
-constexpr int IL_P = 8;          // particles per thread: supercells of up to 2048 particles
-constexpr int IL_RMAX = 64;      // most particles of one cell that are interleaved
+constexpr int IL_RMAX = 128;     // most particles of one cell that are interleaved
+// particles held per thread (IL_P = 8 or 16: supercells of up to 2048 or 4096 particles), chosen
+// from the ensemble's largest supercell capacity
+inline int il_p(int maxcap) { return maxcap <= 256 * 8 ? 8 : 16; }
 
 // One workgroup reorders supercell s in place.  With r the rank of a particle among those of its
 // cell (in storage order) and h the cell's count, the order is: first the pairs (ranks 2q and
@@ -646,6 +648,7 @@ constexpr int IL_RMAX = 64;      // most particles of one cell that are interlea
 // the same cell.  The per-pair-row cell masks M and their prefix counts come from wavefront 0
 // with ballots and shuffles.  The eight 32-bit fields then pass one at a time through an LDS
 // stage (scattered LDS writes, coalesced global loads and stores), two fields ahead in flight.
+template <int IL_P>
 __device__ __forceinline__ void interleave_sc(const EnsGeom& eg, const PartSplit& pm, const int64_t* __restrict__ off,
                                               const int32_t* __restrict__ cnt, long s)
 {
@@ -785,12 +788,13 @@ __device__ __forceinline__ void interleave_sc(const EnsGeom& eg, const PartSplit
     }
 }
 
+template <int IL_P>
 __global__ void __launch_bounds__(256) k_interleave_split(EnsGeom eg, PartSplit pm, const int64_t* __restrict__ off,
                                                           const int32_t* __restrict__ cnt)
 {
     const long s = pk_xcd_supercell(blockIdx.x, eg.xcd_chunk);
     if (s >= eg.nsct) return;
-    interleave_sc(eg, pm, off, cnt, s);
+    interleave_sc<IL_P>(eg, pm, off, cnt, s);
 }
 
 struct InterleaveArgs {
@@ -802,6 +806,7 @@ struct InterleaveArgs {
 
 // batched form, last kernel of a batch step: ensemble j is reordered when its re-sort count
 // base[j] + (steps done in this pass) is a multiple of period; nothing after a halt
+template <int IL_P>
 __global__ void __launch_bounds__(256) k_interleave_split_batch(const InterleaveArgs* __restrict__ jobs,
                                                                 const int2* __restrict__ items, long nitems,
                                                                 const int* __restrict__ halt,
@@ -813,7 +818,7 @@ __global__ void __launch_bounds__(256) k_interleave_split_batch(const Interleave
         const int2 it = items[w];
         if ((base[it.x] + halt[PK_HALT_DONE]) % period != 0) continue;
         const InterleaveArgs& a = jobs[it.x];
-        interleave_sc(a.eg, a.pm, a.off, a.cnt, it.y);
+        interleave_sc<IL_P>(a.eg, a.pm, a.off, a.cnt, it.y);
     }
 }
 
@@ -822,8 +827,12 @@ int pk_ens_interleave_launch(pk_ens* e)
     if (e->posmode != PK_POS_SPLIT || e->np == 0) return PK_OK;
     EnsGeom eg = ens_geom(e);
     pk_ktimer tm(e->ctx, "interleave");
-    hipLaunchKernelGGL(k_interleave_split, dim3((unsigned)(PK_XCD * (long)eg.xcd_chunk)), dim3(256), 0, e->ctx->stream,
-                       eg, part_split(e, e->main), e->off, e->cnt);
+    if (il_p(e->maxcap) == 8)
+        hipLaunchKernelGGL(k_interleave_split<8>, dim3((unsigned)(PK_XCD * (long)eg.xcd_chunk)), dim3(256), 0,
+                           e->ctx->stream, eg, part_split(e, e->main), e->off, e->cnt);
+    else
+        hipLaunchKernelGGL(k_interleave_split<16>, dim3((unsigned)(PK_XCD * (long)eg.xcd_chunk)), dim3(256), 0,
+                           e->ctx->stream, eg, part_split(e, e->main), e->off, e->cnt);
     PK_HIP(e->ctx, hipGetLastError());
     return PK_OK;
 }
@@ -854,8 +863,14 @@ int pk_batch_interleave_prepare(pk_batch* b)
 int pk_batch_interleave_launch(pk_batch* b)
 {
     if (b->ilperiod <= 0 || b->nitems == 0) return PK_OK;
-    hipLaunchKernelGGL(k_interleave_split_batch, dim3((unsigned)b->nitems), dim3(256), 0, b->ctx->stream,
-                       (const InterleaveArgs*)b->iljobs, b->items, b->nitems, b->halt, b->ilbase, b->ilperiod);
+    int maxcap = 0;
+    for (pk_ens* e : b->ens) maxcap = e->maxcap > maxcap ? e->maxcap : maxcap;
+    if (il_p(maxcap) == 8)
+        hipLaunchKernelGGL(k_interleave_split_batch<8>, dim3((unsigned)b->nitems), dim3(256), 0, b->ctx->stream,
+                           (const InterleaveArgs*)b->iljobs, b->items, b->nitems, b->halt, b->ilbase, b->ilperiod);
+    else
+        hipLaunchKernelGGL(k_interleave_split_batch<16>, dim3((unsigned)b->nitems), dim3(256), 0, b->ctx->stream,
+                           (const InterleaveArgs*)b->iljobs, b->items, b->nitems, b->halt, b->ilbase, b->ilperiod);
     PK_HIP(b->ctx, hipGetLastError());
     return PK_OK;
 }

@@ -155,11 +155,13 @@ def _interleave_order(cells, off, ncell, ncps):
     return perm
 
 
-def test_interleave_order_and_set(ctx):
+@pytest.mark.parametrize("ppc", [6, 48])
+def test_interleave_order_and_set(ctx, ppc):
     """pk_ens_interleave keeps every particle (bits included) and every supercell's membership,
-    and orders each supercell as pairs of one cell by (pair rank, cell), odd leftovers last"""
-    _say("interleave")
-    ncell, ncps, ppc = (8, 12, 8), (4, 4, 4), 6
+    and orders each supercell as pairs of one cell by (pair rank, cell), odd leftovers last;
+    48 ppc puts ~3072 particles in a supercell (the kernel form for up to 4096)"""
+    _say("interleave ppc=%d" % ppc)
+    ncell, ncps = (8, 12, 8), (4, 4, 4)
     dt, x, u, w = _inputs(88, ncell, ppc)
     ens = P.EnsembleSupercells(ctx, ncell, (1.0, 1.0, 1.0), ncps, prec=P.F32)
     g = P.YeeGrid.periodic(ctx, ncell, prec=P.F32)
