@@ -632,9 +632,9 @@ int pk_ens_alloc(pk_ens* e, size_t capacity)
 
 // ---- interleave (FP32 split ensembles) ----
 constexpr int IL_RMAX = 128;     // most particles of one cell that are interleaved
-// particles held per thread (IL_P = 8 or 16: supercells of up to 2048 or 4096 particles), chosen
-// from the ensemble's largest supercell capacity
-inline int il_p(int maxcap) { return maxcap <= 256 * 8 ? 8 : 16; }
+// particles held per thread: IL_P = 8 for supercells of up to 2048 particles, 16 for 2049 to 4096
+// (launched only when the largest supercell capacity exceeds 2048)
+inline bool il_big(int maxcap) { return maxcap > 256 * 8; }
 
 // One workgroup reorders supercell s in place.  With r the rank of a particle among those of its
 // cell (in storage order) and h the cell's count, the order is: first the pairs (ranks 2q and
@@ -654,7 +654,8 @@ __device__ __forceinline__ void interleave_sc(const EnsGeom& eg, const PartSplit
 {
     const int n = cnt[s];
     const int ncl = eg.ncps[0] * eg.ncps[1] * eg.ncps[2];
-    if (n < 2 || n > 256 * IL_P || ncl > 64) return;
+    // each size class of supercells has its own form: 16 per thread only above 256 * 8 particles
+    if (n < 2 || n > 256 * IL_P || (IL_P > 8 && n <= 256 * 8) || ncl > 64) return;
     __shared__ unsigned long long cm[64][4];
     __shared__ int hist[64];
     __shared__ unsigned long long M[IL_RMAX / 2];
@@ -827,10 +828,9 @@ int pk_ens_interleave_launch(pk_ens* e)
     if (e->posmode != PK_POS_SPLIT || e->np == 0) return PK_OK;
     EnsGeom eg = ens_geom(e);
     pk_ktimer tm(e->ctx, "interleave");
-    if (il_p(e->maxcap) == 8)
-        hipLaunchKernelGGL(k_interleave_split<8>, dim3((unsigned)(PK_XCD * (long)eg.xcd_chunk)), dim3(256), 0,
-                           e->ctx->stream, eg, part_split(e, e->main), e->off, e->cnt);
-    else
+    hipLaunchKernelGGL(k_interleave_split<8>, dim3((unsigned)(PK_XCD * (long)eg.xcd_chunk)), dim3(256), 0,
+                       e->ctx->stream, eg, part_split(e, e->main), e->off, e->cnt);
+    if (il_big(e->maxcap))
         hipLaunchKernelGGL(k_interleave_split<16>, dim3((unsigned)(PK_XCD * (long)eg.xcd_chunk)), dim3(256), 0,
                            e->ctx->stream, eg, part_split(e, e->main), e->off, e->cnt);
     PK_HIP(e->ctx, hipGetLastError());
@@ -865,10 +865,9 @@ int pk_batch_interleave_launch(pk_batch* b)
     if (b->ilperiod <= 0 || b->nitems == 0) return PK_OK;
     int maxcap = 0;
     for (pk_ens* e : b->ens) maxcap = e->maxcap > maxcap ? e->maxcap : maxcap;
-    if (il_p(maxcap) == 8)
-        hipLaunchKernelGGL(k_interleave_split_batch<8>, dim3((unsigned)b->nitems), dim3(256), 0, b->ctx->stream,
-                           (const InterleaveArgs*)b->iljobs, b->items, b->nitems, b->halt, b->ilbase, b->ilperiod);
-    else
+    hipLaunchKernelGGL(k_interleave_split_batch<8>, dim3((unsigned)b->nitems), dim3(256), 0, b->ctx->stream,
+                       (const InterleaveArgs*)b->iljobs, b->items, b->nitems, b->halt, b->ilbase, b->ilperiod);
+    if (il_big(maxcap))
         hipLaunchKernelGGL(k_interleave_split_batch<16>, dim3((unsigned)b->nitems), dim3(256), 0, b->ctx->stream,
                            (const InterleaveArgs*)b->iljobs, b->items, b->nitems, b->halt, b->ilbase, b->ilperiod);
     PK_HIP(b->ctx, hipGetLastError());
