@@ -162,17 +162,23 @@ def test_batch_autotune_restores_state(ctx):
     b.close()
 
 
-@pytest.mark.parametrize("period,graph,uth,slack", [(2, 1, 0.05, None), (3, 0, 0.05, None), (1, 1, 0.2, (0.0, 0))])
-def test_batch_interleave_matches_one_by_one(ctx, period, graph, uth, slack):
+# a mix of supercell sizes: 40 ppc puts ~2560 particles in a 4^3 supercell (the interleave's
+# form for 2049-4096 particles runs beside the one for up to 2048)
+MIXED = [((8, 8, 8), 40, 21), ((8, 8, 12), 8, 11)]
+
+
+@pytest.mark.parametrize("period,graph,uth,slack,specs", [(2, 1, 0.05, None, SPECS), (3, 0, 0.05, None, SPECS),
+                                                          (1, 1, 0.2, (0.0, 0), SPECS), (2, 1, 0.05, None, MIXED)])
+def test_batch_interleave_matches_one_by_one(ctx, period, graph, uth, slack, specs):
     """with the periodic interleave on, the batch reorders each FP32 ensemble after the same
     re-sorts as one-by-one stepping -- counted per ensemble (one ensemble starts a step ahead),
-    and also after an overflow rebuild (no slack, hot plasma) -- and ends in the same state,
-    particle order included"""
-    _say("batch interleave period=%d graph=%d uth=%g" % (period, graph, uth))
+    and also after an overflow rebuild (no slack, hot plasma), and with supercells of both size
+    classes -- and ends in the same state, particle order included"""
+    _say("batch interleave period=%d graph=%d uth=%g nspecs=%d" % (period, graph, uth, len(specs)))
     ctx.set_option(P.OPT_INTERLEAVE, period)
     dt = courant_dt()
-    mine = _pairs(ctx, P.F32, uth=uth, slack=slack)
-    ref = _pairs(ctx, P.F32, uth=uth, slack=slack)
+    mine = _pairs(ctx, P.F32, specs, uth=uth, slack=slack)
+    ref = _pairs(ctx, P.F32, specs, uth=uth, slack=slack)
     for e, g in (mine[0], ref[0]):
         P.step(e, g, dt, P.TSC, P.ESIRKEPOV)
     b = P.Batch(ctx, mine)
