@@ -37,8 +37,26 @@ def test_no_fallback_without_the_native_library(tmp_path):
 
 
 def test_cxx_surface_compiles():
-    """include/pica_amd/pica.hpp (pica's names over the C ABI) compiles on its own"""
-    src = "#include \"pica_amd/pica.hpp\"\nint main() { return 0; }\n"
+    """include/pica_amd/pica.hpp (pica's names over the C ABI) compiles on its own, with every
+    member and free function of the FP64 and FP32 forms instantiated"""
+    src = """#include "pica_amd/pica.hpp"
+using namespace pica_amd;
+template class pica_amd::YeeGrid<double>;
+template class pica_amd::YeeGrid<float>;
+template class pica_amd::EnsembleSupercells<double>;
+template class pica_amd::EnsembleSupercells<float>;
+template <class R> static void all()
+{
+    void (*push)(EnsembleSupercells<R>&, YeeGrid<R>&, double, FormFactor, Depositor) = &pushParticles<R>;
+    void (*one)(EnsembleSupercells<R>&, YeeGrid<R>&, double, FormFactor, Depositor, int) = &step<R>;
+    void (*many)(std::vector<EnsembleSupercells<R>*>&, YeeGrid<R>&, double, FormFactor, Depositor) = &step<R>;
+    Diagnostics (*diag)(EnsembleSupercells<R>&, YeeGrid<R>&, FormFactor) = &diagnostics<R>;
+    void (YeeSolver::*ue)(YeeGrid<R>&, R) = &YeeSolver::updateE<R>;
+    void (YeeSolver::*ub)(YeeGrid<R>&, R) = &YeeSolver::updateB<R>;
+    (void)push; (void)one; (void)many; (void)diag; (void)ue; (void)ub;
+}
+int main() { all<double>(); all<float>(); return 0; }
+"""
     r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), "-x", "c++", "-"],
                        input=src, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
