@@ -102,6 +102,14 @@ def test_split_one_and_two_supercells_per_dimension(ctx, kernel):
     _compare(ctx, P.TSC, P.ESIRKEPOV, (8, 4, 12), (4, 4, 4), 8, 10, 5, uth=0.1, kernel=kernel)
 
 
+@pytest.mark.parametrize("ncps,ppc", [((2, 2, 2), 12), ((8, 8, 8), 6), ((4, 2, 8), 9)])
+def test_split_supercell_shapes(ctx, ncps, ppc):
+    """the two-particles-per-lane kernel with cubic supercells of 2 and 8 cells (their own
+    compiled forms) and a non-cubic one (the general form), odd ppc included"""
+    _say("split supercells %s ppc=%d" % (ncps, ppc))
+    _compare(ctx, P.TSC, P.ESIRKEPOV, (16, 8, 16), ncps, ppc, 10, 31 + ppc, kernel=P.KERNEL_PER_PARTICLE)
+
+
 @pytest.mark.parametrize("shape", [P.CIC, P.TSC])
 def test_split_sorted_small_supercells(ctx, shape):
     """the cell-sorted kernel on 2^3 supercells"""
