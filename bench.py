@@ -21,7 +21,7 @@ batched: 64 independent periodic plasmas with edges drawn from {32,48,64,96,128}
 {16,24,32}, ppc {4,8}), each stepped one by one (pk_step per plasma) and as one batch
 (pk_batch_run after pk_batch_autotune).
 learned: the current denoiser trained on 32^3 pairs of 4- and 256-ppc runs (48 runs x 10 snapshots,
-8,000 Adam steps), evaluated at 128^3
+32,000 Adam steps), evaluated at 128^3
 against a 256-ppc reference: J error of 4 ppc raw, denoised, and 64 ppc, and the cost of a 64-ppc
 step over a 4-ppc denoised step.
 """

@@ -202,6 +202,10 @@ int pk_dn_forward_host(pk_dn* d, const int n[3], const float* x, float* y);
 int pk_dn_layer_host(pk_dn* d, int layer, const int n[3], const float* in, float* out);
 /* gradient of the mean squared error of forward(x) against y; grad has pk_dn_num_params() values */
 int pk_dn_grad_host(pk_dn* d, const int n[3], const float* x, const float* y, float* grad, double* loss);
+/* the same over nsamples volumes of size n at once: x, y are [nsamples][3][n], the loss is the mean
+   over all of them (the mini-batch form that pk_dn_train runs) */
+int pk_dn_grad_batch_host(pk_dn* d, const int n[3], int nsamples, const float* x, const float* y, float* grad,
+                          double* loss);
 /* training pairs: the physical J of two grids of equal size (low-ppc input, high-ppc target) */
 int pk_dn_add_pair(pk_dn* d, pk_grid* low, pk_grid* high);
 int pk_dn_num_pairs(pk_dn* d);

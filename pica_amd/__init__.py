@@ -53,8 +53,8 @@ ABI_SYMBOLS = (
     "pk_deposit_rho", "pk_diag", "pk_diag_set_reference", "pk_batch_create", "pk_batch_destroy", "pk_batch_set",
     "pk_batch_get", "pk_batch_run", "pk_batch_autotune", "pk_ens_load_modes", "pk_dn_create", "pk_dn_destroy",
     "pk_dn_num_params", "pk_dn_init", "pk_dn_set_params", "pk_dn_get_params", "pk_dn_set_scale", "pk_dn_forward_host",
-    "pk_dn_layer_host", "pk_dn_grad_host", "pk_dn_add_pair", "pk_dn_num_pairs", "pk_dn_clear_pairs", "pk_dn_train",
-    "pk_dn_apply", "pk_rho_save", "pk_project_continuity", "pk_step_denoised", "pk_grid_save", "pk_grid_load",
+    "pk_dn_layer_host", "pk_dn_grad_host", "pk_dn_grad_batch_host", "pk_dn_add_pair", "pk_dn_num_pairs",
+    "pk_dn_clear_pairs", "pk_dn_train", "pk_dn_apply", "pk_rho_save", "pk_project_continuity", "pk_step_denoised", "pk_grid_save", "pk_grid_load",
     "pk_ens_save", "pk_ens_load", "pk_step_species", "pk_ens_interleave",
 )
 
@@ -133,6 +133,7 @@ def lib():
         L.pk_dn_forward_host.argtypes = [vp, vp, vp, vp]
         L.pk_dn_layer_host.argtypes = [vp, ctypes.c_int, vp, vp, vp]
         L.pk_dn_grad_host.argtypes = [vp, vp, vp, vp, vp, ctypes.POINTER(ctypes.c_double)]
+        L.pk_dn_grad_batch_host.argtypes = [vp, vp, ctypes.c_int, vp, vp, vp, ctypes.POINTER(ctypes.c_double)]
         L.pk_dn_add_pair.argtypes = [vp, vp, vp]
         L.pk_dn_num_pairs.argtypes = [vp]
         L.pk_dn_clear_pairs.argtypes = [vp]
@@ -502,6 +503,16 @@ class Denoiser:
         g = np.empty(self.num_params(), np.float32)
         loss = ctypes.c_double()
         self.ctx.check(lib().pk_dn_grad_host(self.h, _i3(x.shape[1:]), _p(x), _p(y), _p(g), ctypes.byref(loss)))
+        return loss.value, g
+
+    def grad_batch(self, x, y):
+        """mean loss and gradient over x.shape[0] volumes; x, y are [samples][3][n0][n1][n2]"""
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        g = np.empty(self.num_params(), np.float32)
+        loss = ctypes.c_double()
+        self.ctx.check(lib().pk_dn_grad_batch_host(self.h, _i3(x.shape[2:]), int(x.shape[0]), _p(x), _p(y), _p(g),
+                                                   ctypes.byref(loss)))
         return loss.value, g
 
     def add_pair(self, low, high):

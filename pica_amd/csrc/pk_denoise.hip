@@ -51,15 +51,20 @@ struct pk_dn {
     long cap = 0;
     uint16_t *xb = nullptr, *a1 = nullptr, *a2 = nullptr, *g3 = nullptr, *dz2 = nullptr, *dz1 = nullptr;
     float *out = nullptr, *tgt = nullptr;
+    float* part = nullptr;    // per-workgroup weight-gradient tiles (k_wgrad)
     double* loss = nullptr;   // device accumulator
     std::vector<pk_dn_pair> pairs;
 };
 
 namespace {
 
+// ns volumes of n0 x n1 x n2 voxels each, stacked in memory (sample s holds voxels
+// [s nv, (s + 1) nv)); each is periodic on its own.  Training stacks its batch this way, so one
+// launch per layer covers every sample.
 struct Vol {
     int n0, n1, n2;
-    long nv;
+    long nv;         // voxels per sample
+    int ns = 1;      // samples
 };
 
 inline unsigned nblk(long n, int b) { return (unsigned)((n + b - 1) / b); }
@@ -127,8 +132,14 @@ __global__ void __launch_bounds__(256) k_conv(Vol V, const uint16_t* __restrict_
     // XCD-aware: the workgroups of one XCD (blockIdx % 8) take one contiguous run of bricks, so
     // the halos that neighbouring bricks share stay in that XCD's L2
     const long nbrick = (long)((V.n0 + BX - 1) / BX) * nby * nbz, chunk = gridDim.x / PK_XCD;
-    const long b = (blockIdx.x % PK_XCD) * chunk + blockIdx.x / PK_XCD;
-    if (b >= nbrick) return;
+    long b = (blockIdx.x % PK_XCD) * chunk + blockIdx.x / PK_XCD;
+    if (b >= nbrick * V.ns) return;
+    const long sample = b / nbrick;
+    b -= sample * nbrick;
+    in += sample * V.nv * CIN;
+    if (out_b) out_b += sample * V.nv * 16;
+    if (out_f) out_f += sample * 3 * V.nv;
+    if (mask_act) mask_act += sample * V.nv * 16;
     const int x0 = (int)(b / ((long)nby * nbz)) * BX, y0 = (int)((b / nbz) % nby) * BY, z0 = (int)(b % nbz) * BZ;
     // all of a thread's halo loads are issued before the first LDS write (one wait, not one per load)
     typedef typename std::conditional<CIN == 16, bf16x8, bf16x4>::type unit_t;
@@ -219,49 +230,132 @@ __global__ void __launch_bounds__(256) k_conv(Vol V, const uint16_t* __restrict_
     }
 }
 
-// dW[t][ci][co] += sum_v in[v + off(t)][ci] g[v][co].  Rows of the 16 x 16 tile: (tap, ci) --
-// one tap with CIN = 16, four taps x 4 channels with CIN = 4; columns: co.  Waves split voxels.
+// dW[t][ci][co] = sum_v in[v + off(t)][ci] g[v][co], as a GEMM per 16 x 16 tile (rows: one tap x
+// 16 channels with CIN = 16, four taps x 4 channels with CIN = 4; columns: co) with K = voxels.  A
+// workgroup takes bricks of BX x BY x BZ voxels with a grid stride; each brick's input with its
+// periodic one-voxel halo and its output gradient are staged in LDS as in k_conv, and each
+// wavefront accumulates its tiles over the brick's 256 voxels on v_mfma_f32_16x16x32_bf16
+// (K-block = 32 voxels: two 16-voxel z-rows; lane (m, q) holds voxels 8q .. 8q + 7 of the block for
+// row / column m).  The workgroup's tiles go to part[blockIdx.x]; k_wgrad_sum adds them into dW.
+constexpr int WG_TILES16 = 27, WG_TILES4 = 7, WGRAD_WG_MAX = 512;
+
 template <int CIN>
 __global__ void __launch_bounds__(256) k_wgrad(Vol V, const uint16_t* __restrict__ in, const uint16_t* __restrict__ g,
-                                               int gpad, int cin, int cout, long chunk, float* __restrict__ dW)
+                                               int gpad, int cout, float* __restrict__ part)
 {
-    const int lane = threadIdx.x & 63;
+    constexpr int NT = CIN == 16 ? WG_TILES16 : WG_TILES4, TPW = (NT + 3) / 4;
+    constexpr int UNITS = CIN == 16 ? 2 : 1, NVOX = BX * BY * BZ;
+    static_assert(NVOX == 256 && BZ == 16, "one brick voxel per thread, K-blocks of two z-rows");
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint16_t* halo = (uint16_t*)smem;                     // [HX][HY][HZ][CIN]
+    uint16_t* gt = halo + HX * HY * HZ * CIN;             // [NVOX][16], zero outside the volume and for co >= cout
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int m = lane & 15, q = lane >> 4;
-    const int group = blockIdx.y;                         // tap (CIN 16) or tap quad (CIN 4)
-    const int tap_m = CIN == 16 ? group : 4 * group + m / 4;
-    const int ci_m = CIN == 16 ? m : m % 4;
-    const bool row_ok = tap_m < 27 && ci_m < cin;
-    const long v_begin = ((long)blockIdx.x * 4 + (threadIdx.x >> 6)) * chunk;
-    const long v_end = v_begin + chunk < V.nv ? v_begin + chunk : V.nv;
-    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-    for (long v0 = v_begin; v0 < v_end; v0 += 16) {
-        bf16x4 a, b;
+    const int nbz = (V.n2 + BZ - 1) / BZ, nby = (V.n1 + BY - 1) / BY;
+    const long nbrick = (long)((V.n0 + BX - 1) / BX) * nby * nbz, total = nbrick * V.ns;
+    typedef typename std::conditional<CIN == 16, bf16x8, bf16x4>::type unit_t;
+    constexpr int NUNIT = HX * HY * HZ * UNITS, PER = (NUNIT + 255) / 256;
+    f32x4 acc[TPW];
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const long v = v0 + 4 * q + j;
-            short av = 0, bv = 0;
-            if (v < v_end) {
-                if (row_ok) {
-                    int x, y, z;
-                    voxel_xyz(V, v, x, y, z);
-                    av = (short)in[neighbour(V, x, y, z, tap_m) * CIN + ci_m];
-                }
-                if (m < cout) bv = (short)g[v * gpad + m];
+    for (int i = 0; i < TPW; i++) acc[i] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (long bb = blockIdx.x; bb < total; bb += gridDim.x) {
+        const long sample = bb / nbrick, b = bb - sample * nbrick;
+        const int x0 = (int)(b / ((long)nby * nbz)) * BX, y0 = (int)((b / nbz) % nby) * BY, z0 = (int)(b % nbz) * BZ;
+        const uint16_t* ins = in + sample * V.nv * CIN;
+        unit_t buf[PER];
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int u = threadIdx.x + 256 * k;
+            if (u < NUNIT) {
+                const int h = u / UNITS, unit = u - h * UNITS;
+                const int hz = h % HZ, hy = (h / HZ) % HY, hx = h / (HZ * HY);
+                const long gv = ((long)wrap(x0 - 1 + hx, V.n0) * V.n1 + wrap(y0 - 1 + hy, V.n1)) * V.n2 +
+                                wrap(z0 - 1 + hz, V.n2);
+                buf[k] = *(const unit_t*)(ins + gv * CIN + (CIN / UNITS) * unit);
             }
-            a[j] = av;
-            b[j] = bv;
         }
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
-    }
-    const int co = m;
-    if (co >= cout) return;
+        bf16x8 glo = bf16x8{0, 0, 0, 0, 0, 0, 0, 0}, ghi = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+        {
+            const int t = threadIdx.x;
+            const int gx = x0 + t / (BY * BZ), gy = y0 + (t / BZ) % BY, gz = z0 + t % BZ;
+            if (gx < V.n0 && gy < V.n1 && gz < V.n2) {
+                const uint16_t* gp = g + (sample * V.nv + ((long)gx * V.n1 + gy) * V.n2 + gz) * gpad;
+                if (gpad == 16) {
+                    glo = *(const bf16x8*)gp;
+                    ghi = *(const bf16x8*)(gp + 8);
+                } else {
+                    const bf16x4 v4 = *(const bf16x4*)gp;
+                    glo = bf16x8{v4[0], v4[1], v4[2], v4[3], 0, 0, 0, 0};
+                }
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-        const int row = 4 * q + j;
-        const int tap = CIN == 16 ? group : 4 * group + row / 4;
-        const int ci = CIN == 16 ? row : row % 4;
-        if (tap < 27 && ci < cin && acc[j] != 0.0f) atomicAdd(&dW[((long)tap * cin + ci) * cout + co], acc[j]);
+                for (int c = 0; c < 8; c++) {
+                    if (c >= cout) glo[c] = 0;
+                    if (c + 8 >= cout) ghi[c] = 0;
+                }
+            }
+        }
+        __syncthreads();   // the previous brick's LDS reads are done
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int u = threadIdx.x + 256 * k;
+            if (u < NUNIT) *(unit_t*)(halo + u * (CIN / UNITS)) = buf[k];
+        }
+        *(bf16x8*)(gt + threadIdx.x * 16) = glo;
+        *(bf16x8*)(gt + threadIdx.x * 16 + 8) = ghi;
+        __syncthreads();
+#pragma unroll 2
+        for (int kb = 0; kb < NVOX / 32; kb++) {
+            const int row = 2 * kb + (q >> 1), tx = row / BY, ty = row % BY, tz = 8 * (q & 1);
+            bf16x8 B;
+#pragma unroll
+            for (int j = 0; j < 8; j++) B[j] = (short)gt[(row * BZ + tz + j) * 16 + m];
+#pragma unroll
+            for (int i = 0; i < TPW; i++) {
+                const int tile = wave + 4 * i;
+                if (tile >= NT) break;
+                const int tap = CIN == 16 ? tile : 4 * tile + m / 4, ci = CIN == 16 ? m : m % 4;
+                bf16x8 A = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+                if (tap < 27) {
+                    const int h0 = ((tx + tap / 9) * HY + (ty + (tap / 3) % 3)) * HZ + tz + tap % 3;
+#pragma unroll
+                    for (int j = 0; j < 8; j++) A[j] = (short)halo[(h0 + j) * CIN + ci];
+                }
+                acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16v8, A),
+                                                                 __builtin_bit_cast(bf16v8, B), acc[i], 0, 0, 0);
+            }
+        }
     }
+    // D[4q + j][m]: row 4q + j of the tile, column co = m
+    float* pp = part + (long)blockIdx.x * NT * 256;
+#pragma unroll
+    for (int i = 0; i < TPW; i++) {
+        const int tile = wave + 4 * i;
+        if (tile < NT)
+#pragma unroll
+            for (int j = 0; j < 4; j++) pp[tile * 256 + (4 * q + j) * 16 + m] = acc[i][j];
+    }
+}
+
+// dW[t][ci][co] += sum over the nwg workgroups' tiles; blockIdx.y takes every gridDim.y-th group of
+// 4 workgroups, so a weight receives gridDim.y atomic adds
+template <int CIN>
+__global__ void __launch_bounds__(256) k_wgrad_sum(int nwg, const float* __restrict__ part, int cin, int cout,
+                                                   float* __restrict__ dW)
+{
+    constexpr int NT = CIN == 16 ? WG_TILES16 : WG_TILES4;
+    const int lane = threadIdx.x & 63, p = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + lane;
+    float s = 0.0f;
+    if (i < NT * 256)
+        for (int w = 4 * blockIdx.y + p; w < nwg; w += 4 * gridDim.y) s += part[(long)w * NT * 256 + i];
+    __shared__ float red[4][64];
+    red[p][lane] = s;
+    __syncthreads();
+    if (p != 0 || i >= NT * 256) return;
+    s = red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane];
+    const int tile = i / 256, row = (i / 16) % 16, co = i % 16;
+    const int tap = CIN == 16 ? tile : 4 * tile + row / 4, ci = CIN == 16 ? row : row % 4;
+    if (tap < 27 && ci < cin && co < cout && s != 0.0f) atomicAdd(&dW[((long)tap * cin + ci) * cout + co], s);
 }
 
 // db[co] += sum_v g[v][co]
@@ -293,7 +387,7 @@ __global__ void __launch_bounds__(256) k_bgrad(long nv, const uint16_t* __restri
 // weights -> MFMA B fragments of v_mfma_f32_16x16x32_bf16, wf[kc][lane][8].  Effective layer:
 // cin_e x cout_e with W_e(t, a, b) = W(t, a, b), or for the input gradient (flip) W(26 - t, b, a);
 // cin_pad 16: chunk kc holds taps 2kc, 2kc + 1 x 16 channels; cin_pad 4: taps 8kc .. 8kc + 7 x 4.
-__global__ void k_pack(const float* __restrict__ W, int cin, int cout, int flip, int cin_pad, uint16_t* __restrict__ wf)
+__device__ void pack_one(const float* __restrict__ W, int cin, int cout, int flip, int cin_pad, uint16_t* __restrict__ wf)
 {
     const int kcs = cin_pad == 16 ? 14 : 4;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -313,6 +407,39 @@ __global__ void k_pack(const float* __restrict__ W, int cin, int cout, int flip,
     if (tap < 27 && a < cin_e && n < cout_e)
         w = flip ? W[((long)(26 - tap) * cin + n) * cout + a] : W[((long)tap * cin + a) * cout + n];
     wf[i] = to_bf16(w);
+}
+
+// the five packed layouts (blockIdx.y): forward W1, W2, W3; flipped-transposed W2, W3
+struct PackDst {
+    uint16_t *wf0, *wf1, *wf2, *wt0, *wt1;
+};
+
+__global__ void k_pack(const float* __restrict__ p, PackDst o)
+{
+    switch (blockIdx.y) {
+    case 0: pack_one(p + OFF_W1, 3, HID, 0, 4, o.wf0); break;
+    case 1: pack_one(p + OFF_W2, HID, HID, 0, 16, o.wf1); break;
+    case 2: pack_one(p + OFF_W3, HID, 3, 0, 16, o.wf2); break;
+    case 3: pack_one(p + OFF_W2, HID, HID, 1, 16, o.wt0); break;
+    default: pack_one(p + OFF_W3, HID, 3, 1, 4, o.wt1); break;
+    }
+}
+
+// a training mini-batch: sample k is pair sel[k]; its input goes to xb (BF16, channel-last) and
+// its target to tgt ([k][3][nv] FP32)
+__global__ void k_gather(long nv, const float* const* __restrict__ lows, const float* const* __restrict__ highs,
+                         const int* __restrict__ sel, uint16_t* __restrict__ xb, float* __restrict__ tgt)
+{
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    const int k = blockIdx.y, pr = sel[k];
+    const float *lo = lows[pr], *hi = highs[pr];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        xb[(k * nv + v) * 4 + c] = to_bf16(lo[(long)c * nv + v]);
+        tgt[(3l * k + c) * nv + v] = hi[(long)c * nv + v];
+    }
+    xb[(k * nv + v) * 4 + 3] = 0;
 }
 
 // network input: x_b[v][0..3] = bf16(scale * src[c][v]) (channel 3 zero); src channel-first FP32
@@ -366,22 +493,31 @@ __global__ void k_f32_to_grid(GridView<T> g, Vol V, double inv_scale, const floa
 }
 
 // MSE loss and its gradient: g3[v][c] = bf16(2 (out - y) / count), loss += sum (out - y)^2 / count
-__global__ void __launch_bounds__(256) k_loss(long nv, const float* __restrict__ out, const float* __restrict__ y,
-                                              double inv_count, uint16_t* __restrict__ g3, double* __restrict__ loss)
+__global__ void __launch_bounds__(256) k_loss(long nv, int ns, const float* __restrict__ out,
+                                              const float* __restrict__ y, double inv_count, uint16_t* __restrict__ g3,
+                                              double* __restrict__ loss)
 {
-    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;   // stacked voxel; out, y are [ns][3][nv]
     double s = 0.0;
-    if (v < nv) {
+    if (v < nv * ns) {
+        const long sample = v / nv, base = 2 * sample * nv + v;   // (3 sample + c) nv + (v - sample nv)
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            const float d = out[(long)c * nv + v] - y[(long)c * nv + v];
+            const float d = out[base + (long)c * nv] - y[base + (long)c * nv];
             g3[v * 4 + c] = to_bf16((float)(2.0 * inv_count) * d);
             s += (double)d * d;
         }
         g3[v * 4 + 3] = 0;
     }
+    // one atomic per workgroup: every wavefront's sum to the same address would serialise
+    __shared__ double part[4];
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    if ((threadIdx.x & 63) == 0 && s != 0.0) atomicAdd(loss, s * inv_count);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        s = part[0] + part[1] + part[2] + part[3];
+        if (s != 0.0) atomicAdd(loss, s * inv_count);
+    }
 }
 
 __global__ void k_adam(int n, float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
@@ -439,12 +575,8 @@ int ensure_cap(pk_dn* d, long nv)
 // pack all weight layouts from the FP32 parameters
 int pack_weights(pk_dn* d)
 {
-    hipStream_t s = d->ctx->stream;
-    hipLaunchKernelGGL(k_pack, dim3(nblk(WF4, 256)), dim3(256), 0, s, d->p + OFF_W1, 3, HID, 0, 4, d->wf[0]);
-    hipLaunchKernelGGL(k_pack, dim3(nblk(WF16, 256)), dim3(256), 0, s, d->p + OFF_W2, HID, HID, 0, 16, d->wf[1]);
-    hipLaunchKernelGGL(k_pack, dim3(nblk(WF16, 256)), dim3(256), 0, s, d->p + OFF_W3, HID, 3, 0, 16, d->wf[2]);
-    hipLaunchKernelGGL(k_pack, dim3(nblk(WF16, 256)), dim3(256), 0, s, d->p + OFF_W2, HID, HID, 1, 16, d->wt[0]);
-    hipLaunchKernelGGL(k_pack, dim3(nblk(WF4, 256)), dim3(256), 0, s, d->p + OFF_W3, HID, 3, 1, 4, d->wt[1]);
+    const PackDst o{d->wf[0], d->wf[1], d->wf[2], d->wt[0], d->wt[1]};
+    hipLaunchKernelGGL(k_pack, dim3(nblk(WF16, 256), 5), dim3(256), 0, d->ctx->stream, d->p, o);
     PK_HIP(d->ctx, hipGetLastError());
     return PK_OK;
 }
@@ -452,7 +584,7 @@ int pack_weights(pk_dn* d)
 // launch size: the brick count rounded up to a multiple of the XCD count (k_conv maps blocks to bricks)
 unsigned conv_blocks(const Vol& V)
 {
-    const long nbrick = (long)((V.n0 + BX - 1) / BX) * ((V.n1 + BY - 1) / BY) * ((V.n2 + BZ - 1) / BZ);
+    const long nbrick = (long)((V.n0 + BX - 1) / BX) * ((V.n1 + BY - 1) / BY) * ((V.n2 + BZ - 1) / BZ) * V.ns;
     return (unsigned)(PK_XCD * ((nbrick + PK_XCD - 1) / PK_XCD));
 }
 
@@ -474,15 +606,23 @@ int forward(pk_dn* d, const Vol& V)
     return PK_OK;
 }
 
+constexpr size_t wgrad_lds(int cin) { return sizeof(uint16_t) * (HX * HY * HZ * cin + BX * BY * BZ * 16); }
+
 int wgrad(pk_dn* d, const Vol& V, const uint16_t* in, int cin_pad, int cin, const uint16_t* g, int gpad, int cout,
           float* dW)
 {
-    const long chunk = 256;          // voxels per wavefront (more wavefronts than taps x volume / 4096)
-    const unsigned bx = (unsigned)((V.nv + 4 * chunk - 1) / (4 * chunk));
-    if (cin_pad == 16)
-        hipLaunchKernelGGL(k_wgrad<16>, dim3(bx, 27), dim3(256), 0, d->ctx->stream, V, in, g, gpad, cin, cout, chunk, dW);
-    else
-        hipLaunchKernelGGL(k_wgrad<4>, dim3(bx, 7), dim3(256), 0, d->ctx->stream, V, in, g, gpad, cin, cout, chunk, dW);
+    hipStream_t s = d->ctx->stream;
+    const long total = (long)((V.n0 + BX - 1) / BX) * ((V.n1 + BY - 1) / BY) * ((V.n2 + BZ - 1) / BZ) * V.ns;
+    const int nwg = (int)(total < WGRAD_WG_MAX ? total : WGRAD_WG_MAX);
+    if (cin_pad == 16) {
+        hipLaunchKernelGGL(k_wgrad<16>, dim3(nwg), dim3(256), wgrad_lds(16), s, V, in, g, gpad, cout, d->part);
+        hipLaunchKernelGGL(k_wgrad_sum<16>, dim3(WG_TILES16 * 256 / 64, 8), dim3(256), 0, s, nwg, d->part, cin, cout,
+                           dW);
+    } else {
+        hipLaunchKernelGGL(k_wgrad<4>, dim3(nwg), dim3(256), wgrad_lds(4), s, V, in, g, gpad, cout, d->part);
+        hipLaunchKernelGGL(k_wgrad_sum<4>, dim3(WG_TILES4 * 256 / 64, 8), dim3(256), 0, s, nwg, d->part, cin, cout,
+                           dW);
+    }
     PK_HIP(d->ctx, hipGetLastError());
     return PK_OK;
 }
@@ -492,19 +632,20 @@ int backward(pk_dn* d, const Vol& V, double inv_count, double* loss)
 {
     hipStream_t s = d->ctx->stream;
     int r;
-    hipLaunchKernelGGL(k_loss, dim3(nblk(V.nv, 256)), dim3(256), 0, s, V.nv, d->out, d->tgt, inv_count, d->g3, loss);
-    const unsigned nbb = nblk(V.nv, 1024) < 256 ? nblk(V.nv, 1024) : 256;   // 4+ voxels per thread
+    const long nt = V.nv * V.ns;
+    hipLaunchKernelGGL(k_loss, dim3(nblk(nt, 256)), dim3(256), 0, s, V.nv, V.ns, d->out, d->tgt, inv_count, d->g3, loss);
+    const unsigned nbb = nblk(nt, 1024) < 256 ? nblk(nt, 1024) : 256;   // 4+ voxels per thread
     if ((r = wgrad(d, V, d->a2, 16, HID, d->g3, 4, 3, d->g + OFF_W3))) return r;
-    hipLaunchKernelGGL(k_bgrad, dim3(nbb), dim3(256), 0, s, V.nv, d->g3, 4, 3, d->g + OFF_B3);
+    hipLaunchKernelGGL(k_bgrad, dim3(nbb), dim3(256), 0, s, nt, d->g3, 4, 3, d->g + OFF_B3);
     const unsigned nb = conv_blocks(V);
     hipLaunchKernelGGL((k_conv<4, OUT_MASKED_BF16>), dim3(nb), dim3(256), conv_lds(4), s, V, d->g3, d->wt[1], (const float*)nullptr,
                        HID, d->dz2, (float*)nullptr, d->a2);
     if ((r = wgrad(d, V, d->a1, 16, HID, d->dz2, 16, HID, d->g + OFF_W2))) return r;
-    hipLaunchKernelGGL(k_bgrad, dim3(nbb), dim3(256), 0, s, V.nv, d->dz2, 16, HID, d->g + OFF_B2);
+    hipLaunchKernelGGL(k_bgrad, dim3(nbb), dim3(256), 0, s, nt, d->dz2, 16, HID, d->g + OFF_B2);
     hipLaunchKernelGGL((k_conv<16, OUT_MASKED_BF16>), dim3(nb), dim3(256), conv_lds(16), s, V, d->dz2, d->wt[0],
                        (const float*)nullptr, HID, d->dz1, (float*)nullptr, d->a1);
     if ((r = wgrad(d, V, d->xb, 4, 3, d->dz1, 16, HID, d->g + OFF_W1))) return r;
-    hipLaunchKernelGGL(k_bgrad, dim3(nbb), dim3(256), 0, s, V.nv, d->dz1, 16, HID, d->g + OFF_B1);
+    hipLaunchKernelGGL(k_bgrad, dim3(nbb), dim3(256), 0, s, nt, d->dz1, 16, HID, d->g + OFF_B1);
     PK_HIP(d->ctx, hipGetLastError());
     return PK_OK;
 }
@@ -520,6 +661,14 @@ int check_dims(pk_dn* d, const int n[3], Vol& V)
     V.nv = (long)n[0] * n[1] * n[2];
     if (V.nv >= (1l << 31)) return pk_fail(d->ctx, PK_E_ARG, "denoiser: volume above 2^31 voxels");
     return ensure_cap(d, V.nv);
+}
+
+// V becomes ns stacked samples of its size; the work buffers grow to hold them
+int stack_samples(pk_dn* d, Vol& V, int ns)
+{
+    if ((long)ns * V.nv >= (1l << 31)) return pk_fail(d->ctx, PK_E_ARG, "denoiser: batch above 2^31 voxels");
+    V.ns = ns;
+    return ensure_cap(d, V.nv * ns);
 }
 
 // host [3][nv] FP32 -> d->xb (BF16, channel-last)
@@ -559,6 +708,7 @@ int pk_dn_create(pk_ctx* c, pk_dn** out)
     if (!r) r = dalloc(c, (void**)&d->wt[0], sizeof(uint16_t) * WF16);
     if (!r) r = dalloc(c, (void**)&d->wt[1], sizeof(uint16_t) * WF4);
     if (!r) r = dalloc(c, (void**)&d->loss, sizeof(double));
+    if (!r) r = dalloc(c, (void**)&d->part, sizeof(float) * WGRAD_WG_MAX * WG_TILES16 * 256);
     if (r) {
         pk_dn_destroy(d);
         return r;
@@ -580,7 +730,7 @@ void pk_dn_destroy(pk_dn* d)
     (void)hipStreamSynchronize(d->ctx->stream);
     for (void* p : {(void*)d->p, (void*)d->g, (void*)d->m, (void*)d->v, (void*)d->wf[0], (void*)d->wf[1],
                     (void*)d->wf[2], (void*)d->wt[0], (void*)d->wt[1], (void*)d->xb, (void*)d->a1, (void*)d->a2,
-                    (void*)d->g3, (void*)d->dz2, (void*)d->dz1, (void*)d->out, (void*)d->tgt, (void*)d->loss})
+                    (void*)d->g3, (void*)d->dz2, (void*)d->dz1, (void*)d->out, (void*)d->tgt, (void*)d->loss, (void*)d->part})
         if (p) (void)hipFree(p);
     for (auto& pr : d->pairs) {
         if (pr.low) (void)hipFree(pr.low);
@@ -687,25 +837,36 @@ int pk_dn_layer_host(pk_dn* d, int layer, const int n[3], const float* in, float
     return r;
 }
 
-int pk_dn_grad_host(pk_dn* d, const int n[3], const float* x, const float* y, float* grad, double* loss)
+int pk_dn_grad_batch_host(pk_dn* d, const int n[3], int nsamples, const float* x, const float* y, float* grad,
+                          double* loss)
 {
-    if (!d || !x || !y || !grad) return PK_E_ARG;
+    if (!d || !x || !y || !grad || nsamples < 1) return PK_E_ARG;
     Vol V;
     int r = check_dims(d, n, V);
+    if (!r) r = stack_samples(d, V, nsamples);
     if (r) return r;
     pk_ctx* c = d->ctx;
     hipLaunchKernelGGL(k_zero_f32, dim3(nblk(NPARAM, 256)), dim3(256), 0, c->stream, (long)NPARAM, d->g);
     hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(1), 0, c->stream, d->loss);
-    if ((r = load_input_host(d, V, x))) return r;
+    // inputs pass through d->tgt ([ns][3][nv] FP32) on their way to d->xb
+    PK_HIP(c, hipMemcpyAsync(d->tgt, x, sizeof(float) * 3 * V.nv * V.ns, hipMemcpyHostToDevice, c->stream));
+    for (int k = 0; k < V.ns; k++)
+        hipLaunchKernelGGL(k_input_f32, dim3(nblk(V.nv, 256)), dim3(256), 0, c->stream, V.nv, d->tgt + 3 * V.nv * k,
+                           1.0, d->xb + 4 * V.nv * k);
     if ((r = forward(d, V))) return r;
-    PK_HIP(c, hipMemcpyAsync(d->tgt, y, sizeof(float) * 3 * V.nv, hipMemcpyHostToDevice, c->stream));
-    if ((r = backward(d, V, 1.0 / (3.0 * V.nv), d->loss))) return r;
+    PK_HIP(c, hipMemcpyAsync(d->tgt, y, sizeof(float) * 3 * V.nv * V.ns, hipMemcpyHostToDevice, c->stream));
+    if ((r = backward(d, V, 1.0 / (3.0 * V.nv * V.ns), d->loss))) return r;
     PK_HIP(c, hipMemcpyAsync(grad, d->g, sizeof(float) * NPARAM, hipMemcpyDeviceToHost, c->stream));
     double l = 0.0;
     PK_HIP(c, hipMemcpyAsync(&l, d->loss, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     PK_HIP(c, hipStreamSynchronize(c->stream));
     if (loss) *loss = l;
     return PK_OK;
+}
+
+int pk_dn_grad_host(pk_dn* d, const int n[3], const float* x, const float* y, float* grad, double* loss)
+{
+    return pk_dn_grad_batch_host(d, n, 1, x, y, grad, loss);
 }
 
 int pk_dn_add_pair(pk_dn* d, pk_grid* low, pk_grid* high)
@@ -759,21 +920,46 @@ int pk_dn_train(pk_dn* d, int iters, int batch, double lr, uint64_t seed, float*
 {
     if (!d || iters < 0 || batch < 1 || !(lr > 0.0)) return PK_E_ARG;
     if (d->pairs.empty()) return pk_fail(d->ctx, PK_E_STATE, "pk_dn_train: no training pairs");
+    if ((long)iters * batch >= (1l << 31)) return pk_fail(d->ctx, PK_E_ARG, "pk_dn_train: iters x batch too large");
     pk_ctx* c = d->ctx;
-    int r;
+    const int npairs = (int)d->pairs.size();
+    // every mini-batch's pair indices, and the pairs' buffers, on the device once
+    std::vector<int> hsel((size_t)iters * batch);
+    for (int it = 0; it < iters; it++)
+        for (int b = 0; b < batch; b++)
+            hsel[(size_t)it * batch + b] = (int)(mix64(seed ^ mix64(((uint64_t)it << 20) + b)) % npairs);
+    std::vector<const float*> hptr(2 * (size_t)npairs);
+    for (int k = 0; k < npairs; k++) {
+        hptr[k] = d->pairs[k].low;
+        hptr[npairs + k] = d->pairs[k].high;
+    }
     double* dev_losses = nullptr;
-    if ((r = dalloc(c, (void**)&dev_losses, sizeof(double) * (iters > 0 ? iters : 1)))) return r;
+    int* dsel = nullptr;
+    const float** dptr = nullptr;
+    int r = dalloc(c, (void**)&dev_losses, sizeof(double) * (iters > 0 ? iters : 1));
+    if (!r) r = dalloc(c, (void**)&dsel, sizeof(int) * (hsel.empty() ? 1 : hsel.size()));
+    if (!r) r = dalloc(c, (void**)&dptr, sizeof(float*) * hptr.size());
+    if (!r && (hipMemcpyAsync(dsel, hsel.data(), sizeof(int) * hsel.size(), hipMemcpyHostToDevice, c->stream) !=
+                   hipSuccess ||
+               hipMemcpyAsync(dptr, hptr.data(), sizeof(float*) * hptr.size(), hipMemcpyHostToDevice, c->stream) !=
+                   hipSuccess))
+        r = pk_fail(c, PK_E_HIP, "pk_dn_train: upload failed");
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     for (int it = 0; it < iters && !r; it++) {
         hipLaunchKernelGGL(k_zero_f32, dim3(nblk(NPARAM, 256)), dim3(256), 0, c->stream, (long)NPARAM, d->g);
         hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(1), 0, c->stream, dev_losses + it);
-        for (int b = 0; b < batch && !r; b++) {
-            const pk_dn_pair& pr = d->pairs[mix64(seed ^ mix64(((uint64_t)it << 20) + b)) % d->pairs.size()];
+        const int* sel = hsel.data() + (size_t)it * batch;
+        bool same = true;
+        for (int b = 1; b < batch; b++)
+            for (int k = 0; k < 3; k++) same = same && d->pairs[sel[b]].n[k] == d->pairs[sel[0]].n[k];
+        // samples of one size go through every layer together (stacked); otherwise one by one
+        const int groups = same ? 1 : batch, per = same ? batch : 1;
+        for (int gi = 0; gi < groups && !r; gi++) {
             Vol V;
-            if ((r = check_dims(d, pr.n, V))) break;
-            hipLaunchKernelGGL(k_input_f32, dim3(nblk(V.nv, 256)), dim3(256), 0, c->stream, V.nv, pr.low, 1.0, d->xb);
+            if ((r = check_dims(d, d->pairs[sel[gi * per]].n, V)) || (r = stack_samples(d, V, per))) break;
+            hipLaunchKernelGGL(k_gather, dim3(nblk(V.nv, 256), per), dim3(256), 0, c->stream, V.nv, dptr, dptr + npairs,
+                               dsel + (size_t)it * batch + gi * per, d->xb, d->tgt);
             if ((r = forward(d, V))) break;
-            PK_HIP(c, hipMemcpyAsync(d->tgt, pr.high, sizeof(float) * 3 * V.nv, hipMemcpyDeviceToDevice, c->stream));
             r = backward(d, V, 1.0 / (3.0 * V.nv * batch), dev_losses + it);
         }
         if (r) break;
@@ -792,7 +978,8 @@ int pk_dn_train(pk_dn* d, int iters, int batch, double lr, uint64_t seed, float*
         for (int it = 0; it < iters && !r; it++) losses[it] = (float)h[it];
     }
     (void)hipStreamSynchronize(c->stream);
-    (void)hipFree(dev_losses);
+    for (void* p : {(void*)dev_losses, (void*)dsel, (void*)dptr})
+        if (p) (void)hipFree(p);
     return r;
 }
 

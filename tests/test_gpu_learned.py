@@ -85,6 +85,79 @@ def test_gradient_matches_oracle(dn):
         o += k
 
 
+def test_batch_gradient_is_the_mean_over_samples(dn):
+    """pk_dn_grad_batch_host stacks the samples in one volume (the mini-batch form of pk_dn_train);
+    each sample stays periodic on its own, so the loss and gradient are the means of the oracle's
+    per-sample ones. Edges 5 and 10 do not fill the conv bricks (2 x 8 x 16), so bricks at the
+    sample boundaries are partial."""
+    _say("denoiser batch gradient")
+    p = _params(7)
+    dn.set_params(p)
+    rng = np.random.default_rng(8)
+    x = rng.standard_normal((3, 3, 5, 8, 10)).astype(np.float32)
+    y = rng.standard_normal((3, 3, 5, 8, 10)).astype(np.float32)
+    loss, g = dn.grad_batch(x, y)
+    refs = [D.net_grad(p, x[k], y[k], emulate=True) for k in range(3)]
+    ref_loss = np.mean([r[0] for r in refs])
+    ref_g = np.mean([r[1] for r in refs], axis=0)
+    assert abs(loss - ref_loss) < 1e-4 * abs(ref_loss)
+    o = 0
+    for name, shp in D.SHAPES:
+        k = int(np.prod(shp))
+        err = rel_err(g[o:o + k], ref_g[o:o + k])
+        print("batch grad %s rel err %.3g" % (name, err), flush=True)
+        assert err < 2e-2, name
+        o += k
+
+
+def _pair_grids(ctx, x, y):
+    """two FP64 periodic grids whose physical J are x and y ([3][n0][n1][n2])"""
+    grids = []
+    for a in (x, y):
+        g = P.YeeGrid.periodic(ctx, a.shape[1:])
+        for c, comp in enumerate((P.JX, P.JY, P.JZ)):
+            f = np.zeros(g.size)
+            f[2:-2, 2:-2, 2:-2] = a[c]
+            g.set(comp, f)
+        grids.append(g)
+    return grids
+
+
+def test_train_first_step_is_adam_on_the_batch_gradient(ctx, dn):
+    """pk_dn_train with one pair: every mini-batch stacks copies of it, so its gradient is the
+    pair's own, and Adam's first step from fresh moments moves each parameter by
+    -lr g / (|g| + eps); the loss of step 0 is the pair's loss. Further steps lower the loss."""
+    _say("denoiser training step")
+    rng = np.random.default_rng(10)
+    x = rng.standard_normal((3, 6, 8, 10)).astype(np.float32)
+    y = (0.5 * x + 0.1 * rng.standard_normal(x.shape)).astype(np.float32)
+    p0 = _params(9)
+    dn.set_scale(1.0)
+    dn.set_params(p0)
+    loss_ref, g_ref = dn.grad(x, y)
+    low, high = _pair_grids(ctx, x, y)
+    dn.add_pair(low, high)
+    lr = 1e-3
+    losses = dn.train(1, 3, lr, seed=5)
+    assert abs(losses[0] - loss_ref) < 1e-5 * loss_ref
+    step = (p0.astype(np.float64) - dn.params()) / lr
+    big = np.abs(g_ref) > 1e-3 * np.abs(g_ref).max()
+    assert big.sum() > 0.9 * big.size
+    assert np.max(np.abs(step[big] - np.sign(g_ref[big]))) < 1e-3
+    assert np.max(np.abs(step)) < 1.0 + 1e-3
+    more = dn.train(40, 3, lr, seed=6)
+    print("loss %.4g -> %.4g" % (losses[0], more[-1]), flush=True)
+    assert np.all(np.isfinite(more)) and more[-1] < 0.7 * losses[0]
+    # a second pair of another size: mini-batches that mix sizes run their samples one by one
+    x2 = rng.standard_normal((3, 4, 4, 8)).astype(np.float32)
+    low2, high2 = _pair_grids(ctx, x2, 0.5 * x2)
+    dn.add_pair(low2, high2)
+    mixed = dn.train(20, 4, lr, seed=7)
+    assert np.all(np.isfinite(mixed))
+    for g in (low, high, low2, high2):
+        g.close()
+
+
 @pytest.mark.parametrize("prec", [P.F64, P.F32])
 def test_apply_on_grid_equals_forward(ctx, dn, prec):
     """pk_dn_apply reads J from the grid and writes the network output back into it; it must give
