@@ -236,14 +236,17 @@ __global__ void __launch_bounds__(256) k_conv(Vol V, const uint16_t* __restrict_
 // periodic one-voxel halo and its output gradient are staged in LDS as in k_conv, and each
 // wavefront accumulates its tiles over the brick's 256 voxels on v_mfma_f32_16x16x32_bf16
 // (K-block = 32 voxels: two 16-voxel z-rows; lane (m, q) holds voxels 8q .. 8q + 7 of the block for
-// row / column m).  The workgroup's tiles go to part[blockIdx.x]; k_wgrad_sum adds them into dW.
+// row / column m).  Tile NT is the bias gradient db[co] = sum_v g[v][co]: A is all ones, so every
+// row holds it; it fills the slot that wavefront 3 has left over (27 = 3 + 4 x 6, 7 = 3 + 4 x 1).
+// The workgroup's NT + 1 tiles go to part[blockIdx.x]; k_wgrad_sum adds them into dW and db.
 constexpr int WG_TILES16 = 27, WG_TILES4 = 7, WGRAD_WG_MAX = 512;
+static_assert(WG_TILES16 % 4 == 3 && WG_TILES4 % 4 == 3, "the bias tile takes wavefront 3's last slot");
 
 template <int CIN>
 __global__ void __launch_bounds__(256) k_wgrad(Vol V, const uint16_t* __restrict__ in, const uint16_t* __restrict__ g,
                                                int gpad, int cout, float* __restrict__ part)
 {
-    constexpr int NT = CIN == 16 ? WG_TILES16 : WG_TILES4, TPW = (NT + 3) / 4;
+    constexpr int NT = CIN == 16 ? WG_TILES16 : WG_TILES4, TPW = (NT + 1 + 3) / 4;
     constexpr int UNITS = CIN == 16 ? 2 : 1, NVOX = BX * BY * BZ;
     static_assert(NVOX == 256 && BZ == 16, "one brick voxel per thread, K-blocks of two z-rows");
     extern __shared__ __align__(16) unsigned char smem[];
@@ -312,10 +315,11 @@ __global__ void __launch_bounds__(256) k_wgrad(Vol V, const uint16_t* __restrict
 #pragma unroll
             for (int i = 0; i < TPW; i++) {
                 const int tile = wave + 4 * i;
-                if (tile >= NT) break;
+                if (tile > NT) break;
                 const int tap = CIN == 16 ? tile : 4 * tile + m / 4, ci = CIN == 16 ? m : m % 4;
-                bf16x8 A = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
-                if (tap < 27) {
+                constexpr short one = 0x3f80;   // BF16 1.0
+                bf16x8 A = tile == NT ? bf16x8{one, one, one, one, one, one, one, one} : bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+                if (tile < NT && tap < 27) {
                     const int h0 = ((tx + tap / 9) * HY + (ty + (tap / 3) % 3)) * HZ + tz + tap % 3;
 #pragma unroll
                     for (int j = 0; j < 8; j++) A[j] = (short)halo[(h0 + j) * CIN + ci];
@@ -326,62 +330,41 @@ __global__ void __launch_bounds__(256) k_wgrad(Vol V, const uint16_t* __restrict
         }
     }
     // D[4q + j][m]: row 4q + j of the tile, column co = m
-    float* pp = part + (long)blockIdx.x * NT * 256;
+    float* pp = part + (long)blockIdx.x * (NT + 1) * 256;
 #pragma unroll
     for (int i = 0; i < TPW; i++) {
         const int tile = wave + 4 * i;
-        if (tile < NT)
+        if (tile <= NT)
 #pragma unroll
             for (int j = 0; j < 4; j++) pp[tile * 256 + (4 * q + j) * 16 + m] = acc[i][j];
     }
 }
 
-// dW[t][ci][co] += sum over the nwg workgroups' tiles; blockIdx.y takes every gridDim.y-th group of
-// 4 workgroups, so a weight receives gridDim.y atomic adds
+// dW[t][ci][co] and db[co] += sum over the nwg workgroups' tiles; blockIdx.y takes every
+// gridDim.y-th group of 4 workgroups, so a weight receives gridDim.y atomic adds
 template <int CIN>
 __global__ void __launch_bounds__(256) k_wgrad_sum(int nwg, const float* __restrict__ part, int cin, int cout,
-                                                   float* __restrict__ dW)
+                                                   float* __restrict__ dW, float* __restrict__ db)
 {
-    constexpr int NT = CIN == 16 ? WG_TILES16 : WG_TILES4;
+    constexpr int NT = CIN == 16 ? WG_TILES16 : WG_TILES4, NE = (NT + 1) * 256;
     const int lane = threadIdx.x & 63, p = threadIdx.x >> 6;
     const int i = blockIdx.x * 64 + lane;
     float s = 0.0f;
-    if (i < NT * 256)
-        for (int w = 4 * blockIdx.y + p; w < nwg; w += 4 * gridDim.y) s += part[(long)w * NT * 256 + i];
+    if (i < NE)
+        for (int w = 4 * blockIdx.y + p; w < nwg; w += 4 * gridDim.y) s += part[(long)w * NE + i];
     __shared__ float red[4][64];
     red[p][lane] = s;
     __syncthreads();
-    if (p != 0 || i >= NT * 256) return;
+    if (p != 0 || i >= NE) return;
     s = red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane];
     const int tile = i / 256, row = (i / 16) % 16, co = i % 16;
+    if (s == 0.0f || co >= cout) return;
+    if (tile == NT) {
+        if (row == 0) atomicAdd(&db[co], s);
+        return;
+    }
     const int tap = CIN == 16 ? tile : 4 * tile + row / 4, ci = CIN == 16 ? row : row % 4;
-    if (tap < 27 && ci < cin && co < cout && s != 0.0f) atomicAdd(&dW[((long)tap * cin + ci) * cout + co], s);
-}
-
-// db[co] += sum_v g[v][co]
-__global__ void __launch_bounds__(256) k_bgrad(long nv, const uint16_t* __restrict__ g, int gpad, int cout,
-                                               float* __restrict__ db)
-{
-    float s[16];
-#pragma unroll
-    for (int c = 0; c < 16; c++) s[c] = 0.0f;
-    for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += (long)gridDim.x * blockDim.x)
-#pragma unroll
-        for (int c = 0; c < 16; c++)
-            if (c < cout) s[c] += from_bf16(g[v * gpad + c]);
-    // wavefront sums, then one sum per workgroup in LDS: 16 atomics per workgroup, not per wavefront
-    __shared__ float part[4][16];
-#pragma unroll
-    for (int c = 0; c < 16; c++) {
-        float t = s[c];
-        for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
-        if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6][c] = t;
-    }
-    __syncthreads();
-    if (threadIdx.x < 16 && (int)threadIdx.x < cout) {
-        const float t = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] + part[3][threadIdx.x];
-        if (t != 0.0f) atomicAdd(&db[threadIdx.x], t);
-    }
+    if (tap < 27 && ci < cin) atomicAdd(&dW[((long)tap * cin + ci) * cout + co], s);
 }
 
 // weights -> MFMA B fragments of v_mfma_f32_16x16x32_bf16, wf[kc][lane][8].  Effective layer:
@@ -497,9 +480,9 @@ __global__ void __launch_bounds__(256) k_loss(long nv, int ns, const float* __re
                                               const float* __restrict__ y, double inv_count, uint16_t* __restrict__ g3,
                                               double* __restrict__ loss)
 {
-    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;   // stacked voxel; out, y are [ns][3][nv]
     double s = 0.0;
-    if (v < nv * ns) {
+    // v: stacked voxel; out, y are [ns][3][nv]
+    for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < nv * ns; v += (long)gridDim.x * blockDim.x) {
         const long sample = v / nv, base = 2 * sample * nv + v;   // (3 sample + c) nv + (v - sample nv)
 #pragma unroll
         for (int c = 0; c < 3; c++) {
@@ -520,12 +503,14 @@ __global__ void __launch_bounds__(256) k_loss(long nv, int ns, const float* __re
     }
 }
 
-__global__ void k_adam(int n, float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+// one Adam step; the gradient is cleared for the next mini-batch
+__global__ void k_adam(int n, float* __restrict__ p, float* __restrict__ g, float* __restrict__ m,
                        float* __restrict__ v, float lr, float b1, float b2, float eps, float c1, float c2)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float gi = g[i];
+    g[i] = 0.0f;
     const float mi = b1 * m[i] + (1.0f - b1) * gi;
     const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
     m[i] = mi;
@@ -608,20 +593,21 @@ int forward(pk_dn* d, const Vol& V)
 
 constexpr size_t wgrad_lds(int cin) { return sizeof(uint16_t) * (HX * HY * HZ * cin + BX * BY * BZ * 16); }
 
+// weight and bias gradients of one layer: dW += sum_v in[v + off(t)] g[v], db += sum_v g[v]
 int wgrad(pk_dn* d, const Vol& V, const uint16_t* in, int cin_pad, int cin, const uint16_t* g, int gpad, int cout,
-          float* dW)
+          float* dW, float* db)
 {
     hipStream_t s = d->ctx->stream;
     const long total = (long)((V.n0 + BX - 1) / BX) * ((V.n1 + BY - 1) / BY) * ((V.n2 + BZ - 1) / BZ) * V.ns;
     const int nwg = (int)(total < WGRAD_WG_MAX ? total : WGRAD_WG_MAX);
     if (cin_pad == 16) {
         hipLaunchKernelGGL(k_wgrad<16>, dim3(nwg), dim3(256), wgrad_lds(16), s, V, in, g, gpad, cout, d->part);
-        hipLaunchKernelGGL(k_wgrad_sum<16>, dim3(WG_TILES16 * 256 / 64, 8), dim3(256), 0, s, nwg, d->part, cin, cout,
-                           dW);
+        hipLaunchKernelGGL(k_wgrad_sum<16>, dim3((WG_TILES16 + 1) * 256 / 64, 8), dim3(256), 0, s, nwg, d->part, cin,
+                           cout, dW, db);
     } else {
         hipLaunchKernelGGL(k_wgrad<4>, dim3(nwg), dim3(256), wgrad_lds(4), s, V, in, g, gpad, cout, d->part);
-        hipLaunchKernelGGL(k_wgrad_sum<4>, dim3(WG_TILES4 * 256 / 64, 8), dim3(256), 0, s, nwg, d->part, cin, cout,
-                           dW);
+        hipLaunchKernelGGL(k_wgrad_sum<4>, dim3((WG_TILES4 + 1) * 256 / 64, 8), dim3(256), 0, s, nwg, d->part, cin,
+                           cout, dW, db);
     }
     PK_HIP(d->ctx, hipGetLastError());
     return PK_OK;
@@ -633,19 +619,16 @@ int backward(pk_dn* d, const Vol& V, double inv_count, double* loss)
     hipStream_t s = d->ctx->stream;
     int r;
     const long nt = V.nv * V.ns;
-    hipLaunchKernelGGL(k_loss, dim3(nblk(nt, 256)), dim3(256), 0, s, V.nv, V.ns, d->out, d->tgt, inv_count, d->g3, loss);
-    const unsigned nbb = nblk(nt, 1024) < 256 ? nblk(nt, 1024) : 256;   // 4+ voxels per thread
-    if ((r = wgrad(d, V, d->a2, 16, HID, d->g3, 4, 3, d->g + OFF_W3))) return r;
-    hipLaunchKernelGGL(k_bgrad, dim3(nbb), dim3(256), 0, s, nt, d->g3, 4, 3, d->g + OFF_B3);
+    const unsigned nl = nblk(nt, 256) < 128 ? nblk(nt, 256) : 128;   // few workgroups: one loss atomic each
+    hipLaunchKernelGGL(k_loss, dim3(nl), dim3(256), 0, s, V.nv, V.ns, d->out, d->tgt, inv_count, d->g3, loss);
+    if ((r = wgrad(d, V, d->a2, 16, HID, d->g3, 4, 3, d->g + OFF_W3, d->g + OFF_B3))) return r;
     const unsigned nb = conv_blocks(V);
     hipLaunchKernelGGL((k_conv<4, OUT_MASKED_BF16>), dim3(nb), dim3(256), conv_lds(4), s, V, d->g3, d->wt[1], (const float*)nullptr,
                        HID, d->dz2, (float*)nullptr, d->a2);
-    if ((r = wgrad(d, V, d->a1, 16, HID, d->dz2, 16, HID, d->g + OFF_W2))) return r;
-    hipLaunchKernelGGL(k_bgrad, dim3(nbb), dim3(256), 0, s, nt, d->dz2, 16, HID, d->g + OFF_B2);
+    if ((r = wgrad(d, V, d->a1, 16, HID, d->dz2, 16, HID, d->g + OFF_W2, d->g + OFF_B2))) return r;
     hipLaunchKernelGGL((k_conv<16, OUT_MASKED_BF16>), dim3(nb), dim3(256), conv_lds(16), s, V, d->dz2, d->wt[0],
                        (const float*)nullptr, HID, d->dz1, (float*)nullptr, d->a1);
-    if ((r = wgrad(d, V, d->xb, 4, 3, d->dz1, 16, HID, d->g + OFF_W1))) return r;
-    hipLaunchKernelGGL(k_bgrad, dim3(nbb), dim3(256), 0, s, nt, d->dz1, 16, HID, d->g + OFF_B1);
+    if ((r = wgrad(d, V, d->xb, 4, 3, d->dz1, 16, HID, d->g + OFF_W1, d->g + OFF_B1))) return r;
     PK_HIP(d->ctx, hipGetLastError());
     return PK_OK;
 }
@@ -708,7 +691,7 @@ int pk_dn_create(pk_ctx* c, pk_dn** out)
     if (!r) r = dalloc(c, (void**)&d->wt[0], sizeof(uint16_t) * WF16);
     if (!r) r = dalloc(c, (void**)&d->wt[1], sizeof(uint16_t) * WF4);
     if (!r) r = dalloc(c, (void**)&d->loss, sizeof(double));
-    if (!r) r = dalloc(c, (void**)&d->part, sizeof(float) * WGRAD_WG_MAX * WG_TILES16 * 256);
+    if (!r) r = dalloc(c, (void**)&d->part, sizeof(float) * WGRAD_WG_MAX * (WG_TILES16 + 1) * 256);
     if (r) {
         pk_dn_destroy(d);
         return r;
@@ -945,9 +928,13 @@ int pk_dn_train(pk_dn* d, int iters, int batch, double lr, uint64_t seed, float*
                    hipSuccess))
         r = pk_fail(c, PK_E_HIP, "pk_dn_train: upload failed");
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-    for (int it = 0; it < iters && !r; it++) {
+    if (!r) {
+        // k_adam clears the gradient after each step
         hipLaunchKernelGGL(k_zero_f32, dim3(nblk(NPARAM, 256)), dim3(256), 0, c->stream, (long)NPARAM, d->g);
-        hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(1), 0, c->stream, dev_losses + it);
+        hipLaunchKernelGGL(k_zero_f32, dim3(nblk(2l * iters, 256) + 1), dim3(256), 0, c->stream, 2l * iters,
+                           (float*)dev_losses);
+    }
+    for (int it = 0; it < iters && !r; it++) {
         const int* sel = hsel.data() + (size_t)it * batch;
         bool same = true;
         for (int b = 1; b < batch; b++)
