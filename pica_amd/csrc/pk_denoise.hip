@@ -13,7 +13,8 @@
 // With 16 input channels a K-chunk is 2 taps x 16 channels (14 chunks, the 28th tap zero); with
 // 3 (padded to 4) it is 8 taps x 4 channels (4 chunks).  The input gradient is the same forward
 // kernel with the weights flipped (tap -> 26 - tap) and transposed; the weight gradient is a GEMM
-// over voxels per tap on v_mfma_f32_16x16x16_bf16.
+// with K = voxels over LDS-staged bricks on the same 16x16x32 MFMA, with the bias gradient as one
+// more tile.  Training stacks a mini-batch's samples so that each layer is one launch.
 //
 // Tap t = (dx+1)*9 + (dy+1)*3 + (dz+1) reads the input at (x+dx, y+dy, z+dz), periodic.
 // Activations are channel-last BF16 ([voxel][16] or [voxel][4]); the network output and the
