@@ -51,7 +51,7 @@ def test_interp_deposit_plain_and_supercell(seed):
 
 def test_supercell_variants_agree_with_plain_for_origin_minus_two_steps():
     """pica's supercell classes ignore the grid origin and assume -2 h; for that origin they match
-    the plain classes to rounding (the survey measured 1.8e-15)"""
+    the plain classes to rounding (differences of order 1e-15)"""
     rng = np.random.default_rng(7)
     s = (14, 14, 14)
     g = O.Grid(s, (-2.0, -2.0, -2.0), (1.0, 1.0, 1.0), f=rng.standard_normal((9,) + s))
