@@ -123,7 +123,8 @@ def test_one_step_particles_bitexact(ctx, shape, dep):
         assert rel_err(f[c], og.f[c]) < 1e-13, "component %d" % c
 
 
-@pytest.mark.parametrize("shape,dep", [(P.CIC, P.DIRECT), (P.TSC, P.ESIRKEPOV)])
+@pytest.mark.parametrize("shape,dep", [(P.CIC, P.DIRECT), (P.CIC, P.ESIRKEPOV), (P.TSC, P.DIRECT),
+                                       (P.TSC, P.ESIRKEPOV)])
 def test_twenty_steps_parity(ctx, shape, dep):
     _say("20 steps shape=%d dep=%d" % (shape, dep))
     dt, og, oe, g, ens = _setup_pair(ctx, (16, 8, 8), (4, 4, 4), 8, 31 + shape, shape, dep, uth=0.05, drift=0.1)
