@@ -923,11 +923,12 @@ int pk_dn_train(pk_dn* d, int iters, int batch, double lr, uint64_t seed, float*
     int r = dalloc(c, (void**)&dev_losses, sizeof(double) * (iters > 0 ? iters : 1));
     if (!r) r = dalloc(c, (void**)&dsel, sizeof(int) * (hsel.empty() ? 1 : hsel.size()));
     if (!r) r = dalloc(c, (void**)&dptr, sizeof(float*) * hptr.size());
-    if (!r && (hipMemcpyAsync(dsel, hsel.data(), sizeof(int) * hsel.size(), hipMemcpyHostToDevice, c->stream) !=
-                   hipSuccess ||
-               hipMemcpyAsync(dptr, hptr.data(), sizeof(float*) * hptr.size(), hipMemcpyHostToDevice, c->stream) !=
-                   hipSuccess))
-        r = pk_fail(c, PK_E_HIP, "pk_dn_train: upload failed");
+    if (!r && iters > 0) {
+        const size_t bsel = sizeof(int) * hsel.size(), bptr = sizeof(float*) * hptr.size();
+        if (hipMemcpyAsync(dsel, hsel.data(), bsel, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+            hipMemcpyAsync(dptr, hptr.data(), bptr, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+            r = pk_fail(c, PK_E_HIP, "pk_dn_train: upload failed");
+    }
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     if (!r) {
         // k_adam clears the gradient after each step

@@ -154,6 +154,9 @@ def test_train_first_step_is_adam_on_the_batch_gradient(ctx, dn):
     dn.add_pair(low2, high2)
     mixed = dn.train(20, 4, lr, seed=7)
     assert np.all(np.isfinite(mixed))
+    before = dn.params()
+    assert dn.train(0, 4, lr, seed=8).size == 0            # no steps: parameters unchanged
+    assert np.array_equal(dn.params(), before)
     for g in (low, high, low2, high2):
         g.close()
 
