@@ -105,10 +105,11 @@ __device__ __forceinline__ long neighbour(const Vol& V, int x, int y, int z, int
 enum { OUT_RELU_BF16 = 0, OUT_F32 = 1, OUT_MASKED_BF16 = 2 };
 
 // One convolution layer over a brick of BX x BY x BZ voxels per workgroup: the brick and its
-// one-voxel periodic halo are staged in LDS (each input voxel leaves L2 about 2.8 times instead
-// of 27), then every wavefront runs 16-voxel z-rows of the brick through the MFMA chain.
+// one-voxel periodic halo are staged in LDS (each input voxel leaves L2 about 2.5 times instead
+// of 27), then every wavefront runs 16-voxel z-rows of the brick through the MFMA chain.  A 4 x 4
+// x 16 brick stages 648 voxels for 256 (2 x 8 x 16: 720, and a 10% slower denoiser at 128^3).
 // CIN = 16 (2 taps x 16 channels per K-chunk) or 4 (8 taps x 4 channels per chunk).
-constexpr int BX = 2, BY = 8, BZ = 16, HX = BX + 2, HY = BY + 2, HZ = BZ + 2;
+constexpr int BX = 4, BY = 4, BZ = 16, HX = BX + 2, HY = BY + 2, HZ = BZ + 2;
 constexpr int BRICK_TILES = BX * BY * (BZ / 16);
 static_assert(BRICK_TILES % 8 == 0, "4 wavefronts x 2 rows per pass");
 

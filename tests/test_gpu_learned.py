@@ -88,7 +88,7 @@ def test_gradient_matches_oracle(dn):
 def test_batch_gradient_is_the_mean_over_samples(dn):
     """pk_dn_grad_batch_host stacks the samples in one volume (the mini-batch form of pk_dn_train);
     each sample stays periodic on its own, so the loss and gradient are the means of the oracle's
-    per-sample ones. Edges 5 and 10 do not fill the conv bricks (2 x 8 x 16), so bricks at the
+    per-sample ones. Edges 5 and 10 do not fill the conv bricks (4 x 4 x 16), so bricks at the
     sample boundaries are partial."""
     _say("denoiser batch gradient")
     p = _params(7)
