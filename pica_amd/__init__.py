@@ -335,8 +335,18 @@ class EnsembleSupercells:
         """EnsembleSupercells::add for every row of x (n x 3 positions), u (n x 3, p/(mc)), w."""
         x = np.asarray(x, np.float64)
         u = np.asarray(u, np.float64)
+        if x.ndim != 2 or x.shape[1] != 3:
+            raise ValueError("add: x must be an (n, 3) array of positions")
+        n = x.shape[0]
+        if u.shape != (n, 3):
+            raise ValueError("add: u must have the shape of x, (%d, 3); got %s" % (n, u.shape))
+        w = np.asarray(w, np.float64)
+        if w.ndim == 0:
+            w = np.full(n, float(w))
+        if w.shape != (n,):
+            raise ValueError("add: w must be a scalar or hold one weight per particle (%d); got %s" % (n, w.shape))
         cols = [np.ascontiguousarray(x[:, d]) for d in range(3)] + [np.ascontiguousarray(u[:, d]) for d in range(3)]
-        cols.append(np.ascontiguousarray(w, np.float64))
+        cols.append(np.ascontiguousarray(w))
         self.ctx.check(lib().pk_ens_upload(self.h, x.shape[0], *[_p(c) for c in cols]))
 
     def load_thermal(self, ppc, uth, drift_x, weight, seed):

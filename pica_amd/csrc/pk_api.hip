@@ -68,6 +68,27 @@ int pk_flags_read(pk_ens* e, int32_t out[PK_NFLAGS])
     return PK_OK;
 }
 
+int pk_ens_usable(pk_ens* e)
+{
+    if (!e->failed) return PK_OK;
+    return pk_fail(e->ctx, PK_E_STATE,
+                   "ensemble unusable: a re-sort could not rebuild its storage, so its particles are "
+                   "inconsistent; upload or load it again");
+}
+
+int pk_ens_rebuild_after_overflow(pk_ens* e)
+{
+    // The insert kernel has already copied the overflowing supercell's leavers into its
+    // neighbours, so a failed rebuild leaves particles duplicated or missing: refuse every later
+    // use of the ensemble until new contents arrive.
+    const int r = pk_ens_rebuild(e, 0);
+    if (r) {
+        e->failed = true;
+        e->ctx->err += " (the ensemble is unusable until it is uploaded or loaded again)";
+    }
+    return r;
+}
+
 int pk_flags_clear(pk_ens* e)
 {
     hipLaunchKernelGGL(k_fill_flags, dim3(1), dim3(64), 0, e->ctx->stream, e->flags, PK_NFLAGS);
@@ -385,15 +406,17 @@ int pk_ens_upload(pk_ens* e, size_t n, const double* x, const double* y, const d
 int pk_ens_resort(pk_ens* e)
 {
     if (!e) return PK_E_ARG;
+    int r = pk_ens_usable(e);
+    if (r) return r;
     if (!e->pending_movers) return PK_OK;
-    int r = pk_ens_insert_launch(e);
+    r = pk_ens_insert_launch(e);
     if (r) return r;
     int32_t fl[PK_NFLAGS];
     r = pk_flags_read(e, fl);
     if (r) return r;
     e->pending_movers = false;
     if (fl[PK_FLAG_OVERFLOW]) {
-        r = pk_ens_rebuild(e, 0);
+        r = pk_ens_rebuild_after_overflow(e);
         if (r) return r;
     }
     r = pk_flags_clear(e);
@@ -456,6 +479,7 @@ int pk_particle_step(pk_ens* e, pk_grid* g, double dt, int shape, int dep)
     if ((shape != PK_CIC && shape != PK_TSC) || (dep != PK_DIRECT && dep != PK_ESIRKEPOV))
         return pk_fail(e->ctx, PK_E_ARG, "pk_particle_step: unknown shape or depositor");
     if (!(dt > 0.0)) return pk_fail(e->ctx, PK_E_ARG, "pk_particle_step: dt must be positive");
+    if ((r = pk_ens_usable(e))) return r;
     if (e->pending_movers) {
         r = pk_ens_resort(e);
         if (r) return r;

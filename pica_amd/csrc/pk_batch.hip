@@ -184,7 +184,7 @@ int run(pk_batch* b, double dt, int shape, int dep, int nsteps)
             int32_t fl[PK_NFLAGS];
             if ((r = pk_flags_read(e, fl))) return r;
             if (fl[PK_FLAG_ERR]) state_err = true;
-            if (fl[PK_FLAG_OVERFLOW] && (r = pk_ens_rebuild(e, 0))) return r;
+            if (fl[PK_FLAG_OVERFLOW] && (r = pk_ens_rebuild_after_overflow(e))) return r;
             if ((r = pk_flags_clear(e))) return r;
         }
         // the interleave the overflowed step skipped, where its count is due

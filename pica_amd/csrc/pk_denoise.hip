@@ -546,6 +546,9 @@ int ensure_cap(pk_dn* d, long nv)
     if (nv <= d->cap) return PK_OK;
     pk_ctx* c = d->ctx;
     PK_HIP(c, hipStreamSynchronize(c->stream));
+    // no capacity while the buffers are replaced: a failed allocation leaves cap 0, so a later
+    // call grows again instead of launching on a freed or null buffer
+    d->cap = 0;
     int r;
     if ((r = dalloc(c, (void**)&d->xb, sizeof(uint16_t) * 4 * nv))) return r;
     if ((r = dalloc(c, (void**)&d->a1, sizeof(uint16_t) * 16 * nv))) return r;

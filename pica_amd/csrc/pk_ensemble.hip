@@ -1022,6 +1022,7 @@ static int rebuild_from(pk_ens* e, long nent, DBuf& keys_in, DBuf& srcs_in, Fiel
     e->maxcap = mc;
     e->np = nent;
     e->pending_movers = false;
+    e->failed = false;
     return PK_OK;
 }
 
@@ -1123,6 +1124,7 @@ int pk_ens_load_impl(pk_ens* e, int ppc, double uth, double drift, double weight
     e->maxcap = (int)cap;
     e->np = (int64_t)per_sc * e->nsct;
     e->pending_movers = false;
+    e->failed = false;
     return PK_OK;
 }
 

@@ -88,6 +88,7 @@ struct pk_ens {
     int maxcap = 0;
     int64_t np = 0;
     bool pending_movers = false;  // particle step done, re-sort not yet done
+    bool failed = false;          // a re-sort rebuild failed: contents inconsistent until upload / load
     long resorts = 0;             // re-sorts done (the interleave period counts them)
     int nfield() const { return prec == PK_F64 ? 7 : 8; }
     size_t esize() const { return prec == PK_F64 ? 8 : 4; }
@@ -253,6 +254,8 @@ int pk_ens_interleave_launch(pk_ens* e);   // reorder now, no re-sort first
 int pk_check_pair(pk_ens* e, pk_grid* g);   // the periodic step's grid/ensemble requirements
 int pk_flags_read(pk_ens* e, int32_t out[PK_NFLAGS]);
 int pk_flags_clear(pk_ens* e);
+int pk_ens_usable(pk_ens* e);               // PK_E_STATE after a failed re-sort rebuild
+int pk_ens_rebuild_after_overflow(pk_ens* e);   // rebuild of an overflowed re-sort; marks failure
 
 // ---- launchers implemented in the .hip files ----
 int pk_field_zero(pk_grid* g, int comp);

@@ -223,6 +223,7 @@ int pk_ens_load(pk_ens* e, const char* path)
     e->maxcap = maxcap;
     e->np = np;
     e->pending_movers = false;
+    e->failed = false;
     e->resorts = h.resorts;
     return PK_OK;
 }

@@ -163,3 +163,40 @@ def test_supercell_at_the_particle_limit(ctx, prec):
         over.add(xo, uo, np.ones(xo.shape[0]) * w0)
     assert ei.value.code == P.PK_E_CAPACITY
     over.close()
+
+
+@pytest.mark.parametrize("prec", [P.F64, P.F32])
+def test_failed_resort_rebuild_refuses_later_calls(ctx, prec):
+    """a re-sort whose rebuild fails (one supercell would pass 65,536 particles) reports
+    PK_E_CAPACITY; the ensemble then refuses every step and read with PK_E_STATE until a new
+    upload, after which it steps normally"""
+    _say("failed re-sort rebuild prec=%d" % prec)
+    rng = np.random.default_rng(92)
+    ncell = (8, 8, 8)
+    xa, ua = _clustered(rng, 65536, (0.1, 0.1, 0.1), (3.9, 3.9, 3.9), 0.0, 0.0)
+    xb = np.array([[4.02, 1.5, 1.5]])                   # supercell (1, 0, 0), about to cross into (0, 0, 0)
+    ub = np.array([[-0.5, 0.0, 0.0]])
+    x, u = _fp32_exact(np.concatenate([xa, xb]), np.concatenate([ua, ub]))
+    _, w0 = plasma(courant_dt(), 1024)
+    w = np.full(x.shape[0], w0)
+    g = P.YeeGrid.periodic(ctx, ncell, prec=prec)
+    ens = P.EnsembleSupercells(ctx, ncell, (1.0, 1.0, 1.0), (4, 4, 4), prec=prec)
+    ens.set_slack(0.0, 0)
+    ens.add(x, u, w)
+    dt = courant_dt()
+    with pytest.raises(P.PicaError) as ei:
+        P.step(ens, g, dt, P.CIC, P.DIRECT)
+    assert ei.value.code == P.PK_E_CAPACITY
+    with pytest.raises(P.PicaError) as ei:
+        P.step(ens, g, dt, P.CIC, P.DIRECT)
+    assert ei.value.code == P.PK_E_STATE
+    with pytest.raises(P.PicaError) as ei:
+        ens.particles()
+    assert ei.value.code == P.PK_E_STATE
+    # fresh contents make it usable again
+    ens.add(x[-1000:], u[-1000:], w[-1000:])
+    g2 = P.YeeGrid.periodic(ctx, ncell, prec=prec)
+    P.run(ens, g2, dt, 2, P.CIC, P.DIRECT)
+    gx, gu, gw, goff = ens.particles()
+    assert gx.shape[0] == 1000
+    ens.close()
