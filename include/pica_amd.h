@@ -51,6 +51,10 @@ enum {
     PK_DIAG_GAUSS_MAX,          /* max |div E - 4 pi rho| over cells */
     PK_DIAG_GAUSS_DRIFT,        /* max |(div E - 4 pi rho) - reference residual| (0 without a reference) */
     PK_DIAG_RHO_MAX,            /* max |4 pi rho| (scale for the residuals) */
+    PK_DIAG_CONTINUITY_MAX,     /* max |(rho - rho_saved)/dt + div J| over cells: rho_saved is the density
+                                   kept by the last pk_rho_save, dt and J those of the last particle step
+                                   on the grid (0 without a saved density) */
+    PK_DIAG_DIVJ_MAX,           /* max |div J| (scale for the continuity residual) */
     PK_DIAG_N
 };
 

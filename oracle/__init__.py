@@ -449,3 +449,35 @@ class _Ref:
 
 
 REF = _Ref()
+
+
+def load_thermal(ncell, ncps, step, ppc, uth, drift, weight, seed, paired):
+    """the library's seeded thermal loader restated on the CPU: particles in the order the device
+    stores them (supercell by supercell, slot order; paired = the FP32 split form's order).
+    Returns x (n, 3), u (n, 3), w (n,), cell (n,) -- cell is the linear physical cell, z fastest."""
+    nsc = [a // b for a, b in zip(ncell, ncps)]
+    n = int(np.prod(nsc)) * int(np.prod(ncps)) * int(ppc)
+    x = [np.zeros(n) for _ in range(3)]
+    u = [np.zeros(n) for _ in range(3)]
+    w = np.zeros(n)
+    cell = np.zeros(n, np.int32)
+    lib().or_load_thermal((ctypes.c_int * 3)(*ncell), (ctypes.c_int * 3)(*ncps), (ctypes.c_double * 3)(*step),
+                          int(ppc), ctypes.c_double(uth), ctypes.c_double(drift), ctypes.c_double(weight),
+                          ctypes.c_uint64(int(seed) & 0xFFFFFFFFFFFFFFFF), int(bool(paired)),
+                          (_dp * 3)(*[_ptr(v) for v in x]), (_dp * 3)(*[_ptr(v) for v in u]), _ptr(w),
+                          _ptr(cell, ctypes.c_int))
+    return np.stack(x, 1), np.stack(u, 1), w, cell
+
+
+def det_log(x):
+    """the loader's ln(x), 0 < x <= 1"""
+    f = lib().or_det_log_x
+    f.restype = ctypes.c_double
+    return f(ctypes.c_double(x))
+
+
+def det_sincos_2pi(U):
+    """the loader's (sin, cos)(2 pi U), 0 <= U < 1"""
+    s, c = ctypes.c_double(), ctypes.c_double()
+    lib().or_det_sincos_2pi_x(ctypes.c_double(U), ctypes.byref(s), ctypes.byref(c))
+    return s.value, c.value

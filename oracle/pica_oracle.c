@@ -24,7 +24,9 @@
  *   - Esirkepov charge-conserving deposition, CIC and TSC shapes (CPC 135 (2001) 144),
  *   - periodic ghost fill / J fold with two ghost layers (grid origin = -2 h),
  *   - the deterministic supercell re-sort rule (stayers first, then incoming movers by
- *     direction class), and rho deposition for the diagnostics.
+ *     direction class), and rho deposition for the diagnostics;
+ *   - the seeded thermal loader (or_load_thermal), the twin of the library's device loader,
+ *     with its ln / sincos pinned against libm and closed forms (tests/test_oracle_loader.py).
  *
  * Storage follows pica's Array3d: z fastest, data[(i*ny + j)*nz + k] (Array.h:143).
  */
@@ -1022,5 +1024,136 @@ void or_energy(const or_grid* g, const or_ens* e, double out[2])
     }
     out[1] = ke * e->m * OR_C * OR_C;
 }
+
+/* ------------------------------------------------------------------------------------ */
+/* Seeded thermal loader: the CPU twin of the library's pk_ens_load_thermal.  pica has no  */
+/* loader; this restates the library's counter-based one (SplitMix64 finaliser keyed by   */
+/* seed and particle id, Box-Muller from ln and sincos written with IEEE +, -, *, / only), */
+/* so that CPU and GPU runs start from bit-identical particles.                            */
+/* ------------------------------------------------------------------------------------ */
+static inline uint64_t or_mix64(uint64_t z)
+{
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+static inline double or_urand(uint64_t key, int j)
+{
+    return (double)(or_mix64(key + (uint64_t)j * 0xD1B54A32D192ED03ull) >> 11) * 0x1.0p-53;
+}
+
+/* ln(x), 0 < x <= 1: x = m 2^e, m in [sqrt(1/2), sqrt(2)], ln m = 2 atanh((m-1)/(m+1)) */
+static double or_det_log(double x)
+{
+    uint64_t bits;
+    memcpy(&bits, &x, 8);
+    int e = (int)((bits >> 52) & 0x7ff) - 1023;
+    const uint64_t mb = (bits & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull;
+    double m;
+    memcpy(&m, &mb, 8);
+    if (m > 1.4142135623730951) {
+        m = m * 0.5;
+        e += 1;
+    }
+    const double s = (m - 1.0) / (m + 1.0), s2 = s * s;
+    static const double inv_odd[12] = {1.0, 1.0 / 3.0, 1.0 / 5.0, 1.0 / 7.0, 1.0 / 9.0, 1.0 / 11.0, 1.0 / 13.0,
+                                       1.0 / 15.0, 1.0 / 17.0, 1.0 / 19.0, 1.0 / 21.0, 1.0 / 23.0};
+    double p = inv_odd[11];
+    for (int k = 10; k >= 0; k--) p = inv_odd[k] + s2 * p;
+    const double lnm = 2.0 * s * p;
+    const double de = (double)e;
+    return de * 6.93147180369123816490e-01 + (de * 1.90821492927058770002e-10 + lnm);
+}
+
+/* sin and cos of 2 pi U, 0 <= U < 1, by octant reduction and Taylor series on [0, pi/4] */
+static void or_det_sincos_2pi(double U, double* sn, double* cs)
+{
+    const double t = U * 8.0;
+    const int q = (int)t;
+    const double f = t - (double)q;
+    const double r = ((q & 1) ? 1.0 - f : f) * 0.78539816339744830962;
+    const double r2 = r * r;
+    /* 1/(2k+1)! for k = 1..9 and 1/(2k)! for k = 1..10 */
+    static const double is[9] = {1.0 / 6.0, 1.0 / 120.0, 1.0 / 5040.0, 1.0 / 362880.0, 1.0 / 39916800.0,
+                                 1.0 / 6227020800.0, 1.0 / 1307674368000.0, 1.0 / 355687428096000.0,
+                                 1.0 / 121645100408832000.0};
+    static const double ic[10] = {0.5, 1.0 / 24.0, 1.0 / 720.0, 1.0 / 40320.0, 1.0 / 3628800.0, 1.0 / 479001600.0,
+                                  1.0 / 87178291200.0, 1.0 / 20922789888000.0, 1.0 / 6402373705728000.0,
+                                  1.0 / 2432902008176640000.0};
+    double ps = is[8];
+    for (int k = 7; k >= 0; k--) ps = is[k] - r2 * ps;
+    const double sr = r - r * r2 * ps;
+    double pc = ic[9];
+    for (int k = 8; k >= 0; k--) pc = ic[k] - r2 * pc;
+    const double cr = 1.0 - r2 * pc;
+    switch (q & 7) {
+    case 0: *sn = sr; *cs = cr; break;
+    case 1: *sn = cr; *cs = sr; break;
+    case 2: *sn = cr; *cs = -sr; break;
+    case 3: *sn = sr; *cs = -cr; break;
+    case 4: *sn = -sr; *cs = -cr; break;
+    case 5: *sn = -cr; *cs = -sr; break;
+    case 6: *sn = -cr; *cs = sr; break;
+    default: *sn = -sr; *cs = cr; break;
+    }
+}
+
+/* Particles of load_thermal in the order the library stores them: supercell by supercell
+ * (z fastest), slot l of a supercell holding particle k of local cell lc with
+ *   unpaired: lc = l mod cells, k = l / cells;
+ *   paired:   pairs (2q, 2q+1) of a cell in adjacent slots, by (q, cell), then particle ppc-1
+ *             of every cell when ppc is odd.
+ * Particle id = cell id * ppc + k; key = mix(seed ^ mix(id)); U_j = urand(key, j):
+ * position (cell + U_0..2) h, u = uth (r1 cos a1, r1 sin a1, r2 cos a2) + (+-drift, 0, 0) with
+ * r = sqrt(-2 ln(1 - U_3 or 5)), a = 2 pi U_4 or 6, +drift for even ids.
+ * Output: absolute positions (x), u, and the cell of each particle (cell, z fastest). */
+void or_load_thermal(const int ncell[3], const int ncps[3], const double step[3], int ppc, double uth,
+                     double drift, double weight, uint64_t seed, int paired, double* x[3], double* u[3],
+                     double* w, int* cell)
+{
+    int nsc[3];
+    for (int d = 0; d < 3; d++) nsc[d] = ncell[d] / ncps[d];
+    const int ncsc = ncps[0] * ncps[1] * ncps[2];
+    const int per_sc = ncsc * ppc;
+    const long nsct = (long)nsc[0] * nsc[1] * nsc[2];
+    for (long s = 0; s < nsct; s++) {
+        const int sidx[3] = {(int)(s / ((long)nsc[1] * nsc[2])), (int)((s / nsc[2]) % nsc[1]), (int)(s % nsc[2])};
+        for (int l = 0; l < per_sc; l++) {
+            int lc = l % ncsc, k = l / ncsc;
+            if (paired) {
+                const int npair = (ppc & ~1) * ncsc;
+                if (l < npair) {
+                    lc = (l >> 1) % ncsc;
+                    k = 2 * ((l >> 1) / ncsc) + (l & 1);
+                } else {
+                    lc = l - npair;
+                    k = ppc - 1;
+                }
+            }
+            const int a = lc / (ncps[1] * ncps[2]), b = (lc / ncps[2]) % ncps[1], c = lc % ncps[2];
+            const int gc[3] = {sidx[0] * ncps[0] + a, sidx[1] * ncps[1] + b, sidx[2] * ncps[2] + c};
+            const uint64_t id = ((((uint64_t)gc[0] * ncell[1] + gc[1]) * ncell[2] + gc[2]) * (uint64_t)ppc) + k;
+            const uint64_t key = or_mix64(seed ^ or_mix64(id));
+            const long p = s * per_sc + l;
+            for (int d = 0; d < 3; d++) x[d][p] = ((double)gc[d] + or_urand(key, d)) * step[d];
+            const double r1 = sqrt(-2.0 * or_det_log(1.0 - or_urand(key, 3)));
+            const double r2 = sqrt(-2.0 * or_det_log(1.0 - or_urand(key, 5)));
+            double s1, c1, s2, c2;
+            or_det_sincos_2pi(or_urand(key, 4), &s1, &c1);
+            or_det_sincos_2pi(or_urand(key, 6), &s2, &c2);
+            u[0][p] = uth * r1 * c1 + ((id & 1ull) ? -drift : drift);
+            u[1][p] = uth * r1 * s1;
+            u[2][p] = uth * r2 * c2;
+            w[p] = weight;
+            cell[p] = (gc[0] * ncell[1] + gc[1]) * ncell[2] + gc[2];
+        }
+    }
+}
+
+/* ln and sincos of the loader, exposed for their tests */
+double or_det_log_x(double x) { return or_det_log(x); }
+void or_det_sincos_2pi_x(double U, double* sn, double* cs) { or_det_sincos_2pi(U, sn, cs); }
 
 int or_version(void) { return 1; }

@@ -487,6 +487,7 @@ int pk_particle_step(pk_ens* e, pk_grid* g, double dt, int shape, int dep)
     r = e->prec == PK_F64 ? pk_particles_f64(e, g, dt, shape, dep) : pk_particles_split(e, g, dt, shape, dep);
     if (r) return r;
     e->pending_movers = true;
+    g->last_dt = dt;
     return PK_OK;
 }
 

@@ -9,7 +9,7 @@ namespace {
 inline unsigned nblk(long n, int b) { return (unsigned)((n + b - 1) / b); }
 constexpr int RB = 256;             // reduction block
 constexpr int RGRID = 1024;         // reduction blocks for grid sums
-constexpr int NR = 5;               // grid reduction slots
+constexpr int NR = 7;               // grid reduction slots
 
 // rho of one supercell's particles into an LDS tile (cell-centre nodes, base m+1, n+2 wide),
 // flushed x 1/V with FP64 atomics.  Shape weights as the oracle's or_deposit_rho.
@@ -108,17 +108,19 @@ __device__ __forceinline__ void block_max(double* sh, double v, double* out)
     __syncthreads();
 }
 
-// grid reductions over physical cells: [sum E^2+B^2, sum rho, max|r|, max|r - ref|, max|4 pi rho|]
-// with r = div E - 4 pi rho at the cell centre of array cell (i, j, k).
+// grid reductions over physical cells: [sum E^2+B^2, sum rho, max|r|, max|r - ref|, max|4 pi rho|,
+// max|c|, max|div J|] with r = div E - 4 pi rho and c = (rho - rho_prev) * inv_dt + div J at the cell
+// centre of array cell (i, j, k) (c only when rho_prev is given).
 template <class T>
 __global__ void __launch_bounds__(RB) k_grid_reduce(GridView<T> g, const double* rho, double* ref, int set_ref,
-                                                    int has_ref, double* partial)
+                                                    int has_ref, const double* rho_prev, double inv_dt,
+                                                    double* partial)
 {
     __shared__ double sh[RB];
     const int G = g.G;
     const int N0 = g.s0 - 2 * G, N1 = g.s1 - 2 * G, N2 = g.s2 - 2 * G;
     const long n = (long)N0 * N1 * N2;
-    double e2 = 0.0, sr = 0.0, mr = 0.0, mdr = 0.0, mrho = 0.0;
+    double e2 = 0.0, sr = 0.0, mr = 0.0, mdr = 0.0, mrho = 0.0, mc = 0.0, mdj = 0.0;
     const long s12 = (long)g.s1 * g.s2;
     for (long t = (long)blockIdx.x * RB + threadIdx.x; t < n; t += (long)gridDim.x * RB) {
         const int i = G + (int)(t / ((long)N1 * N2)), j = G + (int)((t / N2) % N1), k = G + (int)(t % N2);
@@ -141,12 +143,22 @@ __global__ void __launch_bounds__(RB) k_grid_reduce(GridView<T> g, const double*
         mrho = fmax(mrho, fabs(fr));
         if (set_ref) ref[t] = r;
         if (has_ref) mdr = fmax(mdr, fabs(r - ref[t]));
+        if (rho_prev) {
+            // div J with the same forward differences under which Esirkepov deposition conserves charge
+            const double divJ = ((double)g.f[PK_JX][c + s12] - (double)g.f[PK_JX][c]) / g.step[0] +
+                                ((double)g.f[PK_JY][c + g.s2] - (double)g.f[PK_JY][c]) / g.step[1] +
+                                ((double)g.f[PK_JZ][c + 1] - (double)g.f[PK_JZ][c]) / g.step[2];
+            mc = fmax(mc, fabs((rh - rho_prev[c]) * inv_dt + divJ));
+            mdj = fmax(mdj, fabs(divJ));
+        }
     }
     block_sum(sh, e2, &partial[blockIdx.x * NR + 0]);
     block_sum(sh, sr, &partial[blockIdx.x * NR + 1]);
     block_max(sh, mr, &partial[blockIdx.x * NR + 2]);
     block_max(sh, mdr, &partial[blockIdx.x * NR + 3]);
     block_max(sh, mrho, &partial[blockIdx.x * NR + 4]);
+    block_max(sh, mc, &partial[blockIdx.x * NR + 5]);
+    block_max(sh, mdj, &partial[blockIdx.x * NR + 6]);
 }
 
 // sum_p w * u^2 / (1 + gamma), per supercell partial
@@ -178,13 +190,11 @@ __global__ void __launch_bounds__(RB) k_kinetic(EnsGeom eg, PartF64 pf, PartSpli
 __global__ void k_final(const double* gp, int ngp, const double* kp, long nkp, double* out)
 {
     __shared__ double sh[RB];
-    double a[NR] = {0, 0, 0, 0, 0};
+    double a[NR] = {0, 0, 0, 0, 0, 0, 0};
     for (int b = threadIdx.x; b < ngp; b += RB) {
         a[0] += gp[b * NR + 0];
         a[1] += gp[b * NR + 1];
-        a[2] = fmax(a[2], gp[b * NR + 2]);
-        a[3] = fmax(a[3], gp[b * NR + 3]);
-        a[4] = fmax(a[4], gp[b * NR + 4]);
+        for (int k = 2; k < NR; k++) a[k] = fmax(a[k], gp[b * NR + k]);
     }
     double kin = 0.0;
     for (long b = threadIdx.x; b < nkp; b += RB) kin += kp[b];
@@ -194,6 +204,8 @@ __global__ void k_final(const double* gp, int ngp, const double* kp, long nkp, d
     block_max(sh, a[3], &out[3]);
     block_max(sh, a[4], &out[4]);
     block_sum(sh, kin, &out[5]);
+    block_max(sh, a[5], &out[6]);
+    block_max(sh, a[6], &out[7]);
 }
 
 __global__ void k_zero_d(double* p, long n)
@@ -211,7 +223,7 @@ __global__ void k_copy_to_grid(const double* src, T* dst, long n)
 
 int ensure_scratch(pk_grid* g, pk_ens* e)
 {
-    const size_t need = g->n + (size_t)RGRID * NR + (size_t)e->nsct + 16;
+    const size_t need = g->n + (size_t)RGRID * NR + (size_t)e->nsct + 16;   // 16: final results (8 used)
     if (g->scratch && g->scratch_n >= need) return PK_OK;
     if (g->scratch) (void)hipFree(g->scratch);
     g->scratch = nullptr;
@@ -302,12 +314,14 @@ int pk_diag_impl(pk_ens* e, pk_grid* g, int shape, double* out, bool set_ref)
     double* kp = gp + (size_t)RGRID * NR;
     double* res = kp + e->nsct;
     pk_ktimer tm(c, "diag");
+    const double* rp = g->last_dt > 0.0 ? g->rho_prev : nullptr;
+    const double inv_dt = g->last_dt > 0.0 ? 1.0 / g->last_dt : 0.0;
     if (g->prec == PK_F64)
         hipLaunchKernelGGL(k_grid_reduce<double>, dim3(RGRID), dim3(RB), 0, c->stream, grid_view<double>(g), g->scratch,
-                           g->gauss_ref, set_ref ? 1 : 0, (!set_ref && g->gauss_ref) ? 1 : 0, gp);
+                           g->gauss_ref, set_ref ? 1 : 0, (!set_ref && g->gauss_ref) ? 1 : 0, rp, inv_dt, gp);
     else
         hipLaunchKernelGGL(k_grid_reduce<float>, dim3(RGRID), dim3(RB), 0, c->stream, grid_view<float>(g), g->scratch,
-                           g->gauss_ref, set_ref ? 1 : 0, (!set_ref && g->gauss_ref) ? 1 : 0, gp);
+                           g->gauss_ref, set_ref ? 1 : 0, (!set_ref && g->gauss_ref) ? 1 : 0, rp, inv_dt, gp);
     EnsGeom eg = ens_geom(e);
     PartF64 pf{};
     PartSplit ps{};
@@ -322,7 +336,7 @@ int pk_diag_impl(pk_ens* e, pk_grid* g, int shape, double* out, bool set_ref)
     }
     hipLaunchKernelGGL(k_final, dim3(1), dim3(RB), 0, c->stream, gp, RGRID, kp, e->nsct, res);
     PK_HIP(c, hipGetLastError());
-    double h[6];
+    double h[8];
     PK_HIP(c, hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     PK_HIP(c, hipStreamSynchronize(c->stream));
     const double V = g->step[0] * g->step[1] * g->step[2];
@@ -333,5 +347,7 @@ int pk_diag_impl(pk_ens* e, pk_grid* g, int shape, double* out, bool set_ref)
     out[PK_DIAG_GAUSS_MAX] = h[2];
     out[PK_DIAG_GAUSS_DRIFT] = h[3];
     out[PK_DIAG_RHO_MAX] = h[4];
+    out[PK_DIAG_CONTINUITY_MAX] = h[6];
+    out[PK_DIAG_DIVJ_MAX] = h[7];
     return PK_OK;
 }

@@ -383,6 +383,80 @@ __device__ __forceinline__ double urand(uint64_t key, int j)
     return (double)(mix64(key + (uint64_t)j * 0xD1B54A32D192ED03ull) >> 11) * 0x1.0p-53;
 }
 
+// ln(x) for 0 < x <= 1 and (sin, cos)(2 pi U) for 0 <= U < 1 from IEEE +, -, *, / only (this file
+// is built without FP contraction), so that a CPU restatement of the loader with the same
+// operations reproduces every bit.  ln: x = m 2^e with m in [sqrt(1/2), sqrt(2)], ln m = 2 atanh(s),
+// s = (m - 1)/(m + 1), |s| <= 0.172, odd series to s^23 (truncation below 1e-19).  sincos: U in
+// octant q = floor(8U) at fraction f, reduced to r = f pi/4 or (1 - f) pi/4 in [0, pi/4], Taylor
+// series to r^19 / r^20 (truncation below 1e-21).
+__device__ __forceinline__ double det_log(double x)
+{
+    uint64_t bits = (uint64_t)__double_as_longlong(x);
+    int e = (int)((bits >> 52) & 0x7ff) - 1023;
+    double m = __longlong_as_double((long long)((bits & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull));
+    if (m > 1.4142135623730951) {
+        m = m * 0.5;
+        e += 1;
+    }
+    const double s = (m - 1.0) / (m + 1.0), s2 = s * s;
+    double p = 1.0 / 23.0;
+    p = 1.0 / 21.0 + s2 * p;
+    p = 1.0 / 19.0 + s2 * p;
+    p = 1.0 / 17.0 + s2 * p;
+    p = 1.0 / 15.0 + s2 * p;
+    p = 1.0 / 13.0 + s2 * p;
+    p = 1.0 / 11.0 + s2 * p;
+    p = 1.0 / 9.0 + s2 * p;
+    p = 1.0 / 7.0 + s2 * p;
+    p = 1.0 / 5.0 + s2 * p;
+    p = 1.0 / 3.0 + s2 * p;
+    p = 1.0 + s2 * p;
+    const double lnm = 2.0 * s * p;
+    const double de = (double)e;
+    return de * 6.93147180369123816490e-01 + (de * 1.90821492927058770002e-10 + lnm);
+}
+
+__device__ __forceinline__ void det_sincos_2pi(double U, double* sn, double* cs)
+{
+    const double t = U * 8.0;
+    const int q = (int)t;
+    const double f = t - (double)q;
+    const double r = ((q & 1) ? 1.0 - f : f) * 0.78539816339744830962;
+    const double r2 = r * r;
+    double ps = 1.0 / 121645100408832000.0;                // 1/19!
+    ps = 1.0 / 355687428096000.0 - r2 * ps;                  // 1/17!
+    ps = 1.0 / 1307674368000.0 - r2 * ps;                    // 1/15!
+    ps = 1.0 / 6227020800.0 - r2 * ps;                       // 1/13!
+    ps = 1.0 / 39916800.0 - r2 * ps;                         // 1/11!
+    ps = 1.0 / 362880.0 - r2 * ps;                           // 1/9!
+    ps = 1.0 / 5040.0 - r2 * ps;                             // 1/7!
+    ps = 1.0 / 120.0 - r2 * ps;                              // 1/5!
+    ps = 1.0 / 6.0 - r2 * ps;                                // 1/3!
+    const double sr = r - r * r2 * ps;
+    double pc = 1.0 / 2432902008176640000.0;               // 1/20!
+    pc = 1.0 / 6402373705728000.0 - r2 * pc;                 // 1/18!
+    pc = 1.0 / 20922789888000.0 - r2 * pc;                   // 1/16!
+    pc = 1.0 / 87178291200.0 - r2 * pc;                      // 1/14!
+    pc = 1.0 / 479001600.0 - r2 * pc;                        // 1/12!
+    pc = 1.0 / 3628800.0 - r2 * pc;                          // 1/10!
+    pc = 1.0 / 40320.0 - r2 * pc;                            // 1/8!
+    pc = 1.0 / 720.0 - r2 * pc;                              // 1/6!
+    pc = 1.0 / 24.0 - r2 * pc;                               // 1/4!
+    pc = 0.5 - r2 * pc;                                      // 1/2!
+    const double cr = 1.0 - r2 * pc;
+    // octant q: angle q pi/4 + f pi/4; odd octants measured back from the next multiple of pi/2
+    switch (q & 7) {
+    case 0: *sn = sr; *cs = cr; break;
+    case 1: *sn = cr; *cs = sr; break;
+    case 2: *sn = cr; *cs = -sr; break;
+    case 3: *sn = sr; *cs = -cr; break;
+    case 4: *sn = -sr; *cs = -cr; break;
+    case 5: *sn = -cr; *cs = -sr; break;
+    case 6: *sn = -cr; *cs = sr; break;
+    default: *sn = -sr; *cs = cr; break;
+    }
+}
+
 struct LoadParams {
     int ppc;
     double uth, drift, weight;
@@ -422,12 +496,15 @@ __device__ __forceinline__ void thermal_particle(const EnsGeom& eg, const LoadPa
     const uint64_t id = ((((uint64_t)gc[0] * eg.ncell[1] + gc[1]) * eg.ncell[2] + gc[2]) * lp.ppc) + k;
     const uint64_t key = mix64(lp.seed ^ mix64(id));
     U[0] = urand(key, 0); U[1] = urand(key, 1); U[2] = urand(key, 2);
-    // Box-Muller normals from streams 3..6
-    const double r1 = sqrt(-2.0 * log(1.0 - urand(key, 3))), a1 = 2.0 * PK_PI * urand(key, 4);
-    const double r2 = sqrt(-2.0 * log(1.0 - urand(key, 5))), a2 = 2.0 * PK_PI * urand(key, 6);
-    u[0] = lp.uth * r1 * cos(a1) + ((id & 1ull) ? -lp.drift : lp.drift);
-    u[1] = lp.uth * r1 * sin(a1);
-    u[2] = lp.uth * r2 * cos(a2);
+    // Box-Muller normals from streams 3..6 (1 - U is exact and > 0)
+    const double r1 = sqrt(-2.0 * det_log(1.0 - urand(key, 3)));
+    const double r2 = sqrt(-2.0 * det_log(1.0 - urand(key, 5)));
+    double s1, c1, s2, c2;
+    det_sincos_2pi(urand(key, 4), &s1, &c1);
+    det_sincos_2pi(urand(key, 6), &s2, &c2);
+    u[0] = lp.uth * r1 * c1 + ((id & 1ull) ? -lp.drift : lp.drift);
+    u[1] = lp.uth * r1 * s1;
+    u[2] = lp.uth * r2 * c2;
     for (int k = 0; k < lp.nmodes; k++) {
         const double* md = lp.modes + 7 * k;
         double ph = md[6];

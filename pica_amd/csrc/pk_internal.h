@@ -58,6 +58,7 @@ struct pk_grid {
     size_t scratch_n = 0;
     double* gauss_ref = nullptr;  // reference residual for PK_DIAG_GAUSS_DRIFT
     double* rho_prev = nullptr;   // saved FP64 charge density (n) of the continuity projection
+    double last_dt = 0.0;         // dt of the particle step whose current J holds (continuity diagnostic)
     double* proj = nullptr;       // projection work: complex physical volume + twiddles
     size_t proj_n = 0;
     size_t esize() const { return prec == PK_F64 ? 8 : 4; }

@@ -80,3 +80,35 @@ def test_gauss_law_drift(ctx, prec, n, steps, tol):
     print("gauss drift (prec=%d n=%d steps=%d): %.3g, charge change %.3g" % (prec, n, steps, drift, q_rel), flush=True)
     assert drift < tol
     assert q_rel < (1e-12 if prec == P.F64 else 1e-5)
+
+
+@pytest.mark.parametrize("prec,tol", [(P.F64, 1e-11), (P.F32, 2e-4)])
+@pytest.mark.parametrize("shape", [P.CIC, P.TSC])
+def test_device_continuity_residual(ctx, prec, tol, shape):
+    """pk_diag's PK_DIAG_CONTINUITY_MAX after pk_rho_save + one step equals the residual the host
+    computes from the downloaded rho and J; Esirkepov keeps it at round-off, direct does not"""
+    _say("device continuity prec=%d shape=%d" % (prec, shape))
+    for dep in (P.ESIRKEPOV, P.DIRECT):
+        dt, g, ens = _run(ctx, prec, 32, 8, shape, dep, 0)
+        P.run(ens, g, dt, 2, shape, dep)
+        P.deposit_rho(ens, g, shape)
+        rho0 = g.get(P.RHO).astype(np.float64)
+        P.rho_save(ens, g, shape)
+        P.step(ens, g, dt, shape, dep)
+        d = P.diag(ens, g, shape)
+        P.deposit_rho(ens, g, shape)
+        rho1 = g.get(P.RHO).astype(np.float64)
+        jx, jy, jz = (g.get(c).astype(np.float64) for c in (P.JX, P.JY, P.JZ))
+        divj = O.physical_nodes(O.divergence(jx, jy, jz, (1.0, 1.0, 1.0)))
+        host = np.abs(O.physical_nodes(((rho1 - rho0) / dt)[:-1, :-1, :-1]) + divj).max()
+        rel = d[P.DIAG_CONTINUITY_MAX] / d[P.DIAG_DIVJ_MAX]
+        print("device continuity (prec=%d shape=%d dep=%d): %.3g (host %.3g), max|div J| %.3g vs host %.3g" %
+              (prec, shape, dep, rel, host / np.abs(divj).max(), d[P.DIAG_DIVJ_MAX], np.abs(divj).max()), flush=True)
+        assert d[P.DIAG_DIVJ_MAX] == pytest.approx(np.abs(divj).max(), rel=1e-6 if prec == P.F32 else 1e-12)
+        if dep == P.ESIRKEPOV:
+            assert rel < tol
+        else:
+            assert rel > 1e-3
+            assert d[P.DIAG_CONTINUITY_MAX] == pytest.approx(host, rel=1e-3)
+        ens.close()
+        g.close()
