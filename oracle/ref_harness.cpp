@@ -35,6 +35,8 @@ int numTypes = 0;
 using namespace pica;
 
 typedef YeeGrid<Three, double> Grid3;
+typedef YeeGrid<One, double> Grid1;
+typedef YeeGrid<Two, double> Grid2;
 typedef YeeGrid<Three, float> Grid3f;
 typedef ParticleArraySoA<Particle3d> PArray;
 typedef Ensemble<PArray, EnsembleRepresentation_Supercells>::Type PEnsemble;
@@ -67,6 +69,62 @@ void store_grid(const G& g, T* const f[9])
                 f[3][c] = g.bx(i, j, k); f[4][c] = g.by(i, j, k); f[5][c] = g.bz(i, j, k);
                 f[6][c] = g.jx(i, j, k); f[7][c] = g.jy(i, j, k); f[8][c] = g.jz(i, j, k);
             }
+}
+
+// reduced-dimension grids: arrays of s[0] (1D) or s[0] x s[1] (2D, y fastest) values
+void load_grid1(Grid1& g, double* const f[9])
+{
+    const int n = g.getSize().x;
+    for (int i = 0; i < n; i++) {
+        g.ex(i) = f[0][i]; g.ey(i) = f[1][i]; g.ez(i) = f[2][i];
+        g.bx(i) = f[3][i]; g.by(i) = f[4][i]; g.bz(i) = f[5][i];
+        g.jx(i) = f[6][i]; g.jy(i) = f[7][i]; g.jz(i) = f[8][i];
+    }
+}
+
+void store_grid1(const Grid1& g, double* const f[9])
+{
+    const int n = g.getSize().x;
+    for (int i = 0; i < n; i++) {
+        f[0][i] = g.ex(i); f[1][i] = g.ey(i); f[2][i] = g.ez(i);
+        f[3][i] = g.bx(i); f[4][i] = g.by(i); f[5][i] = g.bz(i);
+        f[6][i] = g.jx(i); f[7][i] = g.jy(i); f[8][i] = g.jz(i);
+    }
+}
+
+void load_grid2(Grid2& g, double* const f[9])
+{
+    const Grid2::IndexType s = g.getSize();
+    for (int i = 0; i < s.x; i++)
+        for (int j = 0; j < s.y; j++) {
+            const long c = (long)i * s.y + j;
+            g.ex(i, j) = f[0][c]; g.ey(i, j) = f[1][c]; g.ez(i, j) = f[2][c];
+            g.bx(i, j) = f[3][c]; g.by(i, j) = f[4][c]; g.bz(i, j) = f[5][c];
+            g.jx(i, j) = f[6][c]; g.jy(i, j) = f[7][c]; g.jz(i, j) = f[8][c];
+        }
+}
+
+void store_grid2(const Grid2& g, double* const f[9])
+{
+    const Grid2::IndexType s = g.getSize();
+    for (int i = 0; i < s.x; i++)
+        for (int j = 0; j < s.y; j++) {
+            const long c = (long)i * s.y + j;
+            f[0][c] = g.ex(i, j); f[1][c] = g.ey(i, j); f[2][c] = g.ez(i, j);
+            f[3][c] = g.bx(i, j); f[4][c] = g.by(i, j); f[5][c] = g.bz(i, j);
+            f[6][c] = g.jx(i, j); f[7][c] = g.jy(i, j); f[8][c] = g.jz(i, j);
+        }
+}
+
+Grid1 make_grid1(const int s[1], const double origin[1], const double step[1])
+{
+    return Grid1(Grid1::PositionType(origin[0]), Grid1::PositionType(step[0]), Grid1::IndexType(s[0]));
+}
+
+Grid2 make_grid2(const int s[2], const double origin[2], const double step[2])
+{
+    return Grid2(Grid2::PositionType(origin[0], origin[1]), Grid2::PositionType(step[0], step[1]),
+                 Grid2::IndexType(s[0], s[1]));
 }
 
 void set_species(double q, double m)
@@ -173,6 +231,68 @@ void ref_yee_update_b_f32(const int s[3], const float origin[3], const float ste
     YeeSolver solver;
     solver.updateB(g, dt);
     store_grid(g, f);
+}
+
+// YeeSolver on pica's reduced-dimension grids (YeeSolver.h:41-128); op 0 = updateE, 1 = updateB
+void ref_yee_update_1d(const int s[1], const double origin[1], const double step[1], double* f[9], double dt, int op)
+{
+    Grid1 g = make_grid1(s, origin, step);
+    load_grid1(g, f);
+    YeeSolver solver;
+    if (op == 0) solver.updateE(g, dt);
+    else solver.updateB(g, dt);
+    store_grid1(g, f);
+}
+
+void ref_yee_update_2d(const int s[2], const double origin[2], const double step[2], double* f[9], double dt, int op)
+{
+    Grid2 g = make_grid2(s, origin, step);
+    load_grid2(g, f);
+    YeeSolver solver;
+    if (op == 0) solver.updateE(g, dt);
+    else solver.updateB(g, dt);
+    store_grid2(g, f);
+}
+
+// FieldInterpolatorCIC<YeeGrid<One,double>>::get (FieldInterpolatorYeeGrid.h:51-83).  The Two
+// specialisation (:86-133) does not compile: its get() builds PositionType (a Vector2) from
+// coeffStaggered.z / coeffCollocated.z (:108-118), members a Vector2 lacks.
+void ref_interp_cic_1d(const int s[1], const double origin[1], const double step[1], double* f[9], long n,
+                       const double* pos, double* e, double* b)
+{
+    Grid1 g = make_grid1(s, origin, step);
+    load_grid1(g, f);
+    FieldInterpolatorCIC<Grid1> interp(g);
+    for (long p = 0; p < n; p++) {
+        Vector3<double> ev, bv;
+        interp.get(Grid1::PositionType(pos[p]), ev, bv);
+        e[3 * p] = ev.x; e[3 * p + 1] = ev.y; e[3 * p + 2] = ev.z;
+        b[3 * p] = bv.x; b[3 * p + 1] = bv.y; b[3 * p + 2] = bv.z;
+    }
+}
+
+// CurrentDepositorCIC<YeeGrid<One|Two,double>>::deposit (CurrentDepositorYeeGrid.h:52-122); cur is n x 3
+void ref_deposit_cic_1d(const int s[1], const double origin[1], const double step[1], double* f[9], long n,
+                        const double* pos, const double* cur)
+{
+    Grid1 g = make_grid1(s, origin, step);
+    load_grid1(g, f);
+    CurrentDepositorCIC<Grid1> dep(g);
+    for (long p = 0; p < n; p++)
+        dep.deposit(Grid1::PositionType(pos[p]), Vector3<double>(cur[3 * p], cur[3 * p + 1], cur[3 * p + 2]));
+    store_grid1(g, f);
+}
+
+void ref_deposit_cic_2d(const int s[2], const double origin[2], const double step[2], double* f[9], long n,
+                        const double* pos, const double* cur)
+{
+    Grid2 g = make_grid2(s, origin, step);
+    load_grid2(g, f);
+    CurrentDepositorCIC<Grid2> dep(g);
+    for (long p = 0; p < n; p++)
+        dep.deposit(Grid2::PositionType(pos[2 * p], pos[2 * p + 1]),
+                    Vector3<double>(cur[3 * p], cur[3 * p + 1], cur[3 * p + 2]));
+    store_grid2(g, f);
 }
 
 // FieldInterpolatorCIC<YeeGrid<Three,double>>::get for n positions
