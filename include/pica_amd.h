@@ -252,6 +252,14 @@ int pk_deposit_rho(pk_ens* e, pk_grid* g, int shape);
 int pk_diag(pk_ens* e, pk_grid* g, int shape, double* out);
 /* store the current div E - 4 pi rho as the reference for PK_DIAG_GAUSS_DRIFT */
 int pk_diag_set_reference(pk_ens* e, pk_grid* g, int shape);
+/* No counterpart in pica: for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  out[0] = PK_DIAG_FIELD_ENERGY, out[1] = PK_DIAG_KINETIC_ENERGY,
+ * bit-identical to pk_diag's, computed without any allocation: the reduction partials live in a
+ * static device array of 8.1 MB that each GPU holds for the life of the process.  Limits: at most
+ * 1,048,576 supercells (else PK_E_CAPACITY); the ensemble must have no pending movers (a step
+ * finished by pk_step or pk_ens_resort; else PK_E_STATE).  Calls share the GPU's array: calls
+ * that overlap on one GPU (several contexts or threads on the same device) race. */
+int pk_diag_energies_static(pk_ens* e, pk_grid* g, double* out);
 
 #ifdef __cplusplus
 }

@@ -56,7 +56,7 @@ ABI_SYMBOLS = (
     "pk_dn_num_params", "pk_dn_init", "pk_dn_set_params", "pk_dn_get_params", "pk_dn_set_scale", "pk_dn_forward_host",
     "pk_dn_layer_host", "pk_dn_grad_host", "pk_dn_grad_batch_host", "pk_dn_add_pair", "pk_dn_num_pairs",
     "pk_dn_clear_pairs", "pk_dn_train", "pk_dn_apply", "pk_rho_save", "pk_project_continuity", "pk_step_denoised", "pk_grid_save", "pk_grid_load",
-    "pk_ens_save", "pk_ens_load", "pk_step_species", "pk_ens_interleave",
+    "pk_ens_save", "pk_ens_load", "pk_step_species", "pk_ens_interleave", "pk_diag_energies_static",
 )
 
 _LIB = None
@@ -122,6 +122,7 @@ def lib():
         L.pk_deposit_rho.argtypes = [vp, vp, ctypes.c_int]
         L.pk_diag.argtypes = [vp, vp, ctypes.c_int, vp]
         L.pk_diag_set_reference.argtypes = [vp, vp, ctypes.c_int]
+        L.pk_diag_energies_static.argtypes = [vp, vp, vp]
         L.pk_ens_load_modes.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_uint64, ctypes.c_int, vp]
         L.pk_dn_create.argtypes = [vp, ctypes.POINTER(vp)]
@@ -594,6 +595,14 @@ def deposit_rho(ens, grid, shape=CIC):
 def diag(ens, grid, shape=CIC):
     out = np.zeros(DIAG_N)
     ens.ctx.check(lib().pk_diag(ens.h, grid.h, shape, _p(out)))
+    return out
+
+
+def diag_energies_static(ens, grid):
+    """(field energy, kinetic energy) as pk_diag gives them, computed without any device allocation
+    (pk_diag_energies_static: a static per-GPU scratch; at most 1,048,576 supercells)"""
+    out = np.zeros(2)
+    ens.ctx.check(lib().pk_diag_energies_static(ens.h, grid.h, _p(out)))
     return out
 
 

@@ -116,6 +116,11 @@ int pk_ctx_create(int device, pk_ctx** out)
     // default species table: one electron species (Constants.h:12-13)
     c->mass.push_back(9.10938215e-28);
     c->charge.push_back(-4.80320427e-10);
+    // this device's static diagnostics scratch (pk_diag_energies_static), resolved once per device
+    if (pk_diag_static_init(c, device) != PK_OK) {
+        pk_ctx_destroy(c);
+        return PK_E_HIP;
+    }
     *out = c;
     return PK_OK;
 }
@@ -557,6 +562,19 @@ int pk_diag(pk_ens* e, pk_grid* g, int shape, double* out)
     if (!out) return PK_E_ARG;
     if ((r = pk_ens_resort(e))) return r;
     return pk_diag_impl(e, g, shape, out, false);
+}
+
+int pk_diag_energies_static(pk_ens* e, pk_grid* g, double* out)
+{
+    int r = pk_check_pair(e, g);
+    if (r) return r;
+    if (!out) return PK_E_ARG;
+    if ((r = pk_ens_usable(e))) return r;
+    if (e->pending_movers)
+        return pk_fail(e->ctx, PK_E_STATE,
+                       "pk_diag_energies_static: movers pending (a re-sort may allocate); call pk_ens_resort or "
+                       "finish the step with pk_step first");
+    return pk_diag_energies_static_impl(e, g, out);
 }
 
 int pk_diag_set_reference(pk_ens* e, pk_grid* g, int shape)

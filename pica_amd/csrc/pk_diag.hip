@@ -132,6 +132,7 @@ __global__ void __launch_bounds__(RB) k_grid_reduce(GridView<T> g, const double*
             s += v * v;
         }
         e2 += s;
+        if (!rho) continue;    // field energy only
         const double rh = rho[c];
         sr += rh;
         const double divE = ((double)g.f[PK_EX][c + s12] - (double)g.f[PK_EX][c]) / g.step[0] +
@@ -221,6 +222,17 @@ __global__ void k_copy_to_grid(const double* src, T* dst, long n)
     if (i < n) dst[i] = (T)src[i];
 }
 
+// Scratch of pk_diag_energies_static: a file-scope device array each GPU holds once for the
+// life of the process (no allocation at call time).  Layout: RGRID * NR grid partials, then
+// PK_STATIC_MAX_SUPERCELLS kinetic partials, then 16 results.
+constexpr long PK_STATIC_MAX_SUPERCELLS = 1L << 20;
+constexpr long STATIC_SCRATCH_N = (long)RGRID * NR + PK_STATIC_MAX_SUPERCELLS + 16;
+__device__ __align__(256) double scratch[STATIC_SCRATCH_N];
+
+// the device address of `scratch` per device, taken once with that device current
+constexpr int MAX_DEVICES = 64;
+double* g_static_scratch[MAX_DEVICES] = {};
+
 int ensure_scratch(pk_grid* g, pk_ens* e)
 {
     const size_t need = g->n + (size_t)RGRID * NR + (size_t)e->nsct + 16;   // 16: final results (8 used)
@@ -257,6 +269,20 @@ void launch_rho(pk_ens* e, pk_grid* g, double* rho)
 }
 
 }  // namespace
+
+// Called by pk_ctx_create with `device` current: resolves this device's copy of `scratch`.
+int pk_diag_static_init(pk_ctx* c, int device)
+{
+    if (device < 0 || device >= MAX_DEVICES)
+        return pk_fail(c, PK_E_UNSUPPORTED, "pk_ctx_create: device index beyond the static scratch table");
+    if (g_static_scratch[device]) return PK_OK;
+    void* ptr = nullptr;
+    const hipError_t er = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (er != hipSuccess || !ptr)
+        return pk_fail(c, PK_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(er));
+    g_static_scratch[device] = (double*)ptr;
+    return PK_OK;
+}
 
 // FP64 rho (folded, filled) into g->scratch[0 .. n)
 int pk_rho_fp64(pk_ens* e, pk_grid* g, int shape)
@@ -295,6 +321,54 @@ int pk_rho_impl(pk_ens* e, pk_grid* g, int shape)
                            (float*)g->comp(PK_RHO), n);
     PK_HIP(e->ctx, hipGetLastError());
     PK_HIP(e->ctx, hipStreamSynchronize(e->ctx->stream));
+    return PK_OK;
+}
+
+// Field and kinetic energy with pk_diag's kernels and reduction order (bit-identical to its
+// PK_DIAG_FIELD_ENERGY / PK_DIAG_KINETIC_ENERGY), on the device's static scratch.  Calls on one
+// GPU share that scratch, so calls that overlap on one GPU race; every value a call reads was
+// written by that call (the partial and result slots are written in full), so nothing needs
+// clearing between calls.
+int pk_diag_energies_static_impl(pk_ens* e, pk_grid* g, double out[2])
+{
+    pk_ctx* c = e->ctx;
+    if (e->nsct > PK_STATIC_MAX_SUPERCELLS)
+        return pk_fail(c, PK_E_CAPACITY,
+                       "pk_diag_energies_static: more than 1,048,576 supercells (the static scratch holds one "
+                       "kinetic partial per supercell); use pk_diag");
+    double* base = (c->device >= 0 && c->device < MAX_DEVICES) ? g_static_scratch[c->device] : nullptr;
+    if (!base) return pk_fail(c, PK_E_STATE, "pk_diag_energies_static: static scratch not resolved for this device");
+    double* gp = base;
+    double* kp = gp + (size_t)RGRID * NR;
+    double* res = kp + PK_STATIC_MAX_SUPERCELLS;
+    if (g->prec == PK_F64)
+        hipLaunchKernelGGL(k_grid_reduce<double>, dim3(RGRID), dim3(RB), 0, c->stream, grid_view<double>(g), nullptr,
+                           nullptr, 0, 0, nullptr, 0.0, gp);
+    else
+        hipLaunchKernelGGL(k_grid_reduce<float>, dim3(RGRID), dim3(RB), 0, c->stream, grid_view<float>(g), nullptr,
+                           nullptr, 0, 0, nullptr, 0.0, gp);
+    EnsGeom eg = ens_geom(e);
+    PartF64 pf{};
+    PartSplit ps{};
+    if (e->nsct > 0) {
+        if (e->prec == PK_F64) {
+            pf = part_f64(e, e->main);
+            hipLaunchKernelGGL(k_kinetic<false>, dim3((unsigned)e->nsct), dim3(RB), 0, c->stream, eg, pf, ps, e->off,
+                               e->cnt, kp);
+        } else {
+            ps = part_split(e, e->main);
+            hipLaunchKernelGGL(k_kinetic<true>, dim3((unsigned)e->nsct), dim3(RB), 0, c->stream, eg, pf, ps, e->off,
+                               e->cnt, kp);
+        }
+    }
+    hipLaunchKernelGGL(k_final, dim3(1), dim3(RB), 0, c->stream, gp, RGRID, kp, e->nsct, res);
+    PK_HIP(c, hipGetLastError());
+    double h[8];
+    PK_HIP(c, hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    PK_HIP(c, hipStreamSynchronize(c->stream));
+    const double V = g->step[0] * g->step[1] * g->step[2];
+    out[0] = h[0] * V / (8.0 * PK_PI);
+    out[1] = h[5] * c->mass[e->type] * PK_C * PK_C;
     return PK_OK;
 }
 

@@ -274,5 +274,7 @@ int pk_ens_load_impl(pk_ens* e, int ppc, double uth, double drift, double weight
 int pk_ens_alloc(pk_ens* e, size_t capacity);
 int pk_check_flags(pk_ens* e, bool* overflow);
 int pk_diag_impl(pk_ens* e, pk_grid* g, int shape, double* out, bool set_ref);
+int pk_diag_static_init(pk_ctx* c, int device);
+int pk_diag_energies_static_impl(pk_ens* e, pk_grid* g, double out[2]);
 int pk_rho_impl(pk_ens* e, pk_grid* g, int shape);
 int pk_rho_fp64(pk_ens* e, pk_grid* g, int shape);   // folded, filled FP64 rho in g->scratch[0, n)
