@@ -138,6 +138,11 @@ int pk_ens_cells(pk_ens* e, int32_t* cell_counts, int32_t* cell_of_particle);
 /* reorder every supercell of an FP32 ensemble now, as PK_OPT_INTERLEAVE does periodically
  * (no counterpart in pica); a no-op for FP64 ensembles */
 int pk_ens_interleave(pk_ens* e);
+/* re-bucket every particle into supercells of ncps cells (ncell must be a multiple; at most 12
+ * per side), as if the ensemble had been created with them (EnsembleSupercells.h:25-34 takes the
+ * supercell size at construction; pica cannot change it).  Order inside a new supercell: by old
+ * supercell, then old slot (FP32 ensembles are then interleaved).  Used by pk_batch_autotune. */
+int pk_ens_set_supercells(pk_ens* e, const int ncps[3]);
 /* seeded thermal loader on the device: ppc particles per cell at uniform random positions,
  * u ~ N(0, uth) per component, plus +drift_x on even and -drift_x on odd particle ids. */
 int pk_ens_load_thermal(pk_ens* e, int ppc, double uth, double drift_x, double weight, uint64_t seed);
@@ -171,16 +176,29 @@ enum {
     PK_BATCH_WG_PER_CU = 0,   /* particle kernel: 0 = one workgroup per item, k = persistent, k per CU */
     PK_BATCH_ORDER = 1,       /* items: 0 = ensemble by ensemble, 1 = most particles first */
     PK_BATCH_GRAPH = 2,       /* 1 = replay a captured HIP graph per step, 0 = launch eagerly */
+    PK_BATCH_SUPERCELL = 3,   /* k > 0: every ensemble re-bucketed into k^3 supercells (pk_ens_set_supercells)
+                                 when set; 0 = the supercells the ensembles had when the batch was created */
+    PK_BATCH_INTERLEAVE = 4,  /* FP32 interleave period of the batched step: -1 = the context's
+                                 PK_OPT_INTERLEAVE, 0 = off, k = every k-th re-sort */
     PK_BATCH_NKNOB
 };
+/* shape argument of pk_batch_run / pk_batch_autotune: every ensemble with its own form factor, set
+ * by pk_batch_set_shape (the particle stage then takes one launch per form factor present) */
+enum { PK_SHAPE_PER_INSTANCE = 2 };
 int pk_batch_create(pk_ctx* ctx, int n, pk_ens* const* ens, pk_grid* const* grids, pk_batch** out);
 void pk_batch_destroy(pk_batch* b);
 int pk_batch_set(pk_batch* b, int knob, int value);
+/* form factor (PK_CIC / PK_TSC) of ensemble `index` of the batch, used when a run passes
+ * PK_SHAPE_PER_INSTANCE */
+int pk_batch_set_shape(pk_batch* b, int index, int shape);
 int pk_batch_get(pk_batch* b, int knob, int* value);
 /* nsteps periodic PIC steps (pk_step's sequence) of every ensemble of the batch */
 int pk_batch_run(pk_batch* b, double dt, int shape, int dep, int nsteps);
-/* autotuner: times trial_steps steps of every knob setting from the current state, restoring the
- * state after each trial, and keeps the fastest setting (its ms per step in *best_ms) */
+/* autotuner: from the current state, restored after every trial, times trial_steps steps of
+ * (1) every supercell size 2^3, 4^3, 8^3 that divides all grids (and the batch's original one),
+ * (2) FP32: interleave periods 0, 2, 3, 4, 6, 8, (3) every (graph, order, workgroups per CU) of
+ * {0,1} x {0,1} x {0,1,2,4,8}, each stage with the best of the previous ones, and keeps the fastest
+ * setting (its ms per step in *best_ms; the ensembles end re-bucketed to the chosen supercells) */
 int pk_batch_autotune(pk_batch* b, double dt, int shape, int dep, int trial_steps, double* best_ms);
 
 /* ---- learned form (no counterpart in pica) ----

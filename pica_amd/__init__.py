@@ -34,7 +34,8 @@ FILL_E, FILL_B, FILL_J, FOLD_J, FOLD_RHO = 1, 2, 4, 8, 16
  DIAG_CONTINUITY_MAX, DIAG_DIVJ_MAX) = range(8)
 DIAG_N = 8
 OPT_PARTICLE_KERNEL, OPT_CHANGE_QUEUE, OPT_INTERLEAVE = 0, 1, 2
-BATCH_WG_PER_CU, BATCH_ORDER, BATCH_GRAPH = 0, 1, 2
+BATCH_WG_PER_CU, BATCH_ORDER, BATCH_GRAPH, BATCH_SUPERCELL, BATCH_INTERLEAVE = 0, 1, 2, 3, 4
+SHAPE_PER_INSTANCE = 2
 KERNEL_AUTO, KERNEL_PER_PARTICLE, KERNEL_CELL_SORTED = 0, 1, 2
 
 # pica's CGS constants (Constants.h:10-13)
@@ -57,6 +58,7 @@ ABI_SYMBOLS = (
     "pk_dn_layer_host", "pk_dn_grad_host", "pk_dn_grad_batch_host", "pk_dn_add_pair", "pk_dn_num_pairs",
     "pk_dn_clear_pairs", "pk_dn_train", "pk_dn_apply", "pk_rho_save", "pk_project_continuity", "pk_step_denoised", "pk_grid_save", "pk_grid_load",
     "pk_ens_save", "pk_ens_load", "pk_step_species", "pk_ens_interleave", "pk_diag_energies_static",
+    "pk_ens_set_supercells", "pk_batch_set_shape",
 )
 
 _LIB = None
@@ -144,6 +146,8 @@ def lib():
         L.pk_rho_save.argtypes = [vp, vp, ctypes.c_int]
         L.pk_step_species.argtypes = [vp, ctypes.c_int, vp, ctypes.c_double, ctypes.c_int, ctypes.c_int]
         L.pk_ens_interleave.argtypes = [vp]
+        L.pk_ens_set_supercells.argtypes = [vp, vp]
+        L.pk_batch_set_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         for name in ("pk_grid_save", "pk_grid_load", "pk_ens_save", "pk_ens_load"):
             getattr(L, name).argtypes = [vp, ctypes.c_char_p]
         L.pk_project_continuity.argtypes = [vp, vp, ctypes.c_int, ctypes.c_double]
@@ -394,6 +398,11 @@ class EnsembleSupercells:
         """reorder each supercell (FP32) so that consecutive particles sit in different cells"""
         self.ctx.check(lib().pk_ens_interleave(self.h))
 
+    def set_supercells(self, ncps):
+        """re-bucket every particle into supercells of ncps cells (pk_ens_set_supercells)"""
+        self.ctx.check(lib().pk_ens_set_supercells(self.h, _i3(ncps)))
+        self.ncps = tuple(int(v) for v in ncps)
+
     def save(self, path):
         """raw snapshot of the particles in supercell order (pica_amd.snapshot reads it)"""
         self.ctx.check(lib().pk_ens_save(self.h, os.fsencode(path)))
@@ -440,12 +449,17 @@ class Batch:
         self.ctx.check(lib().pk_batch_get(self.h, int(knob), ctypes.byref(v)))
         return v.value
 
+    def set_shape(self, index, shape):
+        """form factor of pair `index`, used by runs with shape=SHAPE_PER_INSTANCE"""
+        self.ctx.check(lib().pk_batch_set_shape(self.h, int(index), int(shape)))
+
     def run(self, dt, nsteps, shape=CIC, dep=DIRECT):
         self.ctx.check(lib().pk_batch_run(self.h, float(dt), shape, dep, int(nsteps)))
 
     def autotune(self, dt, shape=CIC, dep=DIRECT, trial_steps=3):
-        """times every knob setting from the current state (restored after each trial), keeps the
-        fastest; returns its ms per step"""
+        """from the current state (restored after every trial) times supercell sizes, FP32 interleave
+        periods and launch settings, keeps the fastest (the ensembles end re-bucketed to its supercell
+        size); returns its ms per step"""
         ms = ctypes.c_double()
         self.ctx.check(lib().pk_batch_autotune(self.h, float(dt), shape, dep, int(trial_steps), ctypes.byref(ms)))
         return ms.value

@@ -34,8 +34,9 @@ constexpr unsigned MASK_B = (1u << PK_BX) | (1u << PK_BY) | (1u << PK_BZ);
 constexpr unsigned MASK_J = (1u << PK_JX) | (1u << PK_JY) | (1u << PK_JZ);
 
 // (ensemble, supercell) items: ensemble by ensemble in the XCD-aware supercell order of the
-// single-ensemble kernels, or all items by decreasing particle count (largest first)
-int build_items(pk_batch* b)
+// single-ensemble kernels, or all items by decreasing particle count (largest first); and the
+// particle kernel's items split by form factor in the same order
+int build_items(pk_batch* b, int shape)
 {
     std::vector<int2> items;
     std::vector<int32_t> key;
@@ -60,6 +61,12 @@ int build_items(pk_batch* b)
         items.swap(sorted);
     }
     b->nitems = (long)items.size();
+    std::vector<int2> cls[2];
+    for (const int2& it : items) cls[pk_batch_shape_of(b, it.x, shape)].push_back(it);
+    for (int k = 0; k < 2; k++) {
+        b->cnitems[k] = (long)cls[k].size();
+        if (int r = pk_batch_upload(b, (void**)&b->citems[k], cls[k].data(), sizeof(int2) * cls[k].size())) return r;
+    }
     return pk_batch_upload(b, (void**)&b->items, items.data(), sizeof(int2) * items.size());
 }
 
@@ -68,7 +75,7 @@ int enqueue_step(pk_batch* b)
 {
     int r;
     if ((r = pk_batch_fields_launch(b, PK_BOP_ZERO_J, 0, 0))) return r;
-    r = b->prec == PK_F64 ? pk_batch_f64_launch(b, b->shape, b->dep) : pk_batch_split_launch(b, b->shape, b->dep);
+    r = b->prec == PK_F64 ? pk_batch_f64_launch(b, b->dep) : pk_batch_split_launch(b, b->dep);
     if (r) return r;
     for (int d = 0; d < 3; d++)
         if ((r = pk_batch_fields_launch(b, PK_BOP_FOLD, MASK_J, d))) return r;
@@ -98,7 +105,7 @@ int prepare(pk_batch* b, double dt, int shape, int dep)
     PK_HIP(c, hipStreamSynchronize(c->stream));
     drop_graph(b);
     int r;
-    if ((r = build_items(b))) return r;
+    if ((r = build_items(b, shape))) return r;
     if ((r = pk_batch_fields_prepare(b, dt))) return r;
     r = b->prec == PK_F64 ? pk_batch_f64_prepare(b, dt, shape, dep) : pk_batch_split_prepare(b, dt, shape, dep);
     if (r) return r;
@@ -133,8 +140,14 @@ int prepare(pk_batch* b, double dt, int shape, int dep)
 
 int check_args(pk_batch* b, double dt, int shape, int dep)
 {
-    if ((shape != PK_CIC && shape != PK_TSC) || (dep != PK_DIRECT && dep != PK_ESIRKEPOV))
+    if ((shape != PK_CIC && shape != PK_TSC && shape != PK_SHAPE_PER_INSTANCE) ||
+        (dep != PK_DIRECT && dep != PK_ESIRKEPOV))
         return pk_fail(b->ctx, PK_E_ARG, "pk_batch_run: unknown shape or depositor");
+    if (shape == PK_SHAPE_PER_INSTANCE)
+        for (int s : b->shapes)
+            if (s != PK_CIC && s != PK_TSC)
+                return pk_fail(b->ctx, PK_E_ARG, "pk_batch_run: PK_SHAPE_PER_INSTANCE needs pk_batch_set_shape for "
+                                                 "every ensemble");
     if (!(dt > 0.0)) return pk_fail(b->ctx, PK_E_ARG, "pk_batch_run: dt must be positive");
     return PK_OK;
 }
@@ -145,7 +158,9 @@ int run(pk_batch* b, double dt, int shape, int dep, int nsteps)
     int r;
     for (pk_ens* e : b->ens)
         if ((r = pk_ens_resort(e))) return r;
-    const int period = b->ens[0]->posmode == PK_POS_SPLIT ? c->opt[PK_OPT_INTERLEAVE] : 0;
+    const int period = b->ens[0]->posmode == PK_POS_SPLIT
+                           ? (b->knob[PK_BATCH_INTERLEAVE] >= 0 ? b->knob[PK_BATCH_INTERLEAVE] : c->opt[PK_OPT_INTERLEAVE])
+                           : 0;
     if (b->dirty || dt != b->dt || shape != b->shape || dep != b->dep || period != b->ilperiod)
         if ((r = prepare(b, dt, shape, dep))) return r;
     // re-sort counts before this run; each pass tells the interleave kernel where it starts
@@ -220,6 +235,7 @@ struct Snapshot {
         int maxcap = 0;
         int64_t np = 0;
         long resorts = 0;
+        int ncps[3] = {0, 0, 0};
     };
     std::vector<Ens> ens;
     std::vector<void*> grids;
@@ -261,6 +277,7 @@ int snapshot_take(pk_batch* b, Snapshot& S)
         s.maxcap = e->maxcap;
         s.np = e->np;
         s.resorts = e->resorts;
+        for (int d = 0; d < 3; d++) s.ncps[d] = e->ncps[d];
     }
     PK_HIP(c, hipStreamSynchronize(c->stream));
     return PK_OK;
@@ -274,6 +291,9 @@ int snapshot_restore(pk_batch* b, const Snapshot& S)
         pk_ens* e = b->ens[i];
         const Snapshot::Ens& s = S.ens[i];
         int r;
+        if ((e->ncps[0] != s.ncps[0] || e->ncps[1] != s.ncps[1] || e->ncps[2] != s.ncps[2]) &&
+            (r = pk_ens_reshape(e, s.ncps)))
+            return r;
         if (e->capacity != s.capacity && (r = pk_ens_alloc(e, s.capacity))) return r;
         const size_t mb = e->capacity * e->nfield() * e->esize();
         PK_HIP(c, hipMemcpyAsync(e->main, s.main, mb, hipMemcpyDeviceToDevice, c->stream));
@@ -285,6 +305,7 @@ int snapshot_restore(pk_batch* b, const Snapshot& S)
         e->np = s.np;
         e->resorts = s.resorts;
         e->pending_movers = false;
+        e->failed = false;
         if ((r = pk_flags_clear(e))) return r;
     }
     PK_HIP(c, hipStreamSynchronize(c->stream));
@@ -293,6 +314,11 @@ int snapshot_restore(pk_batch* b, const Snapshot& S)
 }
 
 }  // namespace
+
+int pk_batch_shape_of(const pk_batch* b, int i, int shape)
+{
+    return shape == PK_SHAPE_PER_INSTANCE ? b->shapes[(size_t)i] : shape;
+}
 
 int pk_batch_upload(pk_batch* b, void** dst, const void* src, size_t bytes)
 {
@@ -332,6 +358,8 @@ int pk_batch_create(pk_ctx* c, int n, pk_ens* const* ens, pk_grid* const* grids,
     b->prec = ens[0]->prec;
     b->ens.assign(ens, ens + n);
     b->grids.assign(grids, grids + n);
+    b->shapes.assign(n, -1);
+    for (int d = 0; d < 3; d++) b->ncps0[d] = ens[0]->ncps[d];
     if (hipMalloc((void**)&b->halt, sizeof(int) * PK_HALT_N) != hipSuccess) {
         (void)hipGetLastError();
         delete b;
@@ -346,8 +374,8 @@ void pk_batch_destroy(pk_batch* b)
     if (!b) return;
     (void)hipStreamSynchronize(b->ctx->stream);
     drop_graph(b);
-    for (void* p : {(void*)b->items, b->pjobs, b->ijobs, b->fjobs, (void*)b->fstart, (void*)b->halt, b->iljobs,
-                    (void*)b->ilbase})
+    for (void* p : {(void*)b->items, (void*)b->citems[0], (void*)b->citems[1], b->pjobs, b->ijobs, b->fjobs,
+                    (void*)b->fstart, (void*)b->halt, b->iljobs, (void*)b->ilbase})
         if (p) (void)hipFree(p);
     delete b;
 }
@@ -356,10 +384,30 @@ int pk_batch_set(pk_batch* b, int knob, int value)
 {
     if (!b) return PK_E_ARG;
     if (knob < 0 || knob >= PK_BATCH_NKNOB) return pk_fail(b->ctx, PK_E_ARG, "pk_batch_set: unknown knob");
-    const int hi = knob == PK_BATCH_WG_PER_CU ? 64 : 1;
-    if (value < 0 || value > hi) return pk_fail(b->ctx, PK_E_ARG, "pk_batch_set: value out of range");
+    const int lo = knob == PK_BATCH_INTERLEAVE ? -1 : 0;
+    const int hi = knob == PK_BATCH_WG_PER_CU || knob == PK_BATCH_INTERLEAVE ? 64 : knob == PK_BATCH_SUPERCELL ? 12 : 1;
+    if (value < lo || value > hi) return pk_fail(b->ctx, PK_E_ARG, "pk_batch_set: value out of range");
+    if (knob == PK_BATCH_SUPERCELL) {
+        const int ncps[3] = {value ? value : b->ncps0[0], value ? value : b->ncps0[1], value ? value : b->ncps0[2]};
+        for (pk_ens* e : b->ens)
+            for (int d = 0; d < 3; d++)
+                if (e->ncell[d] % ncps[d] != 0)
+                    return pk_fail(b->ctx, PK_E_ARG, "pk_batch_set: a grid edge is not a multiple of the supercell size");
+        for (pk_ens* e : b->ens)
+            if (int r = pk_ens_set_supercells(e, ncps)) return r;
+    }
     if (b->knob[knob] != value) b->dirty = true;
     b->knob[knob] = value;
+    return PK_OK;
+}
+
+int pk_batch_set_shape(pk_batch* b, int index, int shape)
+{
+    if (!b) return PK_E_ARG;
+    if (index < 0 || index >= b->n()) return pk_fail(b->ctx, PK_E_ARG, "pk_batch_set_shape: index out of range");
+    if (shape != PK_CIC && shape != PK_TSC) return pk_fail(b->ctx, PK_E_ARG, "pk_batch_set_shape: unknown shape");
+    if (b->shapes[(size_t)index] != shape) b->dirty = true;
+    b->shapes[(size_t)index] = shape;
     return PK_OK;
 }
 
@@ -398,41 +446,109 @@ int pk_batch_autotune(pk_batch* b, double dt, int shape, int dep, int trial_step
         (void)hipEventDestroy(t0);
         return pk_fail(c, PK_E_HIP, "pk_batch_autotune: hipEventCreate failed");
     }
-    const int saved[PK_BATCH_NKNOB] = {b->knob[0], b->knob[1], b->knob[2]};
-    int best[PK_BATCH_NKNOB] = {saved[0], saved[1], saved[2]};
+    int saved[PK_BATCH_NKNOB];
+    for (int k = 0; k < PK_BATCH_NKNOB; k++) saved[k] = b->knob[k];
+    // supercell edge of the state the autotuner starts from (0 when not cubic)
+    const int* n0 = S.ens[0].ncps;
+    const int edge0 = n0[0] == n0[1] && n0[1] == n0[2] ? n0[0] : 0;
+    // one trial from the saved state: supercell edge sc (0 = the saved one), then the knobs
+    auto trial = [&](int sc, const int knobs[PK_BATCH_NKNOB], double* ms) -> int {
+        int rr;
+        if ((rr = snapshot_restore(b, S))) return rr;
+        if (sc > 0 && sc != edge0) {
+            const int n3[3] = {sc, sc, sc};
+            for (pk_ens* e : b->ens)
+                if ((rr = pk_ens_set_supercells(e, n3))) return rr;
+        }
+        for (int k = 0; k < PK_BATCH_NKNOB; k++)
+            if (k != PK_BATCH_SUPERCELL) b->knob[k] = knobs[k];
+        b->dirty = true;
+        if ((rr = run(b, dt, shape, dep, 1))) return rr;     // warm-up, prepares and captures
+        if (hipEventRecord(t0, c->stream) != hipSuccess) return PK_E_HIP;
+        if ((rr = run(b, dt, shape, dep, trial_steps))) return rr;
+        if (hipEventRecord(t1, c->stream) != hipSuccess || hipEventSynchronize(t1) != hipSuccess) return PK_E_HIP;
+        float t = 0.f;
+        (void)hipEventElapsedTime(&t, t0, t1);
+        *ms = t;
+        return PK_OK;
+    };
+    int best[PK_BATCH_NKNOB];
+    for (int k = 0; k < PK_BATCH_NKNOB; k++) best[k] = saved[k];
+    best[PK_BATCH_GRAPH] = 1;
+    int best_sc = 0;
     double best_t = 1e300;
-    const int wgs[] = {0, 1, 2, 4, 8};
-    for (int graph = 0; graph <= 1 && !r; graph++)
-        for (int order = 0; order <= 1 && !r; order++)
-            for (int wg : wgs) {
-                if ((r = snapshot_restore(b, S))) break;
-                b->knob[PK_BATCH_WG_PER_CU] = wg;
-                b->knob[PK_BATCH_ORDER] = order;
-                b->knob[PK_BATCH_GRAPH] = graph;
-                b->dirty = true;
-                if ((r = run(b, dt, shape, dep, 1))) break;     // warm-up, prepares and captures
-                if (hipEventRecord(t0, c->stream) != hipSuccess) { r = PK_E_HIP; break; }
-                if ((r = run(b, dt, shape, dep, trial_steps))) break;
-                if (hipEventRecord(t1, c->stream) != hipSuccess || hipEventSynchronize(t1) != hipSuccess) {
-                    r = PK_E_HIP;
-                    break;
+    // a candidate that does not fit (supercell past 65,536 particles or the LDS) is skipped
+    auto consider = [&](int sc, const int knobs[PK_BATCH_NKNOB]) -> int {
+        double ms = 0.0;
+        const int rr = trial(sc, knobs, &ms);
+        if (rr == PK_E_CAPACITY) return PK_OK;
+        if (rr) return rr;
+        if (ms < best_t) {
+            best_t = ms;
+            best_sc = sc;
+            for (int k = 0; k < PK_BATCH_NKNOB; k++) best[k] = knobs[k];
+        }
+        return PK_OK;
+    };
+    // (1) supercell size
+    {
+        const int cand[] = {0, 2, 4, 8};
+        for (int sc : cand) {
+            if (sc > 0 && sc == edge0) continue;
+            bool fits = true;
+            for (pk_ens* e : b->ens)
+                for (int d = 0; d < 3; d++)
+                    if (sc > 0 && e->ncell[d] % sc != 0) fits = false;
+            if (!fits) continue;
+            int kn[PK_BATCH_NKNOB];
+            for (int k = 0; k < PK_BATCH_NKNOB; k++) kn[k] = best[k];
+            if ((r = consider(sc, kn))) break;
+        }
+    }
+    // (2) interleave period (FP32 split ensembles)
+    if (!r && b->prec == PK_F32) {
+        const int per[] = {0, 2, 3, 4, 6, 8};
+        const int sc = best_sc;
+        int kn0[PK_BATCH_NKNOB];
+        for (int k = 0; k < PK_BATCH_NKNOB; k++) kn0[k] = best[k];
+        for (int p : per) {
+            int kn[PK_BATCH_NKNOB];
+            for (int k = 0; k < PK_BATCH_NKNOB; k++) kn[k] = kn0[k];
+            kn[PK_BATCH_INTERLEAVE] = p;
+            if ((r = consider(sc, kn))) break;
+        }
+    }
+    // (3) launch: graph, item order, persistence
+    if (!r) {
+        const int wgs[] = {0, 1, 2, 4, 8};
+        const int sc = best_sc;
+        int kn0[PK_BATCH_NKNOB];
+        for (int k = 0; k < PK_BATCH_NKNOB; k++) kn0[k] = best[k];
+        for (int graph = 0; graph <= 1 && !r; graph++)
+            for (int order = 0; order <= 1 && !r; order++)
+                for (int wg : wgs) {
+                    int kn[PK_BATCH_NKNOB];
+                    for (int k = 0; k < PK_BATCH_NKNOB; k++) kn[k] = kn0[k];
+                    kn[PK_BATCH_GRAPH] = graph;
+                    kn[PK_BATCH_ORDER] = order;
+                    kn[PK_BATCH_WG_PER_CU] = wg;
+                    if ((r = consider(sc, kn))) break;
                 }
-                float ms = 0.f;
-                (void)hipEventElapsedTime(&ms, t0, t1);
-                if (ms < best_t) {
-                    best_t = ms;
-                    best[0] = wg;
-                    best[1] = order;
-                    best[2] = graph;
-                }
-            }
+    }
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
-    const int rr = snapshot_restore(b, S);
+    int rr = snapshot_restore(b, S);
+    if (!rr && !r && best_sc > 0 && best_sc != edge0) {
+        const int n3[3] = {best_sc, best_sc, best_sc};
+        for (pk_ens* e : b->ens)
+            if ((rr = pk_ens_set_supercells(e, n3))) break;
+    }
     for (int k = 0; k < PK_BATCH_NKNOB; k++) b->knob[k] = r ? saved[k] : best[k];
+    if (!r) b->knob[PK_BATCH_SUPERCELL] = best_sc > 0 ? best_sc : edge0;
     b->dirty = true;
     if (r) return r == PK_E_HIP ? pk_fail(c, PK_E_HIP, "pk_batch_autotune: event timing failed") : r;
     if (rr) return rr;
+    if (best_t >= 1e300) return pk_fail(c, PK_E_CAPACITY, "pk_batch_autotune: no setting fits");
     if (best_ms) *best_ms = best_t / trial_steps;
     return PK_OK;
 }

@@ -924,7 +924,8 @@ extern "C" int pk_ens_interleave(pk_ens* e)
 int pk_batch_interleave_prepare(pk_batch* b)
 {
     const int n = b->n();
-    b->ilperiod = b->ens[0]->posmode == PK_POS_SPLIT ? b->ctx->opt[PK_OPT_INTERLEAVE] : 0;
+    const int period = b->knob[PK_BATCH_INTERLEAVE] >= 0 ? b->knob[PK_BATCH_INTERLEAVE] : b->ctx->opt[PK_OPT_INTERLEAVE];
+    b->ilperiod = b->ens[0]->posmode == PK_POS_SPLIT ? period : 0;
     if (b->ilperiod <= 0) return PK_OK;
     std::vector<InterleaveArgs> jobs(n);
     for (int i = 0; i < n; i++) {
@@ -1100,6 +1101,143 @@ static int rebuild_from(pk_ens* e, long nent, DBuf& keys_in, DBuf& srcs_in, Fiel
     e->np = nent;
     e->pending_movers = false;
     e->failed = false;
+    return PK_OK;
+}
+
+// keys of every live particle under another supercell geometry (re-bucketing): particle j of old
+// supercell s is entry ebase[s] + j; its key is its new supercell x 28 (class 0), its source the
+// old main slot.  The split form keys by its cell, the absolute form by pica's getSupercellIndex
+// with the new supercell size (EnsembleSupercells.h:31-33,56).
+__global__ void __launch_bounds__(64) k_keys_rebucket(EnsGeom old_eg, EnsGeom new_eg, StageGeom new_sg, int split,
+                                                      PartF64 pf, PartSplit ps, const int64_t* off,
+                                                      const int32_t* cnt, const int64_t* ebase, uint32_t* keys,
+                                                      uint64_t* srcs)
+{
+    const long s = blockIdx.x;
+    if (s >= old_eg.nsct) return;
+    const int n = cnt[s];
+    for (int j = threadIdx.x; j < n; j += 64) {
+        const long p = off[s] + j;
+        int sc[3];
+        if (split) {
+            int c[3];
+            cell_coords(ps.cell[p], old_eg, c);
+#pragma unroll
+            for (int d = 0; d < 3; d++) sc[d] = c[d] / new_eg.ncps[d];
+        } else {
+            const double x[3] = {pf.x[p], pf.y[p], pf.z[p]};
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const int k = (int)floor((x[d] - 0.0) * new_sg.scinv[d]);
+                sc[d] = k < 0 ? 0 : (k >= new_eg.nsc[d] ? new_eg.nsc[d] - 1 : k);
+            }
+        }
+        keys[ebase[s] + j] = (uint32_t)((((long)sc[0] * new_eg.nsc[1] + sc[1]) * new_eg.nsc[2] + sc[2]) * 28);
+        srcs[ebase[s] + j] = SRC_MAIN | (uint64_t)p;
+    }
+}
+
+__global__ void k_count64(long nsct, const int32_t* cnt, int64_t* out)
+{
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nsct) out[s] = cnt[s];
+    if (s == nsct) out[s] = 0;
+}
+
+// Supercell arrays (off, cnt, nstay, nmove, mvc) of a new supercell shape replace the old ones; the
+// particle buffers are left as they are (the caller rebuilds or overwrites them).
+int pk_ens_reshape(pk_ens* e, const int ncps[3])
+{
+    pk_ctx* c = e->ctx;
+    long nsct = 1;
+    int nsc[3];
+    for (int d = 0; d < 3; d++) {
+        nsc[d] = e->ncell[d] / ncps[d];
+        nsct *= nsc[d];
+    }
+    int64_t* noff = nullptr;
+    int32_t *ncnt = nullptr, *nstay = nullptr, *nmove = nullptr, *nmvc = nullptr;
+    const bool ok = hipMalloc((void**)&noff, sizeof(int64_t) * (nsct + 1)) == hipSuccess &&
+                    hipMalloc((void**)&ncnt, sizeof(int32_t) * nsct) == hipSuccess &&
+                    hipMalloc((void**)&nstay, sizeof(int32_t) * nsct) == hipSuccess &&
+                    hipMalloc((void**)&nmove, sizeof(int32_t) * nsct) == hipSuccess &&
+                    hipMalloc((void**)&nmvc, sizeof(int32_t) * nsct * 27) == hipSuccess;
+    if (!ok) {
+        for (void* p : {(void*)noff, (void*)ncnt, (void*)nstay, (void*)nmove, (void*)nmvc})
+            if (p) (void)hipFree(p);
+        (void)hipGetLastError();
+        return pk_fail(c, PK_E_ALLOC, "supercell re-shape: hipMalloc failed");
+    }
+    PK_HIP(c, hipStreamSynchronize(c->stream));
+    for (void* p : {(void*)e->off, (void*)e->cnt, (void*)e->nstay, (void*)e->nmove, (void*)e->mvc})
+        if (p) (void)hipFree(p);
+    e->off = noff;
+    e->cnt = ncnt;
+    e->nstay = nstay;
+    e->nmove = nmove;
+    e->mvc = nmvc;
+    for (int d = 0; d < 3; d++) {
+        e->ncps[d] = ncps[d];
+        e->nsc[d] = nsc[d];
+    }
+    e->nsct = nsct;
+    return PK_OK;
+}
+
+// Re-bucket every particle into supercells of ncps cells (the particle data move, the storage is
+// rebuilt).  On failure after the geometry changed the ensemble is marked unusable.
+extern "C" int pk_ens_set_supercells(pk_ens* e, const int ncps[3])
+{
+    if (!e || !ncps) return PK_E_ARG;
+    pk_ctx* c = e->ctx;
+    long nsct = 1;
+    int nsc[3];
+    for (int d = 0; d < 3; d++) {
+        if (ncps[d] <= 0 || e->ncell[d] % ncps[d] != 0)
+            return pk_fail(c, PK_E_ARG, "pk_ens_set_supercells: ncell must be a positive multiple of ncps");
+        if (ncps[d] > 12) return pk_fail(c, PK_E_ARG, "pk_ens_set_supercells: supercells are limited to 12 cells per side");
+        nsc[d] = e->ncell[d] / ncps[d];
+        nsct *= nsc[d];
+    }
+    if (nsct * 28 >= (1L << 31)) return pk_fail(c, PK_E_ARG, "pk_ens_set_supercells: too many supercells");
+    int r = pk_ens_resort(e);
+    if (r) return r;
+    if (ncps[0] == e->ncps[0] && ncps[1] == e->ncps[1] && ncps[2] == e->ncps[2]) return PK_OK;
+    const long onsct = e->nsct;
+    const long nent = (long)e->np;
+    pk_ens ne = *e;      // the new geometry, for its EnsGeom / StageGeom
+    for (int d = 0; d < 3; d++) {
+        ne.ncps[d] = ncps[d];
+        ne.nsc[d] = nsc[d];
+    }
+    ne.nsct = nsct;
+    DBuf ecount, ebase, keys, srcs;
+    PK_HIP(c, ecount.alloc(sizeof(int64_t) * (onsct + 1)));
+    PK_HIP(c, ebase.alloc(sizeof(int64_t) * (onsct + 1)));
+    PK_HIP(c, keys.alloc(sizeof(uint32_t) * (nent > 0 ? nent : 1)));
+    PK_HIP(c, srcs.alloc(sizeof(uint64_t) * (nent > 0 ? nent : 1)));
+    hipLaunchKernelGGL(k_count64, dim3(nblk(onsct + 1, 256)), dim3(256), 0, c->stream, onsct, e->cnt,
+                       ecount.as<int64_t>());
+    if ((r = exclusive_scan<int64_t>(c, ecount.as<int64_t>(), ebase.as<int64_t>(), onsct + 1))) return r;
+    PartF64 pf{};
+    PartSplit ps{};
+    if (e->prec == PK_F64) pf = part_f64(e, e->main);
+    else ps = part_split(e, e->main);
+    hipLaunchKernelGGL(k_keys_rebucket, dim3((unsigned)onsct), dim3(64), 0, c->stream, ens_geom(e), ens_geom(&ne),
+                       stage_geom(&ne), e->posmode == PK_POS_SPLIT ? 1 : 0, pf, ps, e->off, e->cnt,
+                       ebase.as<int64_t>(), keys.as<uint32_t>(), srcs.as<uint64_t>());
+    PK_HIP(c, hipGetLastError());
+    PK_HIP(c, hipStreamSynchronize(c->stream));      // the key kernel has read the old arrays
+    if ((r = pk_ens_reshape(e, ncps))) return r;
+    pk_ktimer tm(c, "rebuild");
+    r = rebuild_from(e, nent, keys, srcs, nullptr);
+    if (r) {
+        e->failed = true;
+        c->err += " (the ensemble is unusable until it is uploaded or loaded again)";
+        return r;
+    }
+    // the rebuild keeps the old slot order inside each new supercell; pair same-cell particles again
+    if (e->posmode == PK_POS_SPLIT && c->opt[PK_OPT_INTERLEAVE] > 0) return pk_ens_interleave_launch(e);
     return PK_OK;
 }
 

@@ -218,7 +218,9 @@ struct pk_batch {
     int prec = PK_F64;
     std::vector<pk_ens*> ens;
     std::vector<pk_grid*> grids;
-    int knob[PK_BATCH_NKNOB] = {0, 1, 1};
+    int knob[PK_BATCH_NKNOB] = {0, 1, 1, 0, -1};
+    int ncps0[3] = {0, 0, 0};  // supercell shape the ensembles had when the batch was created
+    std::vector<int> shapes;   // per-ensemble form factor for PK_SHAPE_PER_INSTANCE (-1: unset)
     // prepared launch state, valid while !dirty and (dt, shape, dep) are unchanged
     bool dirty = true;
     double dt = 0.0;
@@ -230,7 +232,11 @@ struct pk_batch {
     void* fjobs = nullptr;     // field-kernel arguments
     long* fstart = nullptr;    // PK_BSTAGES x (n + 1) block prefixes of the field stages
     long fblocks[PK_BSTAGES] = {0, 0, 0, 0, 0};
-    size_t plds = 0;           // dynamic LDS of the particle kernel (largest ensemble)
+    // particle-kernel items of each form factor (PK_CIC, PK_TSC), one launch per non-empty class,
+    // and the dynamic LDS of each class (its largest ensemble)
+    int2* citems[2] = {nullptr, nullptr};
+    long cnitems[2] = {0, 0};
+    size_t cplds[2] = {0, 0};
     int* halt = nullptr;       // device halt words (PK_HALT_*)
     void* iljobs = nullptr;    // interleave-kernel arguments (FP32 split ensembles)
     int64_t* ilbase = nullptr; // per ensemble: re-sort count at the start of the current pass
@@ -243,10 +249,12 @@ struct pk_batch {
 int pk_batch_upload(pk_batch* b, void** dst, const void* src, size_t bytes);
 int pk_batch_fields_prepare(pk_batch* b, double dt);
 int pk_batch_fields_launch(pk_batch* b, int op, unsigned mask, int d);
+int pk_batch_shape_of(const pk_batch* b, int i, int shape);   // form factor of ensemble i in a run
 int pk_batch_split_prepare(pk_batch* b, double dt, int shape, int dep);
-int pk_batch_split_launch(pk_batch* b, int shape, int dep);
+int pk_batch_split_launch(pk_batch* b, int dep);
 int pk_batch_f64_prepare(pk_batch* b, double dt, int shape, int dep);
-int pk_batch_f64_launch(pk_batch* b, int shape, int dep);
+int pk_batch_f64_launch(pk_batch* b, int dep);
+int pk_ens_reshape(pk_ens* e, const int ncps[3]);   // new supercell arrays and geometry, no particles moved
 int pk_batch_insert_prepare(pk_batch* b);
 int pk_batch_insert_launch(pk_batch* b);
 int pk_batch_interleave_prepare(pk_batch* b);

@@ -516,30 +516,39 @@ int launch(pk_ens* e, pk_grid* g, double dt)
     return PK_OK;
 }
 
+// the items of one form factor (SHAPE) in one launch
 template <int SHAPE, int DEP>
 int batch_go(pk_batch* b, bool prepare)
 {
     const void* fn = (const void*)k_particles_batch<SHAPE, DEP>;
+    const size_t lds = b->cplds[SHAPE];
+    const long nitems = b->cnitems[SHAPE];
+    if (nitems == 0) return PK_OK;
     if (prepare) {
-        PK_HIP(b->ctx, hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->plds));
+        PK_HIP(b->ctx, hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         return PK_OK;
     }
-    if (b->nitems == 0) return PK_OK;
     const long wg = b->knob[PK_BATCH_WG_PER_CU];
-    const long nb = wg > 0 && wg * 256 < b->nitems ? wg * 256 : b->nitems;
-    hipLaunchKernelGGL((k_particles_batch<SHAPE, DEP>), dim3((unsigned)nb), dim3(PK_BLOCK), b->plds, b->ctx->stream,
-                       (const F64Args*)b->pjobs, b->items, b->nitems, b->halt);
+    const long nb = wg > 0 && wg * 256 < nitems ? wg * 256 : nitems;
+    hipLaunchKernelGGL((k_particles_batch<SHAPE, DEP>), dim3((unsigned)nb), dim3(PK_BLOCK), lds, b->ctx->stream,
+                       (const F64Args*)b->pjobs, b->citems[SHAPE], nitems, b->halt);
     PK_HIP(b->ctx, hipGetLastError());
     return PK_OK;
 }
 
-int batch_dispatch(pk_batch* b, int shape, int dep, bool prepare)
+// every form factor present, one launch each
+int batch_dispatch(pk_batch* b, int dep, bool prepare)
 {
-    if (shape == PK_CIC && dep == PK_DIRECT) return batch_go<PK_CIC, PK_DIRECT>(b, prepare);
-    if (shape == PK_CIC && dep == PK_ESIRKEPOV) return batch_go<PK_CIC, PK_ESIRKEPOV>(b, prepare);
-    if (shape == PK_TSC && dep == PK_DIRECT) return batch_go<PK_TSC, PK_DIRECT>(b, prepare);
-    if (shape == PK_TSC && dep == PK_ESIRKEPOV) return batch_go<PK_TSC, PK_ESIRKEPOV>(b, prepare);
-    return pk_fail(b->ctx, PK_E_ARG, "particle step: unknown shape/depositor");
+    int r;
+    if (dep == PK_DIRECT) {
+        if ((r = batch_go<PK_CIC, PK_DIRECT>(b, prepare))) return r;
+        return batch_go<PK_TSC, PK_DIRECT>(b, prepare);
+    }
+    if (dep == PK_ESIRKEPOV) {
+        if ((r = batch_go<PK_CIC, PK_ESIRKEPOV>(b, prepare))) return r;
+        return batch_go<PK_TSC, PK_ESIRKEPOV>(b, prepare);
+    }
+    return pk_fail(b->ctx, PK_E_ARG, "particle step: unknown depositor");
 }
 
 }  // namespace
@@ -557,18 +566,19 @@ int pk_batch_f64_prepare(pk_batch* b, double dt, int shape, int dep)
 {
     const int n = b->n();
     std::vector<F64Args> jobs(n);
-    size_t lds = 0;
+    b->cplds[0] = b->cplds[1] = 0;
     for (int i = 0; i < n; i++) {
         pk_ens* e = b->ens[i];
         const StepParams sp = step_params(e, dt);
         jobs[i] = f64_args(e, b->grids[i], ens_geom(e), sp);
-        const size_t l = f64_lds(e, shape, dep, sp.mask_words);
-        if (l > lds) lds = l;
+        const int sh = pk_batch_shape_of(b, i, shape);
+        const size_t l = f64_lds(e, sh, dep, sp.mask_words);
+        if (l > b->cplds[sh]) b->cplds[sh] = l;
     }
-    if (lds > 160 * 1024) return pk_fail(b->ctx, PK_E_CAPACITY, "particle step: supercell too large for LDS");
-    b->plds = lds;
+    if (b->cplds[0] > 160 * 1024 || b->cplds[1] > 160 * 1024)
+        return pk_fail(b->ctx, PK_E_CAPACITY, "particle step: supercell too large for LDS");
     if (int r = pk_batch_upload(b, &b->pjobs, jobs.data(), sizeof(F64Args) * n)) return r;
-    return batch_dispatch(b, shape, dep, true);
+    return batch_dispatch(b, dep, true);
 }
 
-int pk_batch_f64_launch(pk_batch* b, int shape, int dep) { return batch_dispatch(b, shape, dep, false); }
+int pk_batch_f64_launch(pk_batch* b, int dep) { return batch_dispatch(b, dep, false); }

@@ -47,3 +47,16 @@ def random_fields(rng, grid, e_scale, b_scale):
         grid.f[3 + c][...] = rng.standard_normal(grid.s) * b_scale
     for c in range(6):
         O.ghost_fill(grid.f[c])
+
+
+def config3_specs(n=64, seed=3, edges=(32, 48, 64, 96, 128), ppcs=(4, 8, 16, 32)):
+    """independent plasmas of the batching form: per instance a cubic grid edge and a ppc drawn
+    uniformly from the given sets, a drift speed uniform in [0, 0.2]c (returned as u = v / sqrt(1 -
+    v^2)), CIC or TSC with equal odds, and its own seed"""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        v = 0.2 * rng.random()
+        out.append({"edge": int(rng.choice(edges)), "ppc": int(rng.choice(ppcs)), "drift": v / np.sqrt(1.0 - v * v),
+                    "shape": int(rng.integers(0, 2)), "seed": 1000 + i})
+    return out

@@ -6,6 +6,7 @@ round-off; FP32: FP32 round-off), bounded per quantity below."""
 import numpy as np
 import pytest
 
+import oracle as O
 import pica_amd as P
 from helpers import courant_dt, plasma, random_particles, rel_err
 
@@ -138,28 +139,101 @@ def test_batch_slack_overflow_recovery(ctx):
     assert rebuilds > 0
 
 
-def test_batch_autotune_restores_state(ctx):
-    """the autotuner's trials leave the state as it was; the chosen setting then runs correctly"""
-    _say("batch autotune")
+@pytest.mark.parametrize("prec", [P.F64, P.F32])
+def test_batch_autotune_restores_state(ctx, prec):
+    """the autotuner's trials (supercell sizes, interleave periods, launch settings) leave every
+    particle and field as they were, re-bucketed to the chosen supercell size exactly as
+    pk_ens_set_supercells does; the chosen setting then runs like one-by-one stepping with the
+    same supercells and interleave period"""
+    _say("batch autotune prec=%d" % prec)
     dt = courant_dt()
-    mine = _pairs(ctx, P.F32)
-    ref = _pairs(ctx, P.F32)
+    mine = _pairs(ctx, prec)
+    ref = _pairs(ctx, prec)
     b = P.Batch(ctx, mine)
-    before = [_state(e, g) for e, g in mine]
     ms = b.autotune(dt, P.TSC, P.ESIRKEPOV, trial_steps=2)
-    after = [_state(e, g) for e, g in mine]
-    for s0, s1 in zip(before, after):
-        for a0, a1 in zip(s0, s1):
-            assert np.array_equal(a0, a1)
-    knobs = [b.get(k) for k in (P.BATCH_WG_PER_CU, P.BATCH_ORDER, P.BATCH_GRAPH)]
+    knobs = {k: b.get(k) for k in (P.BATCH_WG_PER_CU, P.BATCH_ORDER, P.BATCH_GRAPH, P.BATCH_SUPERCELL,
+                                   P.BATCH_INTERLEAVE)}
     print("autotune: %.4f ms/step with knobs %s" % (ms, knobs), flush=True)
-    assert ms > 0 and knobs[0] in (0, 1, 2, 4, 8) and knobs[1] in (0, 1) and knobs[2] in (0, 1)
-    b.run(dt, 3, P.TSC, P.ESIRKEPOV)
+    assert ms > 0 and knobs[P.BATCH_WG_PER_CU] in (0, 1, 2, 4, 8) and knobs[P.BATCH_ORDER] in (0, 1)
+    assert knobs[P.BATCH_GRAPH] in (0, 1) and knobs[P.BATCH_SUPERCELL] in (2, 4)
+    assert knobs[P.BATCH_INTERLEAVE] in ((-1,) if prec == P.F64 else (0, 2, 3, 4, 6, 8))
+    sc = knobs[P.BATCH_SUPERCELL]
     for e, g in ref:
-        P.run(e, g, dt, 3, P.TSC, P.ESIRKEPOV)
+        e.set_supercells((sc, sc, sc))
     for i, ((e, g), (re_, rg)) in enumerate(zip(mine, ref)):
-        _compare(P.F32, _state(e, g), _state(re_, rg), "ensemble %d" % i)
+        for a0, a1 in zip(_state(e, g), _state(re_, rg)):
+            assert np.array_equal(a0, a1), "ensemble %d changed by the autotuner" % i
+    period = ctx.get_option(P.OPT_INTERLEAVE)
+    if knobs[P.BATCH_INTERLEAVE] >= 0:
+        ctx.set_option(P.OPT_INTERLEAVE, knobs[P.BATCH_INTERLEAVE])
+    try:
+        b.run(dt, 3, P.TSC, P.ESIRKEPOV)
+        for e, g in ref:
+            P.run(e, g, dt, 3, P.TSC, P.ESIRKEPOV)
+    finally:
+        ctx.set_option(P.OPT_INTERLEAVE, period)
+    for i, ((e, g), (re_, rg)) in enumerate(zip(mine, ref)):
+        _compare(prec, _state(e, g), _state(re_, rg), "ensemble %d" % i)
     b.close()
+
+
+@pytest.mark.parametrize("prec", [P.F64, P.F32])
+def test_batch_per_instance_form_factor(ctx, prec):
+    """CIC and TSC instances in one batch (PK_SHAPE_PER_INSTANCE: one particle launch per form
+    factor) give each instance's one-by-one result with its own form factor"""
+    _say("batch per-instance shapes prec=%d" % prec)
+    dt = courant_dt()
+    shapes = [P.CIC, P.TSC, P.TSC, P.CIC]
+    mine = _pairs(ctx, prec)
+    ref = _pairs(ctx, prec)
+    b = P.Batch(ctx, mine)
+    for i, s in enumerate(shapes):
+        b.set_shape(i, s)
+    b.run(dt, 5, P.SHAPE_PER_INSTANCE, P.ESIRKEPOV)
+    for (e, g), s in zip(ref, shapes):
+        P.run(e, g, dt, 5, s, P.ESIRKEPOV)
+    for i, ((e, g), (re_, rg)) in enumerate(zip(mine, ref)):
+        _compare(prec, _state(e, g), _state(re_, rg), "ensemble %d (shape %d)" % (i, shapes[i]))
+    # an unset shape is refused
+    b2 = P.Batch(ctx, _pairs(ctx, prec, SPECS[:2]))
+    b2.set_shape(0, P.TSC)
+    with pytest.raises(P.PicaError) as ei:
+        b2.run(dt, 1, P.SHAPE_PER_INSTANCE, P.ESIRKEPOV)
+    assert ei.value.code == P.PK_E_ARG
+    b2.close()
+    b.close()
+
+
+@pytest.mark.parametrize("ncps", [(2, 2, 2), (8, 8, 8), (2, 4, 8)])
+def test_set_supercells_matches_the_oracle(ctx, ncps):
+    """re-bucketing an FP64 ensemble into other supercells gives pica's bucketing of the same
+    particles (membership and order bit-exact: stable by new supercell in the old order), and 6
+    TSC/Esirkepov steps then match the oracle with those supercells at 1e-12"""
+    _say("set_supercells ncps=%s" % (ncps,))
+    ncell = (16, 16, 8)
+    dt = courant_dt()
+    _, w0 = plasma(dt, 8)
+    e = P.EnsembleSupercells(ctx, ncell, (1.0, 1.0, 1.0), (4, 4, 4))
+    e.load_thermal(8, 0.02, 0.1, w0, 77)
+    g = P.YeeGrid.periodic(ctx, ncell)
+    P.run(e, g, dt, 2, P.TSC, P.ESIRKEPOV)
+    x, u, w, off = e.particles()
+    og = O.Grid.periodic(ncell)
+    og.f[...] = g.get_all()[:9]
+    e.set_supercells(ncps)
+    oe = O.Ensemble(ncell, ncps, (1.0, 1.0, 1.0), x, u, w)
+    gx, gu, gw, goff = e.particles()
+    assert np.array_equal(goff, oe.off) and np.array_equal(gx, oe.positions()) and np.array_equal(gu, oe.momenta())
+    for _ in range(6):
+        O.step(og, oe, dt, O.TSC, O.ESIRKEPOV)
+    P.run(e, g, dt, 6, P.TSC, P.ESIRKEPOV)
+    gx, gu, gw, goff = e.particles()
+    assert np.array_equal(goff, oe.off)
+    assert rel_err(gx, oe.positions()) < 1e-12 and rel_err(gu, oe.momenta()) < 1e-12
+    f = g.get_all()
+    assert max(rel_err(f[c], og.f[c]) for c in range(9)) < 1e-12
+    e.close()
+    g.close()
 
 
 # a mix of supercell sizes: 40 ppc puts ~2560 particles in a 4^3 supercell (the interleave's

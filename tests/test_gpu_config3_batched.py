@@ -1,0 +1,119 @@
+"""The batching form at config 3's shape: independent periodic plasmas with cubic grid edges from
+{32, 48, 64, 96, 128}, ppc from {4, 8, 16, 32}, drift speeds uniform in [0, 0.2]c, CIC or TSC per
+instance (PK_SHAPE_PER_INSTANCE), Esirkepov deposition, stepped as one batch.
+
+- FP64, instances small enough for the oracle: each instance against the oracle, supercell
+  membership and particle order bit-exact, x, u, E, B, J within 1e-12 after 10 steps;
+- FP32, the 64 instances of the workload, autotuned, 100 steps: every instance keeps its charge
+  (1e-5) and its Gauss-law residual (drift below 1e-4 of max |4 pi rho|), loses no particle, and
+  its total energy agrees with an FP64 batch of the same instances from the same seeded load.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+import pica_amd as P
+from helpers import config3_specs, courant_dt, plasma, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print("[case] " + msg, flush=True)
+
+
+def _load(ctx, prec, spec, ncps=(4, 4, 4)):
+    dt = courant_dt()
+    _, w0 = plasma(dt, spec["ppc"])
+    n = spec["edge"]
+    g = P.YeeGrid.periodic(ctx, (n, n, n), prec=prec)
+    e = P.EnsembleSupercells(ctx, (n, n, n), (1.0, 1.0, 1.0), ncps, prec=prec)
+    e.load_thermal(spec["ppc"], 0.01, spec["drift"], w0, spec["seed"])
+    return e, g
+
+
+def test_config3_fp64_instances_match_the_oracle(ctx):
+    specs = [s for s in config3_specs() if s["edge"] * s["edge"] * s["edge"] * s["ppc"] <= 48 ** 3 * 8][:4]
+    assert {s["shape"] for s in specs} == {P.CIC, P.TSC}, "the draw should hold both form factors"
+    _say("config-3 FP64 vs oracle: %s" % [(s["edge"], s["ppc"], round(s["drift"], 3), s["shape"]) for s in specs])
+    dt = courant_dt()
+    pairs = [_load(ctx, P.F64, s) for s in specs]
+    oracles = []
+    for (e, g), s in zip(pairs, specs):
+        x, u, w, off = e.particles()
+        oracles.append((O.Grid.periodic((s["edge"],) * 3), O.Ensemble((s["edge"],) * 3, (4, 4, 4), (1.0, 1.0, 1.0),
+                                                                     x, u, w)))
+    b = P.Batch(ctx, pairs)
+    for i, s in enumerate(specs):
+        b.set_shape(i, s["shape"])
+    b.run(dt, 10, P.SHAPE_PER_INSTANCE, P.ESIRKEPOV)
+    for (og, oe), s in zip(oracles, specs):
+        for _ in range(10):
+            O.step(og, oe, dt, s["shape"], O.ESIRKEPOV)
+    for i, ((e, g), (og, oe)) in enumerate(zip(pairs, oracles)):
+        gx, gu, gw, goff = e.particles()
+        assert np.array_equal(goff, oe.off), "instance %d: supercell membership differs" % i
+        ex, eu = rel_err(gx, oe.positions()), rel_err(gu, oe.momenta())
+        f = g.get_all()
+        ef = max(rel_err(f[c], og.f[c]) for c in range(9))
+        print("instance %d (edge %d ppc %d shape %d): x %.3g u %.3g fields %.3g" %
+              (i, specs[i]["edge"], specs[i]["ppc"], specs[i]["shape"], ex, eu, ef), flush=True)
+        assert ex < 1e-12 and eu < 1e-12 and ef < 1e-12
+    b.close()
+
+
+def test_config3_fp32_workload_100_steps(ctx):
+    specs = config3_specs()
+    _say("config-3 FP32 workload, %d instances, 100 steps" % len(specs))
+    dt = courant_dt()
+    pairs = [_load(ctx, P.F32, s) for s in specs]
+    n0 = [e.size() for e, _ in pairs]
+    shape_of = {i: s["shape"] for i, s in enumerate(specs)}
+    d0 = []
+    for i, (e, g) in enumerate(pairs):
+        P.diag_set_reference(e, g, shape_of[i])
+        d0.append(P.diag(e, g, shape_of[i]))
+    b = P.Batch(ctx, pairs)
+    for i, s in enumerate(specs):
+        b.set_shape(i, s["shape"])
+    ms = b.autotune(dt, P.SHAPE_PER_INSTANCE, P.ESIRKEPOV, trial_steps=2)
+    knobs = {k: b.get(k) for k in (P.BATCH_SUPERCELL, P.BATCH_INTERLEAVE, P.BATCH_GRAPH, P.BATCH_ORDER,
+                                   P.BATCH_WG_PER_CU)}
+    print("autotuned: %.3f ms/step, knobs %s" % (ms, knobs), flush=True)
+    b.run(dt, 100, P.SHAPE_PER_INSTANCE, P.ESIRKEPOV)
+    worst = {"charge": 0.0, "gauss": 0.0}
+    energies32 = []
+    for i, (e, g) in enumerate(pairs):
+        d1 = P.diag(e, g, shape_of[i])
+        assert e.size() == n0[i], "instance %d lost particles" % i
+        q = abs(d1[P.DIAG_CHARGE] - d0[i][P.DIAG_CHARGE]) / abs(d0[i][P.DIAG_CHARGE])
+        gd = d1[P.DIAG_GAUSS_DRIFT] / d1[P.DIAG_RHO_MAX]
+        worst["charge"] = max(worst["charge"], q)
+        worst["gauss"] = max(worst["gauss"], gd)
+        energies32.append(d1[P.DIAG_FIELD_ENERGY] + d1[P.DIAG_KINETIC_ENERGY])
+    print("config-3 FP32 after 100 steps: worst charge change %.3g, worst Gauss drift %.3g" %
+          (worst["charge"], worst["gauss"]), flush=True)
+    assert worst["charge"] < 1e-5 and worst["gauss"] < 1e-4
+    b.close()
+    for e, g in pairs:
+        e.close()
+        g.close()
+    # the same instances in FP64 from the same seeded load: total energy per instance
+    sub = [i for i, s in enumerate(specs) if s["edge"] <= 64][:16]
+    pairs64 = [_load(ctx, P.F64, specs[i]) for i in sub]
+    b64 = P.Batch(ctx, pairs64)
+    for k, i in enumerate(sub):
+        b64.set_shape(k, specs[i]["shape"])
+    b64.run(dt, 100, P.SHAPE_PER_INSTANCE, P.ESIRKEPOV)
+    worst_e = 0.0
+    for k, i in enumerate(sub):
+        e, g = pairs64[k]
+        d = P.diag(e, g, specs[i]["shape"])
+        e64 = d[P.DIAG_FIELD_ENERGY] + d[P.DIAG_KINETIC_ENERGY]
+        worst_e = max(worst_e, abs(energies32[i] - e64) / e64)
+    print("config-3 FP32 vs FP64 total energy after 100 steps (%d instances): worst %.3g" % (len(sub), worst_e),
+          flush=True)
+    assert worst_e < 1e-5
+    b64.close()
