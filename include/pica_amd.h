@@ -218,6 +218,12 @@ int pk_dn_set_params(pk_dn* d, const float* params);   /* also resets the Adam s
 int pk_dn_init(pk_dn* d, uint64_t seed);                 /* He-normal weights, zero biases */
 int pk_dn_get_params(pk_dn* d, float* params);
 int pk_dn_set_scale(pk_dn* d, double scale);            /* network units = scale x J (CGS) */
+/* charge filter S of pk_step_denoised (Esirkepov): a radial filter applied in Fourier space with
+ * gain g(|k|), |k| in radians per cell, given on nbins points evenly spaced over [0, kmax] (linear
+ * interpolation; the last value beyond kmax); gain[0] must be 1 (the filter keeps the total
+ * charge).  The projection then makes D J = -S (rho1 - rho0) / dt: continuity and Gauss's law hold
+ * for the filtered charge S rho.  nbins = 0 removes the filter (the particles' own charge). */
+int pk_dn_set_charge_gain(pk_dn* d, int nbins, double kmax, const double* gain);
 /* one forward pass on a host volume; y = network output [3][n] */
 int pk_dn_forward_host(pk_dn* d, const int n[3], const float* x, float* y);
 /* one convolution layer (1..3) on host input [cin][n]: pre-activation output [cout][n] */
@@ -242,7 +248,8 @@ int pk_dn_apply(pk_dn* d, pk_grid* g);
  * (rho_now - rho_saved)/dt + div J = 0; rho_now becomes the saved density. */
 int pk_rho_save(pk_ens* e, pk_grid* g, int shape);
 int pk_project_continuity(pk_ens* e, pk_grid* g, int shape, double dt);
-/* pk_step with the denoiser and the projection applied to J before the field update */
+/* pk_step with the denoiser and the projection applied to J before the field update (with
+ * Esirkepov deposition, continuity for the denoiser's filtered charge, pk_dn_set_charge_filter) */
 int pk_step_denoised(pk_ens* e, pk_grid* g, pk_dn* d, double dt, int shape, int dep);
 
 /* ---- the step ----

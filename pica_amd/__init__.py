@@ -58,7 +58,7 @@ ABI_SYMBOLS = (
     "pk_dn_layer_host", "pk_dn_grad_host", "pk_dn_grad_batch_host", "pk_dn_add_pair", "pk_dn_num_pairs",
     "pk_dn_clear_pairs", "pk_dn_train", "pk_dn_apply", "pk_rho_save", "pk_project_continuity", "pk_step_denoised", "pk_grid_save", "pk_grid_load",
     "pk_ens_save", "pk_ens_load", "pk_step_species", "pk_ens_interleave", "pk_diag_energies_static",
-    "pk_ens_set_supercells", "pk_batch_set_shape",
+    "pk_ens_set_supercells", "pk_batch_set_shape", "pk_dn_set_charge_gain",
 )
 
 _LIB = None
@@ -133,6 +133,7 @@ def lib():
         for name in ("pk_dn_set_params", "pk_dn_get_params"):
             getattr(L, name).argtypes = [vp, vp]
         L.pk_dn_set_scale.argtypes = [vp, ctypes.c_double]
+        L.pk_dn_set_charge_gain.argtypes = [vp, ctypes.c_int, ctypes.c_double, vp]
         L.pk_dn_init.argtypes = [vp, ctypes.c_uint64]
         L.pk_dn_forward_host.argtypes = [vp, vp, vp, vp]
         L.pk_dn_layer_host.argtypes = [vp, ctypes.c_int, vp, vp, vp]
@@ -558,6 +559,12 @@ class Denoiser:
 
     def apply(self, grid):
         self.ctx.check(lib().pk_dn_apply(self.h, grid.h))
+
+    def set_charge_gain(self, gain, kmax):
+        """radial charge filter of step_denoised: gain g(|k|) on len(gain) points over [0, kmax]
+        (radians per cell), gain[0] = 1; an empty gain removes the filter"""
+        g = np.ascontiguousarray(gain, np.float64)
+        self.ctx.check(lib().pk_dn_set_charge_gain(self.h, int(g.size), float(kmax), _p(g) if g.size else None))
 
     def close(self):
         if getattr(self, "h", None):

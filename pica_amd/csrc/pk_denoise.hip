@@ -55,6 +55,11 @@ struct pk_dn {
     float* part = nullptr;    // per-workgroup weight-gradient tiles (k_wgrad)
     double* loss = nullptr;   // device accumulator
     std::vector<pk_dn_pair> pairs;
+    // charge filter of pk_step_denoised: radial gain g(|k|) on nbins points over [0, kmax]
+    // (linear interpolation, device copy in cgain); cgain_n = 0: none
+    int cgain_n = 0;
+    double cgain_kmax = 0.0;
+    double* cgain = nullptr;
 };
 
 namespace {
@@ -724,6 +729,7 @@ void pk_dn_destroy(pk_dn* d)
         if (pr.low) (void)hipFree(pr.low);
         if (pr.high) (void)hipFree(pr.high);
     }
+    if (d->cgain) (void)hipFree(d->cgain);
     delete d;
 }
 
@@ -769,6 +775,32 @@ int pk_dn_init(pk_dn* d, uint64_t seed)
     }
     return pk_dn_set_params(d, p.data());
 }
+
+int pk_dn_set_charge_gain(pk_dn* d, int nbins, double kmax, const double* gain)
+{
+    if (!d) return PK_E_ARG;
+    pk_ctx* c = d->ctx;
+    if (nbins == 0) {
+        d->cgain_n = 0;
+        return PK_OK;
+    }
+    if (nbins < 2 || nbins > PK_DN_MAX_GAIN || !gain || !(kmax > 0.0) || !std::isfinite(kmax))
+        return pk_fail(c, PK_E_ARG, "pk_dn_set_charge_gain: need 2..1024 bins, kmax > 0 and a gain table");
+    for (int b = 0; b < nbins; b++)
+        if (!std::isfinite(gain[b])) return pk_fail(c, PK_E_ARG, "pk_dn_set_charge_gain: gains must be finite");
+    if (gain[0] != 1.0) return pk_fail(c, PK_E_ARG, "pk_dn_set_charge_gain: g(0) must be 1 (the filter keeps the charge)");
+    if (!d->cgain && hipMalloc((void**)&d->cgain, sizeof(double) * PK_DN_MAX_GAIN) != hipSuccess) {
+        d->cgain = nullptr;
+        (void)hipGetLastError();
+        return pk_fail(c, PK_E_ALLOC, "pk_dn_set_charge_gain: hipMalloc failed");
+    }
+    PK_HIP(c, hipStreamSynchronize(c->stream));
+    PK_HIP(c, hipMemcpy(d->cgain, gain, sizeof(double) * nbins, hipMemcpyHostToDevice));
+    d->cgain_n = nbins;
+    d->cgain_kmax = kmax;
+    return PK_OK;
+}
+
 
 int pk_dn_set_scale(pk_dn* d, double scale)
 {
@@ -1008,3 +1040,11 @@ int pk_dn_apply(pk_dn* d, pk_grid* g)
 }
 
 }  // extern "C"
+
+// the device gain table of pk_step_denoised's projection (n = 0: none)
+void pk_dn_charge_gain(const pk_dn* d, const double** gain, int* n, double* kmax)
+{
+    *gain = d->cgain;
+    *n = d->cgain_n;
+    *kmax = d->cgain_kmax;
+}

@@ -19,6 +19,7 @@
 #define PK_BLOCK 256           // threads per particle workgroup (4 wavefronts of 64)
 #define PK_XCD 8               // MI355X: 8 XCDs, workgroups are dealt round-robin over them
 #define PK_MAX_SC_PARTICLES 65536
+#define PK_DN_MAX_GAIN 1024     // entries of the denoiser's radial charge-filter gain table
 
 struct pk_timer_rec {
     double ms = 0.0;

@@ -3,7 +3,10 @@
 //
 // After J has been changed by anything that ignores charge conservation (the denoiser), the
 // projection subtracts a discrete gradient: J <- J - G phi with  L phi = D J + (rho1 - rho0)/dt
-// (or, when the current before the change was Esirkepov's, L phi = D (J - J_esirkepov)),
+// (or, when the current before the change was Esirkepov's, L phi = D J - S D J_esirkepov with S
+// the denoiser's charge filter, a fixed radial filter with gain g(|k|), g(0) = 1, applied in
+// Fourier space: then D J = -S (rho1 - rho0)/dt, so continuity and Gauss's law hold exactly for
+// the filtered charge S rho; without a filter S is the identity, the particles' own charge),
 // where D is the forward-difference divergence at the cell-centre nodes (the stencil under which
 // Esirkepov deposition conserves charge exactly), G = -D^T the backward-difference gradient onto
 // the J faces, and L = D G the 7-point periodic Laplacian.  Then D J = -(rho1 - rho0)/dt.  L is
@@ -15,6 +18,7 @@
 #include "pk_internal.h"
 
 int pk_dn_apply(pk_dn* d, pk_grid* g);
+void pk_dn_charge_gain(const pk_dn* d, const double** gain, int* n, double* kmax);
 
 namespace {
 
@@ -182,6 +186,44 @@ __global__ void k_correct(GridView<T> g, Phys P, const Cx<T>* __restrict__ w)
     g.f[PK_JZ][c] = (T)((double)g.f[PK_JZ][c] - (ph - pz) / g.step[2]);
 }
 
+// w2[v] = (d0[v], 0): the reference divergence as a complex volume
+template <class R>
+__global__ void k_pack(const double* __restrict__ d0, Cx<R>* __restrict__ w2, long nv)
+{
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < nv) w2[v] = Cx<R>{(R)d0[v], (R)0};
+}
+
+// in Fourier space: w += (1 - g(|k|)) w2, so that the residual D J - d0 becomes D J - S d0 with S
+// the radial filter of gain g (linear interpolation of nb values over [0, kmax], the last value
+// beyond); |k| in radians per cell, k_d = 2 pi f_d / n_d with f_d the signed frequency
+template <class R>
+__global__ void k_gain_combine(Cx<R>* __restrict__ w, const Cx<R>* __restrict__ w2, Phys P,
+                               const double* __restrict__ gain, int nb, double kmax)
+{
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= P.nv) return;
+    const int c[3] = {(int)(v / ((long)P.n[1] * P.n[2])), (int)((v / P.n[2]) % P.n[1]), (int)(v % P.n[2])};
+    double k2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const int f = c[d] <= P.n[d] / 2 ? c[d] : c[d] - P.n[d];
+        const double kd = 2.0 * M_PI * (double)f / (double)P.n[d];
+        k2 += kd * kd;
+    }
+    const double x = sqrt(k2) / kmax * (double)(nb - 1);
+    double gk;
+    if (x >= (double)(nb - 1)) {
+        gk = gain[nb - 1];
+    } else {
+        const int b = (int)x;
+        const double t = x - (double)b;
+        gk = gain[b] + t * (gain[b + 1] - gain[b]);
+    }
+    const R f = (R)(1.0 - gk);
+    w[v] = Cx<R>{w[v].x + f * w2[v].x, w[v].y + f * w2[v].y};
+}
+
 __global__ void k_copy_d(const double* __restrict__ a, double* __restrict__ b, long n)
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -202,7 +244,9 @@ double* jref_of(pk_grid* g, const Phys& P) { return g->proj + 2 * P.nv + 2 * (P.
 
 int ensure_proj(pk_grid* g, const Phys& P)
 {
-    const size_t need = 3 * (size_t)P.nv + 2 * (size_t)(P.n[0] + P.n[1] + P.n[2]) + 16;
+    // complex volume, twiddles and eigenvalues, the reference divergence, and a second complex
+    // volume for the charge filter
+    const size_t need = 5 * (size_t)P.nv + 2 * (size_t)(P.n[0] + P.n[1] + P.n[2]) + 16;
     if (g->proj && g->proj_n >= need) return PK_OK;
     PK_HIP(g->ctx, hipStreamSynchronize(g->ctx->stream));
     if (g->proj) (void)hipFree(g->proj);
@@ -259,10 +303,18 @@ int fft3(pk_grid* g, const Phys& P, Cx<R>* w, const double2* tw[3], int inverse,
     return PK_OK;
 }
 
-// rho1 != null: residual D J + (rho1 - rho_prev) / dt; else D J - d0 with the divergence d0 of
-// the reference current saved after the complex volume in g->proj
+// the charge filter of the residual: gain table (device), its size and range; n = 0: none
+struct ChargeGain {
+    const double* g = nullptr;
+    int n = 0;
+    double kmax = 0.0;
+};
+
+// rho1 != null: residual D J + (rho1 - rho_prev) / dt; else D J - S d0 with d0 the divergence of
+// the reference current saved after the complex volume in g->proj and S the radial charge filter
+// cg (none when cg.n = 0)
 template <class T>
-int project(pk_grid* g, const Phys& P, const double* rho1, double dt)
+int project(pk_grid* g, const Phys& P, const double* rho1, double dt, ChargeGain cg = ChargeGain())
 {
     hipStream_t s = g->ctx->stream;
     Cx<T>* w = (Cx<T>*)g->proj;
@@ -281,6 +333,12 @@ int project(pk_grid* g, const Phys& P, const double* rho1, double dt)
                            jref_of(g, P), w);
     int r;
     if ((r = fft3(g, P, w, tw, 0, nullptr))) return r;
+    if (!rho1 && cg.n > 0) {
+        Cx<T>* w2 = (Cx<T>*)(jref_of(g, P) + P.nv);
+        hipLaunchKernelGGL(k_pack<T>, dim3(nblk(P.nv, 256)), dim3(256), 0, s, jref_of(g, P), w2, P.nv);
+        if ((r = fft3(g, P, w2, tw, 0, nullptr))) return r;
+        hipLaunchKernelGGL(k_gain_combine<T>, dim3(nblk(P.nv, 256)), dim3(256), 0, s, w, w2, P, cg.g, cg.n, cg.kmax);
+    }
     if ((r = fft3(g, P, w, tw, 1, lam))) return r;
     hipLaunchKernelGGL(k_correct<T>, dim3(nblk(P.nv, 256)), dim3(256), 0, s, grid_view<T>(g), P, w);
     PK_HIP(g->ctx, hipGetLastError());
@@ -377,7 +435,9 @@ int pk_step_denoised(pk_ens* e, pk_grid* g, pk_dn* d, double dt, int shape, int 
         if ((r = pk_dn_apply(d, g))) return r;
         {
             pk_ktimer tm(e->ctx, "project");
-            r = g->prec == PK_F64 ? project<double>(g, P, nullptr, dt) : project<float>(g, P, nullptr, dt);
+            ChargeGain cg;
+            pk_dn_charge_gain(d, &cg.g, &cg.n, &cg.kmax);
+            r = g->prec == PK_F64 ? project<double>(g, P, nullptr, dt, cg) : project<float>(g, P, nullptr, dt, cg);
             if (r) return r;
         }
         if ((r = pk_field_ghost(g, PK_FILL_J))) return r;

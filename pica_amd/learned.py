@@ -5,9 +5,18 @@ Workload: periodic electron plasmas loaded with a few random long-wavelength vel
 (pk_ens_load_modes) on a thermal background, TSC shape, Esirkepov deposition, FP32 split
 positions, dt at Courant 0.5, omega_p dt = 0.1.  A training pair is the deposited J of a
 low-ppc run and of a high-ppc run of the same modes at the same step; the network maps the first
-to the second.  Evaluation on unseen modes compares, against a high-ppc reference run:
-  - the raw low-ppc J, the denoised low-ppc J (offline, no feedback) and a mid-ppc run's J;
-  - a low-ppc run stepped with pk_step_denoised (denoiser + continuity projection in the loop);
+to the second.  The charge filter: the projection of the denoised step fixes the longitudinal part of J by
+continuity, so without a filter that part keeps the low-ppc charge noise whatever the network
+does.  A radial filter S with gain g(|k|) is learned from the training pairs as the Wiener gain of
+the longitudinal current per |k| shell, g = sum Re(DJ_high conj DJ_low) / sum |DJ_low|^2 (DJ the
+Fourier transform of the divergence, weighted by 1 / |lambda|, lambda the Laplacian's eigenvalue),
+clipped to [0, 1], g(0) = 1; the denoised step then keeps continuity, and Gauss's law, for the
+filtered charge S rho exactly.
+
+Evaluation on unseen modes and held-out seeds compares, against a high-ppc reference run, the J
+error (total, and its longitudinal and transverse parts) and the field energy of:
+  - the raw low-ppc run, the denoised low-ppc J (offline, no feedback) and a mid-ppc run;
+  - a low-ppc run stepped with pk_step_denoised (denoiser + filtered-continuity projection);
 and times a mid-ppc plain step against a low-ppc denoised step.
 """
 import math
@@ -15,8 +24,8 @@ import time
 
 import numpy as np
 
-from . import (C, ELECTRON_CHARGE, ESIRKEPOV, F32, JX, JY, JZ, TSC, Denoiser, EnsembleSupercells, YeeGrid,
-               plasma_parameters, step, step_denoised)
+from . import (C, DIAG_FIELD_ENERGY, ELECTRON_CHARGE, ESIRKEPOV, F32, JX, JY, JZ, TSC, Denoiser,
+               EnsembleSupercells, YeeGrid, diag, plasma_parameters, step, step_denoised)
 
 UTH = 0.02        # thermal spread of u = p/(mc)
 U_AMP = 0.05      # velocity mode amplitude scale (units of c)
@@ -44,10 +53,10 @@ def network_scale():
     return 1.0 / (abs(ELECTRON_CHARGE) * n0 * C * U_AMP)
 
 
-def plasma(ctx, ncell, ppc, modes, seed):
+def plasma(ctx, ncell, ppc, modes, seed, ncps=(4, 4, 4)):
     _, w = plasma_parameters(DT, ppc)
     g = YeeGrid.periodic(ctx, ncell, prec=F32)
-    e = EnsembleSupercells(ctx, ncell, (1.0, 1.0, 1.0), (4, 4, 4), prec=F32)
+    e = EnsembleSupercells(ctx, ncell, (1.0, 1.0, 1.0), ncps, prec=F32)
     e.load_modes(ppc, UTH, 0.0, w, seed, modes)
     return e, g
 
@@ -62,8 +71,70 @@ def rel_l2(a, ref):
     return float(np.sqrt(np.sum((a - ref) ** 2) / np.sum(ref ** 2)))
 
 
-def generate_pairs(ctx, dn, runs, ncell=(32, 32, 32), ppc_low=4, ppc_high=256, steps=40, every=4, seed=0):
-    """runs x (steps / every) training pairs added to dn"""
+def _ops(n):
+    """Fourier symbols of D (forward differences), G = -D^T and L = D G on a periodic n-grid, unit steps"""
+    K = np.meshgrid(*[2.0 * np.pi * np.fft.fftfreq(m) for m in n], indexing="ij")
+    Dh = [np.exp(1j * k) - 1.0 for k in K]
+    Gh = [1.0 - np.exp(-1j * k) for k in K]
+    lam = sum(d * g for d, g in zip(Dh, Gh)).real
+    return K, Dh, Gh, lam
+
+
+def div_hat(j):
+    """Fourier transform of D J (the divergence at the cell-centre nodes) of a physical J [3][n]"""
+    _, Dh, _, _ = _ops(j.shape[1:])
+    return sum(Dh[c] * np.fft.fftn(j[c]) for c in range(3))
+
+
+def split_lt(j):
+    """longitudinal and transverse parts of J: J_L = G L^-1 D J, J_T = J - J_L (the split under
+    which the projection acts: it leaves J_T and sets D J)"""
+    _, Dh, Gh, lam = _ops(j.shape[1:])
+    lam = lam.copy()
+    lam.flat[0] = 1.0
+    phi = sum(Dh[c] * np.fft.fftn(j[c]) for c in range(3)) / lam
+    phi.flat[0] = 0.0
+    jl = np.stack([np.fft.ifftn(Gh[c] * phi).real for c in range(3)])
+    return jl, j - jl
+
+
+class ChargeGainFit:
+    """accumulates, over training pairs, the per-|k|-shell sums of the charge filter's Wiener gain:
+    A = sum |DJ_low|^2 / |lambda|, B = sum Re(DJ_high conj DJ_low) / |lambda|"""
+
+    def __init__(self, nbins=33):
+        self.nbins = nbins
+        self.kmax = float(np.pi * np.sqrt(3.0))
+        self.A = np.zeros(nbins)
+        self.B = np.zeros(nbins)
+
+    def add(self, j_low, j_high):
+        K, _, _, lam = _ops(j_low.shape[1:])
+        kabs = np.sqrt(sum(k * k for k in K))
+        nz = lam != 0
+        w = np.zeros_like(lam)
+        w[nz] = 1.0 / np.abs(lam[nz])
+        dl, dh = div_hat(j_low), div_hat(j_high)
+        b = np.clip(np.rint(kabs / self.kmax * (self.nbins - 1)).astype(int), 0, self.nbins - 1)
+        self.A += np.bincount(b[nz], (np.abs(dl) ** 2 * w)[nz], self.nbins)
+        self.B += np.bincount(b[nz], ((dh * np.conj(dl)).real * w)[nz], self.nbins)
+
+    def fit(self):
+        """(gain, kmax): gain on nbins points over [0, kmax], g(0) = 1, shells without data
+        interpolated from their neighbours"""
+        have = self.A > 0
+        g = np.where(have, self.B / np.where(have, self.A, 1.0), np.nan)
+        g = np.clip(g, 0.0, 1.0)
+        g[0] = 1.0
+        idx = np.arange(self.nbins)
+        ok = ~np.isnan(g)
+        g = np.interp(idx, idx[ok], g[ok])
+        return g, self.kmax
+
+
+def generate_pairs(ctx, dn, runs, ncell=(32, 32, 32), ppc_low=4, ppc_high=256, steps=40, every=4, seed=0,
+                   cfit=None):
+    """runs x (steps / every) training pairs added to dn (and to the charge-filter fit cfit)"""
     rng = np.random.default_rng(seed)
     for r in range(runs):
         modes = random_modes(rng, ncell)
@@ -74,6 +145,8 @@ def generate_pairs(ctx, dn, runs, ncell=(32, 32, 32), ppc_low=4, ppc_high=256, s
             step(*hi, DT, TSC, ESIRKEPOV)
             if s % every == 0:
                 dn.add_pair(lo[1], hi[1])
+                if cfit is not None:
+                    cfit.add(current(lo[1]), current(hi[1]))
         for obj in lo + hi:
             obj.close()
 
@@ -98,15 +171,17 @@ def timed_steps(fn, nsteps, sync):
     return (time.perf_counter() - t0) / nsteps
 
 
-def evaluate(ctx, dn, ncell=(128, 128, 128), ppc_low=4, ppc_mid=64, ppc_ref=256, steps=32, every=8, seed=99):
-    """errors against the ppc_ref run at every `every` steps, and step costs"""
+def evaluate(ctx, dn, ncell=(128, 128, 128), ppc_low=4, ppc_mid=64, ppc_ref=256, steps=32, every=8, seed=99,
+             low_ncps=(4, 4, 4)):
+    """errors against the ppc_ref run at every `every` steps (J total / longitudinal / transverse, and
+    the field energy), for one held-out seed"""
     rng = np.random.default_rng(seed)
     modes = random_modes(rng, ncell)
     scale = network_scale()
     ref = plasma(ctx, ncell, ppc_ref, modes, 7 * seed + 1)
-    low = plasma(ctx, ncell, ppc_low, modes, 7 * seed + 2)
+    low = plasma(ctx, ncell, ppc_low, modes, 7 * seed + 2, low_ncps)
     mid = plasma(ctx, ncell, ppc_mid, modes, 7 * seed + 3)
-    dnl = plasma(ctx, ncell, ppc_low, modes, 7 * seed + 2)      # same particles as `low`, denoised in the loop
+    dnl = plasma(ctx, ncell, ppc_low, modes, 7 * seed + 2, low_ncps)  # same particles as `low`, denoised in the loop
     rows = []
     for s in range(1, steps + 1):
         for e, g in (ref, low, mid):
@@ -115,21 +190,35 @@ def evaluate(ctx, dn, ncell=(128, 128, 128), ppc_low=4, ppc_mid=64, ppc_ref=256,
         if s % every == 0:
             jr, jl, jm, jd = current(ref[1]), current(low[1]), current(mid[1]), current(dnl[1])
             jo = dn.forward((jl * scale).astype(np.float32)).astype(np.float64) / scale
-            rows.append({"step": s, "raw_low": rel_l2(jl, jr), "denoised_low": rel_l2(jo, jr),
-                         "mid": rel_l2(jm, jr), "denoised_in_loop": rel_l2(jd, jr)})
+            rl, rt = split_lt(jr)
+            row = {"step": s}
+            for name, j in (("raw_low", jl), ("denoised_low", jo), ("mid", jm), ("denoised_in_loop", jd)):
+                a, b = split_lt(j)
+                row[name] = rel_l2(j, jr)
+                row[name + "_L"] = rel_l2(a, rl)
+                row[name + "_T"] = rel_l2(b, rt)
+            wref = diag(ref[0], ref[1], TSC)[DIAG_FIELD_ENERGY]
+            for name, (e, g) in (("raw_low", low), ("mid", mid), ("denoised_in_loop", dnl)):
+                row[name + "_field_energy"] = abs(diag(e, g, TSC)[DIAG_FIELD_ENERGY] - wref) / wref
+            rows.append(row)
     for obj in ref + low + mid + dnl:
         obj.close()
-    # step costs on fresh plasmas
+    return rows
+
+
+def costs(ctx, dn, ncell=(128, 128, 128), ppc_low=4, ppc_mid=64, low_ncps=(4, 4, 4), seed=5):
+    """ms per step of a mid-ppc plain step and of a low-ppc denoised step (fresh plasmas), with the
+    per-kernel HIP-event times of the denoised step"""
+    modes = random_modes(np.random.default_rng(seed), ncell)
     cost = {}
     mid = plasma(ctx, ncell, ppc_mid, modes, 1)
     cost["mid_plain_ms"] = 1e3 * timed_steps(lambda: step(*mid, DT, TSC, ESIRKEPOV), 10, ctx.sync)
     for obj in mid:
         obj.close()
-    low = plasma(ctx, ncell, ppc_low, modes, 2)
+    low = plasma(ctx, ncell, ppc_low, modes, 2, low_ncps)
     cost["low_plain_ms"] = 1e3 * timed_steps(lambda: step(*low, DT, TSC, ESIRKEPOV), 10, ctx.sync)
     cost["low_denoised_ms"] = 1e3 * timed_steps(lambda: step_denoised(low[0], low[1], dn, DT, TSC, ESIRKEPOV), 10,
                                                 ctx.sync)
-    # per-kernel HIP-event times of the denoised step
     ctx.timer_reset()
     ctx.timer_enable(True)
     for _ in range(5):
@@ -137,31 +226,42 @@ def evaluate(ctx, dn, ncell=(128, 128, 128), ppc_low=4, ppc_mid=64, ppc_ref=256,
     ctx.sync()
     ctx.timer_enable(False)
     cost["low_denoised_kernels_ms"] = {k: ctx.timer(k)[0] / 5 for k in ("particles", "insert", "denoise", "project",
-                                                                        "yee_e", "yee_b", "ghost", "zero")}
+                                                                        "yee_e", "yee_b", "ghost", "zero",
+                                                                        "interleave")}
     for obj in low:
         obj.close()
     cost["mid_over_low_denoised"] = cost["mid_plain_ms"] / cost["low_denoised_ms"]
-    mean = {k: float(np.mean([r[k] for r in rows])) for k in ("raw_low", "denoised_low", "mid", "denoised_in_loop")}
-    return {"grid": list(ncell), "ppc": {"low": ppc_low, "mid": ppc_mid, "reference": ppc_ref}, "snapshots": rows,
-            "mean_rel_l2_vs_reference": mean, "cost": cost}
+    cost["low_supercells"] = list(low_ncps)
+    return cost
 
 
 def run(ctx, train_runs=48, train_grid=(32, 32, 32), eval_grid=(128, 128, 128), schedule=None, seed=0,
-        ppc_target=256):
+        ppc_target=256, eval_seeds=(99, 199, 299)):
     """the whole learned-form pipeline; returns (report dict, trained Denoiser)"""
     dn = Denoiser(ctx)
     dn.set_scale(network_scale())
     dn.init(seed + 17)
+    cfit = ChargeGainFit()
     t0 = time.perf_counter()
-    generate_pairs(ctx, dn, train_runs, train_grid, ppc_high=ppc_target, seed=seed)
+    generate_pairs(ctx, dn, train_runs, train_grid, ppc_high=ppc_target, seed=seed, cfit=cfit)
     t1 = time.perf_counter()
     losses = train(dn, **({"schedule": schedule} if schedule else {}))
+    gain, kmax = cfit.fit()
+    dn.set_charge_gain(gain, kmax)
     ctx.sync()
     t2 = time.perf_counter()
     report = {"train": {"grid": list(train_grid), "ppc": {"input": 4, "target": ppc_target}, "pairs": dn.num_pairs(),
                         "iters": int(losses.size),
                         "loss_first": float(np.mean(losses[:20])), "loss_last": float(np.mean(losses[-20:])),
+                        "charge_gain": {"kmax": kmax, "gain": [round(float(v), 4) for v in gain]},
                         "data_s": t1 - t0, "train_s": t2 - t1}}
-    report["eval"] = evaluate(ctx, dn, eval_grid)
     dn.clear_pairs()
+    rows = []
+    for sd in eval_seeds:
+        rows += [dict(r, seed=sd) for r in evaluate(ctx, dn, eval_grid, seed=sd)]
+    keys = [k for k in rows[0] if k not in ("step", "seed")]
+    report["eval"] = {"grid": list(eval_grid), "ppc": {"low": 4, "mid": 64, "reference": 256},
+                      "held_out_seeds": list(eval_seeds), "snapshots": rows,
+                      "mean_rel_l2_vs_reference": {k: float(np.mean([r[k] for r in rows])) for k in keys},
+                      "cost": costs(ctx, dn, eval_grid)}
     return report, dn

@@ -263,3 +263,125 @@ def test_mode_loader(ctx):
     print("mode loader: thermal mean %s, std %s" % (th.mean(0), th.std(0)), flush=True)
     assert np.all(np.abs(th.mean(0)) < 5 * uth / math.sqrt(x.shape[0]))
     assert np.all(np.abs(th.std(0) / uth - 1) < 0.02)
+
+
+def _radial_filter(a, gain, kmax):
+    """the denoiser's charge filter S on a periodic physical array: gain g(|k|) (linear
+    interpolation of the table over [0, kmax], the last value beyond), k_d = 2 pi f_d / n_d"""
+    K = np.meshgrid(*[2.0 * np.pi * np.fft.fftfreq(n) for n in a.shape], indexing="ij")
+    x = np.sqrt(sum(k * k for k in K)) / kmax * (len(gain) - 1)
+    g = np.interp(x, np.arange(len(gain)), gain)
+    return np.fft.ifftn(np.fft.fftn(a) * g).real
+
+
+GAIN = np.array([1.0, 1.0, 0.95, 0.8, 0.5, 0.25, 0.1, 0.03, 0.0, 0.0])
+
+
+@pytest.mark.parametrize("prec,tol", [(P.F64, 1e-11), (P.F32, 2e-4)])
+def test_denoised_step_filtered_charge_continuity(ctx, dn, prec, tol):
+    """with a charge filter S (pk_dn_set_charge_gain), a denoised step satisfies
+    (S rho1 - S rho0)/dt + D J = 0 at every node, and over several steps div E - 4 pi S rho keeps its
+    initial value (Gauss's law for the filtered charge); removing the filter restores continuity
+    with the particles' own charge"""
+    _say("filtered-charge continuity prec=%d" % prec)
+    ncell, dt = (16, 16, 32), courant_dt()
+    n0, w0 = plasma(dt, 4)
+    g = P.YeeGrid.periodic(ctx, ncell, prec=prec)
+    e = P.EnsembleSupercells(ctx, ncell, (1.0, 1.0, 1.0), (4, 4, 4), prec=prec)
+    e.load_modes(4, 0.02, 0.0, w0, 12, [[1, 0, 0, 0.05, 0, 0, 0.3], [0, 1, 1, 0, 0.03, 0.02, 1.1]])
+    dn.set_params(_params(8))
+    dn.set_scale(1.0 / (abs(P.ELECTRON_CHARGE) * n0 * P.C * 0.05))
+    kmax = math.pi * math.sqrt(3.0)
+    dn.set_charge_gain(GAIN, kmax)
+    G = O.GHOST
+    phys = lambda a: np.ascontiguousarray(a[G:-G, G:-G, G:-G], np.float64)
+
+    def rho_now():
+        P.deposit_rho(e, g, P.TSC)
+        return phys(g.get(P.RHO))
+
+    def gauss_filtered():
+        ex, ey, ez = (g.get(c).astype(np.float64) for c in (P.EX, P.EY, P.EZ))
+        dive = O.physical_nodes(O.divergence(ex, ey, ez, (1.0, 1.0, 1.0)))
+        return dive - 4.0 * O.PI * _radial_filter(rho_now(), GAIN, kmax)
+
+    r0 = gauss_filtered()
+    rho0 = rho_now()
+    P.step_denoised(e, g, dn, dt, P.TSC, P.ESIRKEPOV)
+    rho1 = rho_now()
+    jx, jy, jz = (g.get(c).astype(np.float64) for c in (P.JX, P.JY, P.JZ))
+    divj = O.physical_nodes(O.divergence(jx, jy, jz, (1.0, 1.0, 1.0)))
+    res = (_radial_filter(rho1, GAIN, kmax) - _radial_filter(rho0, GAIN, kmax)) / dt + divj
+    plain = (rho1 - rho0) / dt + divj
+    cont, unfiltered = np.abs(res).max() / np.abs(divj).max(), np.abs(plain).max() / np.abs(divj).max()
+    for _ in range(4):
+        P.step_denoised(e, g, dn, dt, P.TSC, P.ESIRKEPOV)
+    drift = np.abs(gauss_filtered() - r0).max() / (4.0 * O.PI * np.abs(rho1).max())
+    print("filtered charge prec=%d: continuity %.3g (vs the particles' charge %.3g), Gauss drift %.3g" %
+          (prec, cont, unfiltered, drift), flush=True)
+    assert cont < tol and unfiltered > 1e-3 and drift < tol
+    # without the filter: continuity with the particles' charge again
+    dn.set_charge_gain([], 1.0)
+    rho0 = rho_now()
+    P.step_denoised(e, g, dn, dt, P.TSC, P.ESIRKEPOV)
+    rho1 = rho_now()
+    jx, jy, jz = (g.get(c).astype(np.float64) for c in (P.JX, P.JY, P.JZ))
+    divj = O.physical_nodes(O.divergence(jx, jy, jz, (1.0, 1.0, 1.0)))
+    assert np.abs((rho1 - rho0) / dt + divj).max() / np.abs(divj).max() < tol
+    e.close()
+    g.close()
+
+
+@pytest.mark.parametrize("prec,tol", [(P.F64, 1e-11), (P.F32, 2e-4)])
+def test_learned_form_at_128cube(ctx, dn, prec, tol):
+    """config 4's grid (128^3, 4 ppc, TSC): the continuity projection of a direct-deposited current
+    equals the oracle's FFT solve, and 5 denoised steps (filtered charge) keep Gauss's law for
+    the filtered charge"""
+    _say("learned form 128^3 prec=%d" % prec)
+    ncell, dt = (128, 128, 128), courant_dt()
+    n0, w0 = plasma(dt, 4)
+    g = P.YeeGrid.periodic(ctx, ncell, prec=prec)
+    e = P.EnsembleSupercells(ctx, ncell, (1.0, 1.0, 1.0), (4, 4, 4), prec=prec)
+    e.load_modes(4, 0.02, 0.0, w0, 13, [[2, 1, 0, 0.05, 0.01, 0, 0.3], [0, 3, 1, 0, 0.03, 0.02, 1.1]])
+    G = O.GHOST
+    phys = lambda a: np.ascontiguousarray(a[G:-G, G:-G, G:-G], np.float64)
+    # projection against the oracle (direct deposition breaks continuity)
+    P.run(e, g, dt, 1, P.TSC, P.DIRECT)
+    P.deposit_rho(e, g, P.TSC)
+    rho0 = phys(g.get(P.RHO))
+    P.rho_save(e, g, P.TSC)
+    for c in (P.JX, P.JY, P.JZ):
+        g.zero(c)
+    P.particle_step(e, g, dt, P.TSC, P.DIRECT)
+    g.ghost_ops(P.FOLD_J | P.FILL_J)
+    j_raw = [phys(g.get(c)) for c in (P.JX, P.JY, P.JZ)]
+    P.project_continuity(e, g, dt, P.TSC)
+    j_new = [phys(g.get(c)) for c in (P.JX, P.JY, P.JZ)]
+    P.deposit_rho(e, g, P.TSC)
+    rho1 = phys(g.get(P.RHO))
+    ref = D.project_continuity(*j_raw, rho0, rho1, dt, (1.0, 1.0, 1.0))
+    err = max(rel_err(a, b) for a, b in zip(j_new, ref))
+    # denoised steps with the filtered charge: Gauss's law for S rho
+    dn.set_params(_params(8))
+    dn.set_scale(1.0 / (abs(P.ELECTRON_CHARGE) * n0 * P.C * 0.05))
+    kmax = math.pi * math.sqrt(3.0)
+    dn.set_charge_gain(GAIN, kmax)
+
+    def gauss_filtered():
+        P.deposit_rho(e, g, P.TSC)
+        ex, ey, ez = (g.get(c).astype(np.float64) for c in (P.EX, P.EY, P.EZ))
+        dive = O.physical_nodes(O.divergence(ex, ey, ez, (1.0, 1.0, 1.0)))
+        rho = phys(g.get(P.RHO))
+        return dive - 4.0 * O.PI * _radial_filter(rho, GAIN, kmax), np.abs(rho).max()
+
+    r0, _ = gauss_filtered()
+    for _ in range(5):
+        P.step_denoised(e, g, dn, dt, P.TSC, P.ESIRKEPOV)
+    r1, rmax = gauss_filtered()
+    drift = np.abs(r1 - r0).max() / (4.0 * O.PI * rmax)
+    print("128^3 prec=%d: projection vs oracle %.3g, filtered Gauss drift %.3g" % (prec, err, drift), flush=True)
+    assert err < (1e-12 if prec == P.F64 else 1e-5)
+    assert drift < tol
+    dn.set_charge_gain([], 1.0)
+    e.close()
+    g.close()
