@@ -80,6 +80,13 @@ private:
     pk_ctx* h_ = nullptr;
 };
 
+// the context of device 0 that pica_compat.hpp's objects share (created on first use, lives until exit)
+inline Context& defaultContext()
+{
+    static Context ctx(0);
+    return ctx;
+}
+
 // pica::YeeGrid<Three, Real>(origin, step, size) (YeeGrid.h:201-202); Real = double or float
 template <class Real>
 class YeeGrid {

@@ -60,3 +60,11 @@ int main() { all<double>(); all<float>(); return 0; }
     r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), "-x", "c++", "-"],
                        input=src, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_pica_compat_surface_compiles():
+    """include/pica_amd/pica_compat.hpp (pica's own namespace and signatures) compiles the pica
+    caller tests/cxx/pica_loop.cpp with only its includes and particle loop switched"""
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-DUSE_PICA_AMD", "-I", os.path.join(ROOT, "include"),
+                        os.path.join(ROOT, "tests", "cxx", "pica_loop.cpp")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

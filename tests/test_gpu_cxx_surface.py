@@ -53,3 +53,43 @@ def test_cxx_surface_matches_oracle(tmp_path):
     assert rel_err(p[:, 3:6], oe.momenta()) < 1e-12
     for c in range(9):
         assert rel_err(fld[c], og.f[c]) < 1e-12
+
+
+def test_pica_caller_switched_by_includes(tmp_path):
+    """tests/cxx/pica_loop.cpp, a pica caller written against pica's API, built with
+    -DUSE_PICA_AMD: only its includes and its particle loop differ (pica_compat.hpp supplies
+    pica's YeeGrid<Three>, element accessors, Particle3d, ParticleArraySoA, EnsembleSupercells,
+    YeeSolver, ParticleTypes).  Its result equals the oracle's CIC / direct run (particle order
+    included: the library re-sorts as the oracle does) and, where oracle/_ref/pica_loop was built
+    from pica's headers, the same program run on pica itself (as a set)."""
+    print("[case] pica caller via pica_compat.hpp", flush=True)
+    from pica_loop_io import case, oracle_run, read_output, write_input
+    exe = tmp_path / "pica_loop"
+    libdir = os.path.dirname(P.LIBPATH)
+    subprocess.run(["g++", "-std=c++17", "-O2", "-DUSE_PICA_AMD", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "cxx", "pica_loop.cpp"), "-o", str(exe), "-L", libdir, "-lpica_amd",
+                    "-Wl,-rpath," + libdir], check=True)
+    ncell, ncps, x, u, w, dt = case()
+    write_input(tmp_path / "in.bin", ncell, ncps, x, u, w)
+    r = subprocess.run([str(exe), str(tmp_path / "in.bin"), str(tmp_path / "out.bin"), "6"], capture_output=True,
+                       text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    p, f = read_output(tmp_path / "out.bin", ncell)
+    og, oe = oracle_run(ncell, ncps, x, u, w, dt, 6)
+    assert np.array_equal(p[:, 6], oe.w), "particle order differs from the oracle's"
+    assert rel_err(p[:, :3], oe.positions()) < 1e-12 and rel_err(p[:, 3:6], oe.momenta()) < 1e-12
+    for c in range(9):
+        assert rel_err(f[c], og.f[c]) < 1e-12, "component %d" % c
+    ref = os.path.join(ROOT, "oracle", "_ref", "pica_loop")
+    if os.path.exists(ref):
+        r = subprocess.run([ref, str(tmp_path / "in.bin"), str(tmp_path / "ref.bin"), "6"], capture_output=True,
+                           text=True, timeout=120)
+        assert r.returncode == 0
+        q, g = read_output(tmp_path / "ref.bin", ncell)
+        ia, ib = np.argsort(p[:, 6]), np.argsort(q[:, 6])
+        assert np.array_equal(p[ia, 6], q[ib, 6])
+        assert rel_err(p[ia, :6], q[ib, :6]) < 1e-12
+        for c in range(9):
+            assert rel_err(f[c], g[c]) < 1e-12, "component %d vs pica" % c
+        print("pica_compat run equals pica's own run of the same program", flush=True)
