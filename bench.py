@@ -16,14 +16,19 @@ roofline: the fused particle kernel, algorithmic bytes per particle-step (FP64:
 104 + 72/ppc) x particles / its mean launch time from HIP events on the library's stream.
 cpu_baseline: the CPU oracle (OpenMP restatement of pica's path) on a bounded sample of the
 same workload, on this host's cores.
-batched: 64 independent periodic plasmas with edges drawn from {32,48,64,96,128} and ppc from
-{4,8,16,32} (4.8e8 particles, FP32 split positions, TSC, Esirkepov), and 512 small ones (edges
-{16,24,32}, ppc {4,8}), each stepped one by one (pk_step per plasma) and as one batch
-(pk_batch_run after pk_batch_autotune).
-learned: the current denoiser trained on 32^3 pairs of 4- and 256-ppc runs (48 runs x 10 snapshots,
-32,000 Adam steps), evaluated at 128^3
-against a 256-ppc reference: J error of 4 ppc raw, denoised, and 64 ppc, and the cost of a 64-ppc
-step over a 4-ppc denoised step.
+batched: the batching form's workload, 64 independent periodic plasmas with cubic edges drawn
+from {32,48,64,96,128}, ppc from {4,8,16,32}, drift speeds uniform in [0,0.2]c and CIC or TSC per
+plasma (6.3e8 particles, FP32 split positions, Esirkepov), 100 steps, stepped one by one (pk_step
+per plasma) and as one batch (pk_batch_run after pk_batch_autotune over supercell size, interleave
+period and launch settings); and 512 small plasmas (edges {16,24,32}, ppc {4,8}), 20 steps.
+learned: the current denoiser and its charge filter trained on 32^3 pairs of 4- and 256-ppc runs
+(48 runs x 10 snapshots, 32,000 Adam steps), evaluated at 128^3 on three held-out seeds against a
+256-ppc reference: J error (total, longitudinal, transverse) and field-energy error of 4 ppc raw,
+denoised and 64 ppc, and the cost of a 64-ppc step over a 4-ppc denoised step.
+roofline extras: the FDTD kernels against the HBM roofline (240 B FP64 / 120 B FP32 per
+cell-update), and the particle kernel's LDS and global atomic rates against the chip's peaks, from
+the per-launch atomic counts of the rocprofv3 summary in profiles/ (commit stamped) and this run's
+kernel time.
 """
 import argparse
 import json
@@ -48,14 +53,26 @@ def alg_bytes(prec, ppc):
     return 60.0 + 36.0 / ppc          # read cell,off,u,w 32 B + write cell,off,u 28 B; FP32 grid
 
 
-def measured_traffic_per_particle(prec):
-    """HBM bytes per particle of one particle-kernel launch, from the rocprofv3 FETCH_SIZE /
-    WRITE_SIZE runs summarised in profiles/ (calibrated against a plain copy); None if absent"""
-    path = os.path.join(ROOT, "profiles", "particles_tsc_esirkepov_256cube.md")
+PROFILE = os.path.join("profiles", "particles_tsc_esirkepov_256cube.md")
+# FDTD algorithmic bytes per cell-update (B/2, E, B/2): 12 + 9 + 9 values read or written
+FDTD_BYTES = {"f64": 240.0, "f32": 120.0}
+# peaks of the atomic rates: LDS bandwidth 128 B per clock per CU (a 64-lane FP64 atomic moves
+# 512 B), 256 CUs at 2.4 GHz; and the measured conflict-free rate of ds_add_f64 on this part
+LDS_F64_ATOMIC_PEAK = 256 * 2.4e9 * 128 / 512          # wave instructions per second
+LDS_F64_ATOMIC_MEASURED = 7.1e10
+
+
+def profile_value(key):
+    """a `key: value` line of the profile summary in profiles/ (None if absent); the summary
+    names the commit it was measured at in its `commit:` line"""
     try:
-        for line in open(path):
-            if line.startswith("traffic_bytes_per_particle_%s:" % prec):
-                return float(line.split(":")[1])
+        for line in open(os.path.join(ROOT, PROFILE)):
+            if line.startswith(key + ":"):
+                v = line.split(":", 1)[1].strip()
+                try:
+                    return float(v)
+                except ValueError:
+                    return v
     except OSError:
         pass
     return None
@@ -118,14 +135,30 @@ def run_gpu(args, prec):
         mean_s = kernels["particles"]["ms_mean"] * 1e-3
         ach = alg_bytes(prec, args.ppc) * npart / mean_s
         out["particle_kernel_rate"] = npart / mean_s
-        tpp = measured_traffic_per_particle(prec) if (args.shape, args.dep, args.n, args.ppc, args.sc) == \
-            ("tsc", "esirkepov", 256, 16, 4) else None
+        headline = (args.shape, args.dep, args.n, args.ppc, args.sc) == ("tsc", "esirkepov", 256, 16, 4)
+        tpp = profile_value("traffic_bytes_per_particle_%s" % prec) if headline else None
         out["roofline"] = {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                           "frac": ach / HBM_PEAK, "traffic": tpp * npart if tpp else None}
+                           "frac": ach / HBM_PEAK, "traffic": tpp * npart if tpp else None,
+                           "traffic_source": "%s at commit %s" % (PROFILE, profile_value("commit")) if tpp else None}
+        # atomic rates: per-launch counts from the profile, this run's mean kernel time
+        lds = profile_value("lds_atomic_wave_instr_per_particle_%s" % prec) if headline else None
+        glb = profile_value("l2_atomic_requests_per_particle_%s" % prec) if headline else None
+        if lds or glb:
+            out["atomics"] = {"source": "%s at commit %s" % (PROFILE, profile_value("commit"))}
+        if lds:
+            rate = lds * npart / mean_s
+            out["atomics"]["lds_f64_atomic_wave_instr_per_s"] = rate
+            out["atomics"]["lds_frac_of_bandwidth_peak"] = rate / LDS_F64_ATOMIC_PEAK
+            out["atomics"]["lds_frac_of_measured_conflict_free_rate"] = rate / LDS_F64_ATOMIC_MEASURED
+        if glb:
+            out["atomics"]["l2_atomic_requests_per_s"] = glb * npart / mean_s
     if "yee_e" in kernels and "yee_b" in kernels:
-        # FDTD cell updates per second of the B/2, E, B/2 kernels alone
+        # FDTD cell updates per second of the B/2, E, B/2 kernels alone, and their HBM roofline
         fd = (kernels["yee_e"]["ms_mean"] + 2 * kernels["yee_b"]["ms_mean"]) * 1e-3
         out["fdtd_cell_updates_per_s"] = float(np.prod(ncell)) / fd
+        a = out["fdtd_cell_updates_per_s"] * FDTD_BYTES[prec]
+        out["fdtd_roofline"] = {"bound": "hbm", "achieved": a / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                                "frac": a / HBM_PEAK, "bytes_per_cell_update": FDTD_BYTES[prec]}
     ens.close()
     grid.close()
     ctx.close()
@@ -133,55 +166,81 @@ def run_gpu(args, prec):
 
 
 def batch_specs(n=64, seed=3, edges=(32, 48, 64, 96, 128), ppcs=(4, 8, 16, 32)):
-    """a batched workload: (edge, ppc, seed) of each plasma"""
+    """the batching form's plasmas: per plasma a cubic edge and a ppc drawn uniformly from the
+    sets, a drift speed uniform in [0, 0.2]c (as u = v / sqrt(1 - v^2)), CIC or TSC with equal odds
+    and its own seed (the draw of tests/helpers.config3_specs)"""
     rng = np.random.default_rng(seed)
-    e = rng.choice(list(edges), n)
-    p = rng.choice(list(ppcs), n)
-    return [(int(a), int(b), 1000 + i) for i, (a, b) in enumerate(zip(e, p))]
+    out = []
+    for i in range(n):
+        v = 0.2 * rng.random()
+        out.append({"edge": int(rng.choice(edges)), "ppc": int(rng.choice(ppcs)), "drift": v / math.sqrt(1.0 - v * v),
+                    "shape": int(rng.integers(0, 2)), "seed": 1000 + i})
+    return out
 
 
-def run_batched(args, specs):
-    """one-by-one vs batched stepping of a batched workload; aggregate particle-steps/s"""
+def run_batched(args, specs, nsteps):
+    """one-by-one vs batched stepping of a batched workload (FP32, Esirkepov, each plasma its own
+    form factor and drift), both over the same steps from the same seeded load; aggregate
+    particle-steps/s"""
     import pica_amd as P
     ctx = P.Context(0)
     dt = 0.5 / P.C
-    shape = P.TSC if args.shape == "tsc" else P.CIC
-    dep = P.ESIRKEPOV if args.dep == "esirkepov" else P.DIRECT
-    pairs = []
-    for edge, ppc, seed in specs:
-        n0, w0 = P.plasma_parameters(dt, ppc)
-        g = P.YeeGrid.periodic(ctx, (edge,) * 3, prec=P.F32)
-        e = P.EnsembleSupercells(ctx, (edge,) * 3, (1.0, 1.0, 1.0), (args.sc,) * 3, prec=P.F32)
-        e.load_thermal(ppc, 0.01, 0.1 / math.sqrt(0.99), w0, seed)
-        pairs.append((e, g))
+
+    def load():
+        pairs = []
+        for sp in specs:
+            n0, w0 = P.plasma_parameters(dt, sp["ppc"])
+            g = P.YeeGrid.periodic(ctx, (sp["edge"],) * 3, prec=P.F32)
+            e = P.EnsembleSupercells(ctx, (sp["edge"],) * 3, (1.0, 1.0, 1.0), (args.sc,) * 3, prec=P.F32)
+            e.load_thermal(sp["ppc"], 0.01, sp["drift"], w0, sp["seed"])
+            pairs.append((e, g))
+        return pairs
+
+    pairs = load()
     npart = sum(e.size() for e, _ in pairs)
-    out = {"instances": len(pairs), "particles": npart, "steps": args.batch_steps}
-    for e, g in pairs:
-        P.step(e, g, dt, shape, dep)
+    shapes = [sp["shape"] for sp in specs]
+    out = {"instances": len(pairs), "particles": npart, "steps": nsteps,
+           "form_factors": {"cic": shapes.count(P.CIC), "tsc": shapes.count(P.TSC)},
+           "edges": sorted({sp["edge"] for sp in specs}), "ppc": sorted({sp["ppc"] for sp in specs}),
+           "drift_c": [round(min(sp["drift"] / math.sqrt(1 + sp["drift"] ** 2) for sp in specs), 4),
+                       round(max(sp["drift"] / math.sqrt(1 + sp["drift"] ** 2) for sp in specs), 4)]}
+    for (e, g), sh in zip(pairs, shapes):
+        P.step(e, g, dt, sh, P.ESIRKEPOV)
     ctx.sync()
     t0 = time.perf_counter()
-    for _ in range(args.batch_steps):
-        for e, g in pairs:
-            P.step(e, g, dt, shape, dep)
+    for _ in range(nsteps):
+        for (e, g), sh in zip(pairs, shapes):
+            P.step(e, g, dt, sh, P.ESIRKEPOV)
     ctx.sync()
     t1 = time.perf_counter()
-    out["one_by_one"] = {"value": npart * args.batch_steps / (t1 - t0), "ms_per_step": 1e3 * (t1 - t0) / args.batch_steps}
+    out["one_by_one"] = {"value": npart * nsteps / (t1 - t0), "ms_per_step": 1e3 * (t1 - t0) / nsteps}
     print("batched workload one by one: %.3g particle-steps/s" % out["one_by_one"]["value"], file=sys.stderr, flush=True)
+    for e, g in pairs:
+        e.close()
+        g.close()
+    pairs = load()
     b = P.Batch(ctx, pairs)
-    tuned_ms = b.autotune(dt, shape, dep, trial_steps=2)
+    for i, sh in enumerate(shapes):
+        b.set_shape(i, sh)
+    tuned_ms = b.autotune(dt, P.SHAPE_PER_INSTANCE, P.ESIRKEPOV, trial_steps=2)
     knobs = {k: b.get(v) for k, v in (("wg_per_cu", P.BATCH_WG_PER_CU), ("order", P.BATCH_ORDER),
-                                         ("graph", P.BATCH_GRAPH))}
-    b.run(dt, 1, shape, dep)
+                                         ("graph", P.BATCH_GRAPH), ("supercell", P.BATCH_SUPERCELL),
+                                         ("interleave", P.BATCH_INTERLEAVE))}
+    b.run(dt, 1, P.SHAPE_PER_INSTANCE, P.ESIRKEPOV)
     ctx.sync()
+    ctx.timer_reset()
+    ctx.timer_enable(True)         # only the rebuild count is read: the batched step records no events
     t0 = time.perf_counter()
-    b.run(dt, args.batch_steps, shape, dep)
+    b.run(dt, nsteps, P.SHAPE_PER_INSTANCE, P.ESIRKEPOV)
     ctx.sync()
     t1 = time.perf_counter()
-    out["batched"] = {"value": npart * args.batch_steps / (t1 - t0), "ms_per_step": 1e3 * (t1 - t0) / args.batch_steps,
-                      "autotuned_knobs": knobs, "autotune_trial_ms_per_step": tuned_ms}
+    ctx.timer_enable(False)
+    out["batched"] = {"value": npart * nsteps / (t1 - t0), "ms_per_step": 1e3 * (t1 - t0) / nsteps,
+                      "autotuned_knobs": knobs, "autotune_trial_ms_per_step": tuned_ms,
+                      "slack_rebuilds": ctx.timer("rebuild")[1]}
     out["speedup"] = out["batched"]["value"] / out["one_by_one"]["value"]
-    print("batched workload as one batch: %.3g particle-steps/s" % out["batched"]["value"], file=sys.stderr,
-          flush=True)
+    print("batched workload as one batch: %.3g particle-steps/s (knobs %s)" % (out["batched"]["value"], knobs),
+          file=sys.stderr, flush=True)
     b.close()
     for e, g in pairs:
         e.close()
@@ -199,10 +258,36 @@ def run_learned():
     dn.close()
     ctx.close()
     ev = report["eval"]
-    print("learned: mean rel L2 vs %d ppc %s, cost ratio %.2f" % (ev["ppc"]["reference"], ev["mean_rel_l2_vs_reference"],
-                                                                  ev["cost"]["mid_over_low_denoised"]),
+    m = ev["mean_rel_l2_vs_reference"]
+    print("learned: J error vs %d ppc: 4 ppc raw %.4f, denoised in the loop %.4f, 64 ppc %.4f; cost ratio %.2f"
+          % (ev["ppc"]["reference"], m["raw_low"], m["denoised_in_loop"], m["mid"], ev["cost"]["mid_over_low_denoised"]),
           file=sys.stderr, flush=True)
     return report
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_threads():
+    return int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+
+
+def cpu_load(n, ppc, sc, seed):
+    """the device loader's particles, made by its CPU twin in the oracle (identical inputs)"""
+    import oracle as O
+    dt = 0.5 / O.C
+    wp = 0.1 / dt
+    n0 = wp * wp * O.E_MASS / (4.0 * O.PI * O.E_CHARGE * O.E_CHARGE)
+    x, u, w, _ = O.load_thermal((n, n, n), (sc,) * 3, (1.0, 1.0, 1.0), ppc, 0.01, 0.1 / math.sqrt(0.99), n0 / ppc,
+                                seed, False)
+    return x, u, w, dt
 
 
 def cpu_reference(args):
@@ -211,58 +296,43 @@ def cpu_reference(args):
     import oracle as O
     if O.ref() is None:
         return None
-    rng = np.random.default_rng(8)
     n = args.cpu_n
-    ppc = args.ppc
-    npart = n ** 3 * ppc
-    dt = 0.5 / O.C
-    x = rng.random((npart, 3)) * n
-    u = rng.standard_normal((npart, 3)) * 0.01
-    w = np.full(npart, 1e9)
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    x, u, w, dt = cpu_load(n, args.ppc, args.sc, 12345)
+    threads = cpu_threads()
     t0 = time.perf_counter()
     O.ref_run_cic_direct((n, n, n), (1.0, 1.0, 1.0), (args.sc,) * 3, O.E_CHARGE, O.E_MASS, dt, args.cpu_steps, x, u,
                          w, nthreads=threads)
     t1 = time.perf_counter()
-    return {"value": npart * args.cpu_steps / (t1 - t0), "unit": "particle-steps/s", "cores": threads,
-            "kind": "reference",
+    return {"value": x.shape[0] * args.cpu_steps / (t1 - t0), "unit": "particle-steps/s", "cores": threads,
+            "kind": "reference", "cpu_model": cpu_model(),
             "sample": "pica's own FieldInterpolatorCICSupercell + BorisPusher + CurrentDepositorCICSupercell + "
-                      "YeeSolver, %d^3 x %d ppc, %d steps incl. bucketing and re-sort glue, FP64 (pica has no "
-                      "TSC/Esirkepov)" % (n, ppc, args.cpu_steps)}
+                      "YeeSolver, %d^3 x %d ppc from the seeded loader's CPU twin, %d steps incl. bucketing and "
+                      "re-sort glue, FP64 (pica has no TSC/Esirkepov)" % (n, args.ppc, args.cpu_steps)}
 
 
 def cpu_baseline(args):
     """CPU oracle on a bounded sample of the same workload: sample^3 cells, same ppc, shape,
-    depositor and supercells, a few steps, all host cores that OMP_NUM_THREADS allows."""
+    depositor and supercells, the device loader's particles (its CPU twin), a few steps, all host
+    cores that OMP_NUM_THREADS allows."""
     import oracle as O
-    rng = np.random.default_rng(7)
     n = args.cpu_n
-    ncell = (n, n, n)
-    dt = 0.5 / O.C
-    wp = 0.1 / dt
-    n0 = wp * wp * O.E_MASS / (4.0 * O.PI * O.E_CHARGE * O.E_CHARGE)
-    ppc = args.ppc
-    npart = n ** 3 * ppc
-    cells = np.repeat(np.arange(n ** 3), ppc)
-    x = np.stack([cells // (n * n) + rng.random(npart), (cells // n) % n + rng.random(npart),
-                  cells % n + rng.random(npart)], 1).astype(np.float64)
-    u = rng.standard_normal((npart, 3)) * 0.01
-    u[:, 0] += np.where(np.arange(npart) % 2 == 0, 1.0, -1.0) * 0.1 / math.sqrt(0.99)
-    w = np.full(npart, n0 / ppc)
-    grid = O.Grid.periodic(ncell)
-    ens = O.Ensemble(ncell, (args.sc,) * 3, (1.0, 1.0, 1.0), x, u, w)
+    x, u, w, dt = cpu_load(n, args.ppc, args.sc, 12345)
+    npart = x.shape[0]
+    grid = O.Grid.periodic((n, n, n))
+    ens = O.Ensemble((n, n, n), (args.sc,) * 3, (1.0, 1.0, 1.0), x, u, w)
     shape = O.TSC if args.shape == "tsc" else O.CIC
     dep = O.ESIRKEPOV if args.dep == "esirkepov" else O.DIRECT
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    threads = cpu_threads()
     O.step(grid, ens, dt, shape, dep, threads)     # warm-up
     t0 = time.perf_counter()
     for _ in range(args.cpu_steps):
         O.step(grid, ens, dt, shape, dep, threads)
     t1 = time.perf_counter()
     return {"value": npart * args.cpu_steps / (t1 - t0), "unit": "particle-steps/s", "cores": threads,
-            "kind": "port",
-            "sample": "%d^3 cells x %d ppc (%d particles), %d timed steps of the whole PIC step, FP64, %s + %s, "
-                      "%d^3 supercells" % (n, ppc, npart, args.cpu_steps, args.shape.upper(), args.dep, args.sc)}
+            "kind": "port", "cpu_model": cpu_model(),
+            "sample": "%d^3 cells x %d ppc (%d particles from the seeded loader's CPU twin, the GPU run's loader), "
+                      "%d timed steps of the whole PIC step, FP64, %s + %s, %d^3 supercells"
+                      % (n, args.ppc, npart, args.cpu_steps, args.shape.upper(), args.dep, args.sc)}
 
 
 def main():
@@ -277,27 +347,17 @@ def main():
     ap.add_argument("--dep", default="esirkepov", choices=["direct", "esirkepov"])
     ap.add_argument("--prec", default="both", choices=["f64", "f32", "both"],
                     help="f32: the split-position FP32 form (the headline value); f64: the plain FP64 path")
-    ap.add_argument("--cpu-n", type=int, default=96)
+    ap.add_argument("--cpu-n", type=int, default=128)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-events", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-plasma batched workload")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-denoiser workload")
     ap.add_argument("--batch-n", type=int, default=64)
-    ap.add_argument("--batch-steps", type=int, default=10)
+    ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--kernel", default="auto", choices=["auto", "per_particle", "cell_sorted"],
                     help="FP32 Esirkepov particle kernel (auto: the per-particle kernel)")
-    ap.add_argument("--copy-gib", type=float, default=0.0,
-                    help="only copy this many GiB device-to-device once (counter calibration for profiles/)")
     args = ap.parse_args()
-    if args.copy_gib > 0:
-        import torch
-        a = torch.ones(int(args.copy_gib * 2 ** 28), device="cuda")
-        b = torch.empty_like(a)
-        b.copy_(a)
-        torch.cuda.synchronize()
-        print("copied %.3f GiB" % args.copy_gib, file=sys.stderr)
-        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (multi-GPU domain decomposition is design-only); "
               "run with --gpus 1", file=sys.stderr)
@@ -332,6 +392,8 @@ def main():
         "roofline": res.get("roofline"),
         "particle_kernel_rate": res.get("particle_kernel_rate"),
         "fdtd_cell_updates_per_s": res.get("fdtd_cell_updates_per_s"),
+        "fdtd_roofline": res.get("fdtd_roofline"),
+        "atomics": res.get("atomics"),
         "kernels_ms_mean": {k: v["ms_mean"] for k, v in res["kernels"].items()},
     }
     if plain is not None:
@@ -339,21 +401,23 @@ def main():
                               "particle_kernel_rate": plain.get("particle_kernel_rate"),
                               "roofline": plain.get("roofline"),
                               "fdtd_cell_updates_per_s": plain.get("fdtd_cell_updates_per_s"),
+                              "fdtd_roofline": plain.get("fdtd_roofline"),
+                              "atomics": plain.get("atomics"),
                               "kernels_ms_mean": {k: v["ms_mean"] for k, v in plain["kernels"].items()}}
         line["fp32_speedup_vs_fp64"] = res["particle_steps_per_s"] / plain["particle_steps_per_s"]
         # identical seeded inputs, same number of steps: total-energy difference of the FP32 form
         line["fp32_energy_rel_err_vs_fp64"] = abs(res["energy"] - plain["energy"]) / abs(plain["energy"])
         line["energy_drift_fp64"] = abs(plain["energy"] - plain["energy0"]) / abs(plain["energy0"])
     if not args.no_batched:
-        line["batched_workload"] = run_batched(args, batch_specs(args.batch_n))
+        line["batched_workload"] = run_batched(args, batch_specs(args.batch_n), args.batch_steps)
         # many small plasmas, where one launch per stage for all of them matters
-        line["batched_small_workload"] = run_batched(args, batch_specs(512, 5, (16, 24, 32), (4, 8)))
+        line["batched_small_workload"] = run_batched(args, batch_specs(512, 5, (16, 24, 32), (4, 8)), 20)
     if not args.no_learned:
         line["learned"] = run_learned()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args)
         line["gpu_vs_cpu"] = line["value"] / line["cpu_baseline"]["value"]
-        ref = cpu_reference(argparse.Namespace(cpu_n=64, ppc=args.ppc, sc=args.sc, cpu_steps=2))
+        ref = cpu_reference(argparse.Namespace(cpu_n=96, ppc=args.ppc, sc=args.sc, cpu_steps=2))
         if ref:
             line["cpu_baseline_reference"] = ref
     print(json.dumps(line), flush=True)

@@ -40,19 +40,22 @@ int build_items(pk_batch* b, int shape)
 {
     std::vector<int2> items;
     std::vector<int32_t> key;
+    const bool by_count = b->knob[PK_BATCH_ORDER] == 1;
     for (int i = 0; i < b->n(); i++) {
         const pk_ens* e = b->ens[i];
-        std::vector<int32_t> cnt((size_t)e->nsct);
-        PK_HIP(b->ctx, hipMemcpy(cnt.data(), e->cnt, sizeof(int32_t) * e->nsct, hipMemcpyDeviceToHost));
+        // the counts only order the items (largest first); ensemble order needs none
+        std::vector<int32_t> cnt(by_count ? (size_t)e->nsct : 0);
+        if (by_count)
+            PK_HIP(b->ctx, hipMemcpy(cnt.data(), e->cnt, sizeof(int32_t) * e->nsct, hipMemcpyDeviceToHost));
         const long chunk = (e->nsct + PK_XCD - 1) / PK_XCD;
         for (long w = 0; w < PK_XCD * chunk; w++) {
             const long s = (w % PK_XCD) * chunk + w / PK_XCD;
             if (s >= e->nsct) continue;
             items.push_back(make_int2(i, (int)s));
-            key.push_back(cnt[(size_t)s]);
+            if (by_count) key.push_back(cnt[(size_t)s]);
         }
     }
-    if (b->knob[PK_BATCH_ORDER] == 1) {
+    if (by_count) {
         std::vector<size_t> idx(items.size());
         for (size_t k = 0; k < idx.size(); k++) idx[k] = k;
         std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t c) { return key[a] > key[c]; });
