@@ -69,6 +69,8 @@ def profile_value(key):
         for line in open(os.path.join(ROOT, PROFILE)):
             if line.startswith(key + ":"):
                 v = line.split(":", 1)[1].strip()
+                if key == "commit":          # a hash such as 24408e4 is not a number
+                    return v
                 try:
                     return float(v)
                 except ValueError:
