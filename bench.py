@@ -21,8 +21,9 @@ from {32,48,64,96,128}, ppc from {4,8,16,32}, drift speeds uniform in [0,0.2]c a
 plasma (6.3e8 particles, FP32 split positions, Esirkepov), 100 steps, stepped one by one (pk_step
 per plasma) and as one batch (pk_batch_run after pk_batch_autotune over supercell size, interleave
 period and launch settings); and 512 small plasmas (edges {16,24,32}, ppc {4,8}), 20 steps.
-learned: the current denoiser and its charge filter trained on 32^3 pairs of 4- and 256-ppc runs
-(48 runs x 10 snapshots, 32,000 Adam steps), evaluated at 128^3 on three held-out seeds against a
+learned: the current denoiser and its charge filter trained on 128^3 two-stream run pairs (4 and
+256 ppc of the same initial distribution; --learned-runs pairs of runs, 32 snapshots each, 4 random
+32^3 crops per snapshot; 32,000 Adam steps), evaluated at 128^3 on three held-out seeds against a
 256-ppc reference: J error (total, longitudinal, transverse) and field-energy error of 4 ppc raw,
 denoised and 64 ppc, and the cost of a 64-ppc step over a 4-ppc denoised step.
 roofline extras: the FDTD kernels against the HBM roofline (240 B FP64 / 120 B FP32 per
@@ -251,12 +252,12 @@ def run_batched(args, specs, nsteps):
     return out
 
 
-def run_learned():
-    """train the current denoiser on 32^3 pairs (4 vs 256 ppc) and evaluate it at 128^3"""
+def run_learned(runs):
+    """train the current denoiser on crops of 128^3 run pairs (4 vs 256 ppc) and evaluate it at 128^3"""
     import pica_amd as P
     from pica_amd import learned
     ctx = P.Context(0)
-    report, dn = learned.run(ctx)
+    report, dn = learned.run(ctx, train_runs=runs)
     dn.close()
     ctx.close()
     ev = report["eval"]
@@ -355,6 +356,8 @@ def main():
     ap.add_argument("--no-events", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-plasma batched workload")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-denoiser workload")
+    ap.add_argument("--learned-runs", type=int, default=12,
+                    help="128^3 run pairs the denoiser trains on (the learned form's full set is 200)")
     ap.add_argument("--batch-n", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--kernel", default="auto", choices=["auto", "per_particle", "cell_sorted"],
@@ -415,7 +418,7 @@ def main():
         # many small plasmas, where one launch per stage for all of them matters
         line["batched_small_workload"] = run_batched(args, batch_specs(512, 5, (16, 24, 32), (4, 8)), 20)
     if not args.no_learned:
-        line["learned"] = run_learned()
+        line["learned"] = run_learned(args.learned_runs)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args)
         line["gpu_vs_cpu"] = line["value"] / line["cpu_baseline"]["value"]

@@ -236,6 +236,12 @@ int pk_dn_grad_batch_host(pk_dn* d, const int n[3], int nsamples, const float* x
                           double* loss);
 /* training pairs: the physical J of two grids of equal size (low-ppc input, high-ppc target) */
 int pk_dn_add_pair(pk_dn* d, pk_grid* low, pk_grid* high);
+/* ncrops training pairs cut from two periodic grids of equal size: each an edge^3 crop of both
+   grids' physical J at one offset (drawn from seed and the crop's index, wrapping periodically;
+   written to offsets[3 i .. 3 i + 2] when offsets is not NULL). The crop is not periodic, so the
+   training loss leaves out its voxels within 3 (the network's receptive radius) of a face.
+   7 <= edge <= every grid edge. Lets a mini-batch draw small volumes from large runs. */
+int pk_dn_add_pair_crops(pk_dn* d, pk_grid* low, pk_grid* high, int ncrops, int edge, uint64_t seed, int* offsets);
 int pk_dn_num_pairs(pk_dn* d);
 int pk_dn_clear_pairs(pk_dn* d);
 /* iters Adam steps on mini-batches of `batch` pairs drawn with `seed`; per-step mean loss to losses */

@@ -107,19 +107,30 @@ def net_forward(p, x, emulate=True):
     return out, {"W": W, "xb": xb, "a1": a1, "a2": a2, "z1": z1, "z2": z2, "r": r}
 
 
-def net_loss(p, x, y, emulate=True):
+def interior(shape, margin):
+    """mask [n0][n1][n2] of the voxels at least `margin` from every face (all of them for margin 0)"""
+    m = np.zeros(shape, bool)
+    m[margin:shape[0] - margin, margin:shape[1] - margin, margin:shape[2] - margin] = True
+    return m
+
+
+def net_loss(p, x, y, emulate=True, margin=0):
+    """mean squared error over the channels of the interior voxels (see `interior`)"""
     out, _ = net_forward(p, x, emulate)
-    return float(np.mean((out - y) ** 2))
+    m = interior(out.shape[1:], margin)
+    return float(np.sum(((out - y) ** 2)[:, m]) / (3 * m.sum()))
 
 
-def net_grad(p, x, y, emulate=True):
-    """(loss, gradient vector) of the mean squared error; with emulate=True the back-propagated
-    gradients are rounded to BF16 where the device rounds them"""
+def net_grad(p, x, y, emulate=True, margin=0):
+    """(loss, gradient vector) of the mean squared error over the interior voxels; with emulate=True
+    the back-propagated gradients are rounded to BF16 where the device rounds them"""
     out, c = net_forward(p, x, emulate)
     r, W = c["r"], c["W"]
-    diff = out - y
-    loss = float(np.mean(diff ** 2))
-    g3 = r(2.0 * diff / diff.size)
+    m = interior(out.shape[1:], margin)
+    diff = (out - y) * m
+    count = 3 * m.sum()
+    loss = float(np.sum(diff ** 2) / count)
+    g3 = r(2.0 * diff / count)
     G = {"W3": weight_grad(c["a2"], g3), "b3": g3.reshape(3, -1).sum(1)}
     dz2 = r(conv3d_transpose(g3, W["W3"]) * (c["a2"] > 0))
     G["W2"] = weight_grad(c["a1"], dz2)

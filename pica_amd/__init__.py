@@ -58,7 +58,7 @@ ABI_SYMBOLS = (
     "pk_dn_layer_host", "pk_dn_grad_host", "pk_dn_grad_batch_host", "pk_dn_add_pair", "pk_dn_num_pairs",
     "pk_dn_clear_pairs", "pk_dn_train", "pk_dn_apply", "pk_rho_save", "pk_project_continuity", "pk_step_denoised", "pk_grid_save", "pk_grid_load",
     "pk_ens_save", "pk_ens_load", "pk_step_species", "pk_ens_interleave", "pk_diag_energies_static",
-    "pk_ens_set_supercells", "pk_batch_set_shape", "pk_dn_set_charge_gain",
+    "pk_ens_set_supercells", "pk_batch_set_shape", "pk_dn_set_charge_gain", "pk_dn_add_pair_crops",
 )
 
 _LIB = None
@@ -140,6 +140,7 @@ def lib():
         L.pk_dn_grad_host.argtypes = [vp, vp, vp, vp, vp, ctypes.POINTER(ctypes.c_double)]
         L.pk_dn_grad_batch_host.argtypes = [vp, vp, ctypes.c_int, vp, vp, vp, ctypes.POINTER(ctypes.c_double)]
         L.pk_dn_add_pair.argtypes = [vp, vp, vp]
+        L.pk_dn_add_pair_crops.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, vp]
         L.pk_dn_num_pairs.argtypes = [vp]
         L.pk_dn_clear_pairs.argtypes = [vp]
         L.pk_dn_train.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_uint64, vp]
@@ -544,6 +545,14 @@ class Denoiser:
 
     def add_pair(self, low, high):
         self.ctx.check(lib().pk_dn_add_pair(self.h, low.h, high.h))
+
+    def add_pair_crops(self, low, high, ncrops, edge, seed):
+        """ncrops edge^3 crops of the two periodic grids' J as training pairs (the loss leaves out
+        voxels within the network's receptive radius of a crop face); returns the offsets [ncrops][3]"""
+        off = np.zeros((max(int(ncrops), 1), 3), np.int32)
+        self.ctx.check(lib().pk_dn_add_pair_crops(self.h, low.h, high.h, int(ncrops), int(edge),
+                                                  int(seed) & 0xFFFFFFFFFFFFFFFF, _p(off)))
+        return off[:int(ncrops)]
 
     def num_pairs(self):
         return int(lib().pk_dn_num_pairs(self.h))

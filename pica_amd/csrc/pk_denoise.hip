@@ -30,6 +30,7 @@ typedef __bf16 bf16v8 __attribute__((ext_vector_type(8)));   // operand type of 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int HID = 16;
+constexpr int PK_DN_RADIUS = 3;   // receptive radius of the three 3x3x3 layers
 constexpr int NPARAM = 27 * 3 * HID + HID + 27 * HID * HID + HID + 27 * HID * 3 + 3;   // 9539
 constexpr int WF16 = 14 * 64 * 8, WF4 = 4 * 64 * 8;   // packed B fragments (BF16 values) per layer
 constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + 27 * 3 * HID, OFF_W2 = OFF_B1 + HID, OFF_B2 = OFF_W2 + 27 * HID * HID,
@@ -39,6 +40,8 @@ struct pk_dn_pair {
     float* low = nullptr;    // [3][nv] normalised input current
     float* high = nullptr;   // [3][nv] normalised target current
     int n[3] = {0, 0, 0};
+    int margin = 0;          // voxels this close to a face are left out of the loss (crops: the
+                             // network's receptive radius, so no output sees the wrapped edge)
 };
 
 struct pk_dn {
@@ -455,6 +458,23 @@ __global__ void k_grid_to_f32(GridView<T> g, Vol V, double scale, float* __restr
     for (int c = 0; c < 3; c++) dst[(long)c * V.nv + v] = (float)(scale * (double)g.f[PK_JX + c][gi]);
 }
 
+// an e0 x e1 x e2 crop of a grid's physical J at offset o (periodic wrap) -> channel-first FP32
+// scaled; the same rounding as k_grid_to_f32
+template <class T>
+__global__ void k_grid_crop_to_f32(GridView<T> g, int n0, int n1, int n2, Vol V, int o0, int o1, int o2, double scale,
+                                   float* __restrict__ dst)
+{
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V.nv) return;
+    int x, y, z;
+    voxel_xyz(V, v, x, y, z);
+    x += o0; y += o1; z += o2;
+    x -= x >= n0 ? n0 : 0; y -= y >= n1 ? n1 : 0; z -= z >= n2 ? n2 : 0;
+    const long gi = ((long)(x + g.G) * g.s1 + (y + g.G)) * g.s2 + (z + g.G);
+#pragma unroll
+    for (int c = 0; c < 3; c++) dst[(long)c * V.nv + v] = (float)(scale * (double)g.f[PK_JX + c][gi]);
+}
+
 // physical J of a grid -> the network input xb[v][0..3] (BF16, scaled; the same roundings as
 // k_grid_to_f32 then k_input_f32)
 template <class T>
@@ -482,18 +502,27 @@ __global__ void k_f32_to_grid(GridView<T> g, Vol V, double inv_scale, const floa
     for (int c = 0; c < 3; c++) g.f[PK_JX + c][gi] = (T)(inv_scale * (double)src[(long)c * V.nv + v]);
 }
 
-// MSE loss and its gradient: g3[v][c] = bf16(2 (out - y) / count), loss += sum (out - y)^2 / count
-__global__ void __launch_bounds__(256) k_loss(long nv, int ns, const float* __restrict__ out,
+// MSE loss and its gradient: g3[v][c] = bf16(2 (out - y) / count), loss += sum (out - y)^2 / count,
+// over the voxels at least `margin` from every face of their sample (the others get a zero gradient)
+__global__ void __launch_bounds__(256) k_loss(Vol V, int margin, const float* __restrict__ out,
                                               const float* __restrict__ y, double inv_count, uint16_t* __restrict__ g3,
                                               double* __restrict__ loss)
 {
     double s = 0.0;
+    const long nv = V.nv;
     // v: stacked voxel; out, y are [ns][3][nv]
-    for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < nv * ns; v += (long)gridDim.x * blockDim.x) {
+    for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < nv * V.ns; v += (long)gridDim.x * blockDim.x) {
         const long sample = v / nv, base = 2 * sample * nv + v;   // (3 sample + c) nv + (v - sample nv)
+        bool in = true;
+        if (margin > 0) {
+            int x, yy, z;
+            voxel_xyz(V, v - sample * nv, x, yy, z);
+            in = x >= margin && x < V.n0 - margin && yy >= margin && yy < V.n1 - margin && z >= margin &&
+                 z < V.n2 - margin;
+        }
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            const float d = out[base + (long)c * nv] - y[base + (long)c * nv];
+            const float d = in ? out[base + (long)c * nv] - y[base + (long)c * nv] : 0.0f;
             g3[v * 4 + c] = to_bf16((float)(2.0 * inv_count) * d);
             s += (double)d * d;
         }
@@ -624,13 +653,13 @@ int wgrad(pk_dn* d, const Vol& V, const uint16_t* in, int cin_pad, int cin, cons
 }
 
 // loss of d->out against d->tgt (added to *loss) and gradients accumulated into d->g
-int backward(pk_dn* d, const Vol& V, double inv_count, double* loss)
+int backward(pk_dn* d, const Vol& V, int margin, double inv_count, double* loss)
 {
     hipStream_t s = d->ctx->stream;
     int r;
     const long nt = V.nv * V.ns;
     const unsigned nl = nblk(nt, 256) < 128 ? nblk(nt, 256) : 128;   // few workgroups: one loss atomic each
-    hipLaunchKernelGGL(k_loss, dim3(nl), dim3(256), 0, s, V.nv, V.ns, d->out, d->tgt, inv_count, d->g3, loss);
+    hipLaunchKernelGGL(k_loss, dim3(nl), dim3(256), 0, s, V, margin, d->out, d->tgt, inv_count, d->g3, loss);
     if ((r = wgrad(d, V, d->a2, 16, HID, d->g3, 4, 3, d->g + OFF_W3, d->g + OFF_B3))) return r;
     const unsigned nb = conv_blocks(V);
     hipLaunchKernelGGL((k_conv<4, OUT_MASKED_BF16>), dim3(nb), dim3(256), conv_lds(4), s, V, d->g3, d->wt[1], (const float*)nullptr,
@@ -875,7 +904,7 @@ int pk_dn_grad_batch_host(pk_dn* d, const int n[3], int nsamples, const float* x
                            1.0, d->xb + 4 * V.nv * k);
     if ((r = forward(d, V))) return r;
     PK_HIP(c, hipMemcpyAsync(d->tgt, y, sizeof(float) * 3 * V.nv * V.ns, hipMemcpyHostToDevice, c->stream));
-    if ((r = backward(d, V, 1.0 / (3.0 * V.nv * V.ns), d->loss))) return r;
+    if ((r = backward(d, V, 0, 1.0 / (3.0 * V.nv * V.ns), d->loss))) return r;
     PK_HIP(c, hipMemcpyAsync(grad, d->g, sizeof(float) * NPARAM, hipMemcpyDeviceToHost, c->stream));
     double l = 0.0;
     PK_HIP(c, hipMemcpyAsync(&l, d->loss, sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -919,6 +948,52 @@ int pk_dn_add_pair(pk_dn* d, pk_grid* low, pk_grid* high)
     }
     PK_HIP(d->ctx, hipGetLastError());
     d->pairs.push_back(pr);
+    return PK_OK;
+}
+
+int pk_dn_add_pair_crops(pk_dn* d, pk_grid* low, pk_grid* high, int ncrops, int edge, uint64_t seed, int* offsets)
+{
+    if (!d || !low || !high || ncrops < 0) return PK_E_ARG;
+    if (low->ctx != d->ctx || high->ctx != d->ctx)
+        return pk_fail(d->ctx, PK_E_ARG, "pk_dn_add_pair_crops: other context");
+    int n[3];
+    for (int k = 0; k < 3; k++) {
+        n[k] = low->s[k] - 2 * low->ghost;
+        if (high->s[k] - 2 * high->ghost != n[k] || low->prec != high->prec)
+            return pk_fail(d->ctx, PK_E_ARG, "pk_dn_add_pair_crops: grids differ in size or precision");
+        if (edge < 2 * PK_DN_RADIUS + 1 || edge > n[k])
+            return pk_fail(d->ctx, PK_E_ARG, "pk_dn_add_pair_crops: crop edge outside [7, grid edge]");
+    }
+    Vol V{edge, edge, edge, (long)edge * edge * edge};
+    hipStream_t s = d->ctx->stream;
+    for (int i = 0; i < ncrops; i++) {
+        pk_dn_pair pr;
+        pr.n[0] = pr.n[1] = pr.n[2] = edge;
+        pr.margin = PK_DN_RADIUS;
+        int o[3];
+        for (int k = 0; k < 3; k++)
+            o[k] = (int)(mix64(seed ^ mix64(((uint64_t)i << 2) + (uint64_t)k)) % (uint64_t)n[k]);
+        if (offsets)
+            for (int k = 0; k < 3; k++) offsets[3 * i + k] = o[k];
+        int r;
+        if ((r = dalloc(d->ctx, (void**)&pr.low, sizeof(float) * 3 * V.nv))) return r;
+        if ((r = dalloc(d->ctx, (void**)&pr.high, sizeof(float) * 3 * V.nv))) {
+            (void)hipFree(pr.low);
+            return r;
+        }
+        for (int k = 0; k < 2; k++) {
+            pk_grid* g = k ? high : low;
+            float* dst = k ? pr.high : pr.low;
+            if (g->prec == PK_F64)
+                hipLaunchKernelGGL(k_grid_crop_to_f32<double>, dim3(nblk(V.nv, 256)), dim3(256), 0, s,
+                                   grid_view<double>(g), n[0], n[1], n[2], V, o[0], o[1], o[2], d->scale, dst);
+            else
+                hipLaunchKernelGGL(k_grid_crop_to_f32<float>, dim3(nblk(V.nv, 256)), dim3(256), 0, s,
+                                   grid_view<float>(g), n[0], n[1], n[2], V, o[0], o[1], o[2], d->scale, dst);
+        }
+        PK_HIP(d->ctx, hipGetLastError());
+        d->pairs.push_back(pr);
+    }
     return PK_OK;
 }
 
@@ -975,8 +1050,10 @@ int pk_dn_train(pk_dn* d, int iters, int batch, double lr, uint64_t seed, float*
     for (int it = 0; it < iters && !r; it++) {
         const int* sel = hsel.data() + (size_t)it * batch;
         bool same = true;
-        for (int b = 1; b < batch; b++)
+        for (int b = 1; b < batch; b++) {
+            same = same && d->pairs[sel[b]].margin == d->pairs[sel[0]].margin;
             for (int k = 0; k < 3; k++) same = same && d->pairs[sel[b]].n[k] == d->pairs[sel[0]].n[k];
+        }
         // samples of one size go through every layer together (stacked); otherwise one by one
         const int groups = same ? 1 : batch, per = same ? batch : 1;
         for (int gi = 0; gi < groups && !r; gi++) {
@@ -985,7 +1062,9 @@ int pk_dn_train(pk_dn* d, int iters, int batch, double lr, uint64_t seed, float*
             hipLaunchKernelGGL(k_gather, dim3(nblk(V.nv, 256), per), dim3(256), 0, c->stream, V.nv, dptr, dptr + npairs,
                                dsel + (size_t)it * batch + gi * per, d->xb, d->tgt);
             if ((r = forward(d, V))) break;
-            r = backward(d, V, 1.0 / (3.0 * V.nv * batch), dev_losses + it);
+            const int m = d->pairs[sel[gi * per]].margin;
+            const double counted = (double)(V.n0 - 2 * m) * (V.n1 - 2 * m) * (V.n2 - 2 * m);
+            r = backward(d, V, m, 1.0 / (3.0 * counted * batch), dev_losses + it);
         }
         if (r) break;
         d->adam_t += 1;

@@ -1,11 +1,15 @@
 """Learned form driver: training pairs from plain runs, training, and evaluation of the current
 denoiser (pk_dn_* in the native library; this module only orchestrates calls into it).
 
-Workload: periodic electron plasmas loaded with a few random long-wavelength velocity modes
-(pk_ens_load_modes) on a thermal background, TSC shape, Esirkepov deposition, FP32 split
-positions, dt at Courant 0.5, omega_p dt = 0.1.  A training pair is the deposited J of a
-low-ppc run and of a high-ppc run of the same modes at the same step; the network maps the first
-to the second.  The charge filter: the projection of the denoised step fixes the longitudinal part of J by
+Workload: 128^3 periodic two-stream electron plasmas (Maxwellian u with sigma 0.01, drift +-0.1c
+in x on alternate particles, as the plain 256^3 workload) carrying a few seeded long-wavelength
+velocity modes (pk_ens_load_modes), TSC shape, Esirkepov deposition, FP32 split positions, dt at
+Courant 0.5, omega_p dt = 0.1.  A run pair is a 4-ppc and a 256-ppc plasma of the same initial
+distribution (the same modes, each its own particles); at each of 32 snapshots (every step) a few
+random 32^3 crops of both currents become training pairs, and the network maps the first to the
+second.  The modes give the pair a common signal: without them the two runs' currents are
+independent noise.  A crop is not periodic, so the loss leaves out its voxels within the
+network's receptive radius (3) of a face.  The charge filter: the projection of the denoised step fixes the longitudinal part of J by
 continuity, so without a filter that part keeps the low-ppc charge noise whatever the network
 does.  A radial filter S with gain g(|k|) is learned from the training pairs as the Wiener gain of
 the longitudinal current per |k| shell, g = sum Re(DJ_high conj DJ_low) / sum |DJ_low|^2 (DJ the
@@ -27,7 +31,8 @@ import numpy as np
 from . import (C, DIAG_FIELD_ENERGY, ELECTRON_CHARGE, ESIRKEPOV, F32, JX, JY, JZ, TSC, Denoiser,
                EnsembleSupercells, YeeGrid, diag, plasma_parameters, step, step_denoised)
 
-UTH = 0.02        # thermal spread of u = p/(mc)
+UTH = 0.01        # thermal spread of u = p/(mc)
+U_DRIFT = 0.1 / math.sqrt(1.0 - 0.1 * 0.1)    # u of the +-0.1c streams
 U_AMP = 0.05      # velocity mode amplitude scale (units of c)
 DT = 0.5 / C      # Courant 0.5 at h = 1 cm
 
@@ -57,7 +62,7 @@ def plasma(ctx, ncell, ppc, modes, seed, ncps=(4, 4, 4)):
     _, w = plasma_parameters(DT, ppc)
     g = YeeGrid.periodic(ctx, ncell, prec=F32)
     e = EnsembleSupercells(ctx, ncell, (1.0, 1.0, 1.0), ncps, prec=F32)
-    e.load_modes(ppc, UTH, 0.0, w, seed, modes)
+    e.load_modes(ppc, UTH, U_DRIFT, w, seed, modes)
     return e, g
 
 
@@ -132,23 +137,33 @@ class ChargeGainFit:
         return g, self.kmax
 
 
-def generate_pairs(ctx, dn, runs, ncell=(32, 32, 32), ppc_low=4, ppc_high=256, steps=40, every=4, seed=0,
-                   cfit=None):
-    """runs x (steps / every) training pairs added to dn (and to the charge-filter fit cfit)"""
+def generate_pairs(ctx, dn, runs, ncell=(128, 128, 128), ppc_low=4, ppc_high=256, snapshots=32, every=1, crops=4,
+                   edge=32, seed=0, cfit=None, cfit_runs=4, cfit_every=8, log=None):
+    """runs x snapshots x crops training pairs added to dn: `runs` seeded run pairs, a snapshot every
+    `every` steps, `crops` random edge^3 crops of both currents per snapshot (the whole periodic grid
+    when edge covers it). The charge-filter fit cfit takes the whole currents of every cfit_every-th
+    snapshot of the first cfit_runs runs."""
     rng = np.random.default_rng(seed)
+    whole = all(edge >= n for n in ncell)
     for r in range(runs):
         modes = random_modes(rng, ncell)
         lo = plasma(ctx, ncell, ppc_low, modes, 100000 * seed + 2 * r)
         hi = plasma(ctx, ncell, ppc_high, modes, 100000 * seed + 2 * r + 1)
-        for s in range(1, steps + 1):
+        for s in range(1, snapshots * every + 1):
             step(*lo, DT, TSC, ESIRKEPOV)
             step(*hi, DT, TSC, ESIRKEPOV)
-            if s % every == 0:
+            if s % every:
+                continue
+            if whole:
                 dn.add_pair(lo[1], hi[1])
-                if cfit is not None:
-                    cfit.add(current(lo[1]), current(hi[1]))
+            else:
+                dn.add_pair_crops(lo[1], hi[1], crops, edge, (seed << 32) + (r << 12) + s)
+            if cfit is not None and r < cfit_runs and (s // every) % cfit_every == 0:
+                cfit.add(current(lo[1]), current(hi[1]))
         for obj in lo + hi:
             obj.close()
+        if log:
+            log("training run %d/%d done, %d pairs" % (r + 1, runs, dn.num_pairs()))
 
 
 def train(dn, schedule=((16000, 2e-3), (12000, 5e-4), (4000, 1e-4)), batch=8, seed=1):
@@ -235,22 +250,25 @@ def costs(ctx, dn, ncell=(128, 128, 128), ppc_low=4, ppc_mid=64, low_ncps=(4, 4,
     return cost
 
 
-def run(ctx, train_runs=48, train_grid=(32, 32, 32), eval_grid=(128, 128, 128), schedule=None, seed=0,
-        ppc_target=256, eval_seeds=(99, 199, 299)):
+def run(ctx, train_runs=12, train_grid=(128, 128, 128), eval_grid=(128, 128, 128), schedule=None, seed=0,
+        ppc_target=256, eval_seeds=(99, 199, 299), snapshots=32, crops=4, edge=32, log=None):
     """the whole learned-form pipeline; returns (report dict, trained Denoiser)"""
     dn = Denoiser(ctx)
     dn.set_scale(network_scale())
     dn.init(seed + 17)
     cfit = ChargeGainFit()
     t0 = time.perf_counter()
-    generate_pairs(ctx, dn, train_runs, train_grid, ppc_high=ppc_target, seed=seed, cfit=cfit)
+    generate_pairs(ctx, dn, train_runs, train_grid, ppc_high=ppc_target, snapshots=snapshots, crops=crops, edge=edge,
+                   seed=seed, cfit=cfit, log=log)
     t1 = time.perf_counter()
     losses = train(dn, **({"schedule": schedule} if schedule else {}))
     gain, kmax = cfit.fit()
     dn.set_charge_gain(gain, kmax)
     ctx.sync()
     t2 = time.perf_counter()
-    report = {"train": {"grid": list(train_grid), "ppc": {"input": 4, "target": ppc_target}, "pairs": dn.num_pairs(),
+    report = {"train": {"grid": list(train_grid), "ppc": {"input": 4, "target": ppc_target}, "runs": train_runs,
+                        "snapshots_per_run": snapshots, "crops_per_snapshot": crops, "crop_edge": edge,
+                        "drift_c": 0.1, "uth": UTH, "pairs": dn.num_pairs(),
                         "iters": int(losses.size),
                         "loss_first": float(np.mean(losses[:20])), "loss_last": float(np.mean(losses[-20:])),
                         "charge_gain": {"kmax": kmax, "gain": [round(float(v), 4) for v in gain]},
@@ -265,3 +283,26 @@ def run(ctx, train_runs=48, train_grid=(32, 32, 32), eval_grid=(128, 128, 128), 
                       "mean_rel_l2_vs_reference": {k: float(np.mean([r[k] for r in rows])) for k in keys},
                       "cost": costs(ctx, dn, eval_grid)}
     return report, dn
+
+
+def main(argv=None):
+    """python -m pica_amd.learned [--runs N]: the whole pipeline on device 0, one JSON report line"""
+    import argparse
+    import json
+
+    from . import Context
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--runs", type=int, default=200, help="128^3 training run pairs (4 and 256 ppc)")
+    ap.add_argument("--snapshots", type=int, default=32)
+    ap.add_argument("--crops", type=int, default=4)
+    args = ap.parse_args(argv)
+    ctx = Context(0)
+    report, dn = run(ctx, train_runs=args.runs, snapshots=args.snapshots, crops=args.crops,
+                     log=lambda m: print(m, flush=True))
+    dn.close()
+    ctx.close()
+    print(json.dumps(report), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -161,6 +161,50 @@ def test_train_first_step_is_adam_on_the_batch_gradient(ctx, dn):
         g.close()
 
 
+def test_crop_pairs_train_on_the_interior(ctx, dn):
+    """pk_dn_add_pair_crops: each pair is an edge^3 crop of both grids at its offset, wrapping
+    periodically, and training on it is Adam on the oracle's loss over the crop's interior (voxels
+    at least 3 from a face). The offsets depend only on the seed and the crop's index."""
+    _say("denoiser crop pairs")
+    rng = np.random.default_rng(12)
+    n, edge = (12, 10, 14), 8
+    x = rng.standard_normal((3,) + n).astype(np.float32)
+    y = (0.5 * x + 0.1 * rng.standard_normal(x.shape)).astype(np.float32)
+    low, high = _pair_grids(ctx, x, y)
+    p0 = _params(4)
+    dn.set_scale(1.0)
+    offs = dn.add_pair_crops(low, high, 3, edge, seed=21)
+    assert offs.shape == (3, 3) and np.all(offs >= 0) and np.all(offs < np.array(n))
+    assert dn.num_pairs() == 3
+    dn.clear_pairs()
+    assert np.array_equal(dn.add_pair_crops(low, high, 3, edge, seed=21), offs)
+    dn.clear_pairs()
+    lr = 1e-3
+    for sd in (21, 22, 23):
+        o = dn.add_pair_crops(low, high, 1, edge, seed=sd)[0]     # one pair: every mini-batch is copies of it
+        if sd == 21:
+            assert np.array_equal(o, offs[0])
+        idx = [np.arange(o[k], o[k] + edge) % n[k] for k in range(3)]
+        xc = x[:, idx[0]][:, :, idx[1]][:, :, :, idx[2]]
+        yc = y[:, idx[0]][:, :, idx[1]][:, :, :, idx[2]]
+        ref_loss, ref_g = D.net_grad(p0.astype(np.float64), xc, yc, emulate=True, margin=3)
+        dn.set_params(p0)
+        losses = dn.train(1, 2, lr, seed=5)
+        print("crop at %s: loss %.6g vs oracle %.6g" % (tuple(o), losses[0], ref_loss), flush=True)
+        assert abs(losses[0] - ref_loss) < 1e-5 * ref_loss
+        step = (p0.astype(np.float64) - dn.params()) / lr
+        big = np.abs(ref_g) > 1e-3 * np.abs(ref_g).max()
+        assert big.sum() > 0.9 * big.size
+        assert np.max(np.abs(step[big] - np.sign(ref_g[big]))) < 1e-3
+        dn.clear_pairs()
+    with pytest.raises(P.PicaError):
+        dn.add_pair_crops(low, high, 1, 6, seed=1)          # below 2 x 3 + 1
+    with pytest.raises(P.PicaError):
+        dn.add_pair_crops(low, high, 1, 11, seed=1)         # above the 10-cell edge
+    low.close()
+    high.close()
+
+
 @pytest.mark.parametrize("prec", [P.F64, P.F32])
 def test_apply_on_grid_equals_forward(ctx, dn, prec):
     """pk_dn_apply reads J from the grid and writes the network output back into it; it must give
@@ -385,3 +429,33 @@ def test_learned_form_at_128cube(ctx, dn, prec, tol):
     dn.set_charge_gain([], 1.0)
     e.close()
     g.close()
+
+
+def test_trained_network_beats_the_raw_current(ctx):
+    """what training gains: a network trained on crops of a few 32^3 two-stream run pairs (4 vs 256
+    ppc, the learned form's pipeline at a small size) brings the 4-ppc current of held-out seeds
+    much closer to the 256-ppc reference than the raw 4-ppc current, offline and inside the loop;
+    the untrained network does not"""
+    _say("trained network vs raw current, 32^3")
+    from pica_amd import learned as L
+    d = P.Denoiser(ctx)
+    d.set_scale(L.network_scale())
+    d.init(17)
+    untrained = L.evaluate(ctx, d, ncell=(32, 32, 32), steps=8, every=8, seed=77)[0]
+    L.generate_pairs(ctx, d, runs=3, ncell=(32, 32, 32), snapshots=8, every=2, crops=4, edge=16, seed=3)
+    assert d.num_pairs() == 3 * 8 * 4
+    losses = L.train(d, schedule=((3000, 2e-3),))
+    assert np.all(np.isfinite(losses)) and np.mean(losses[-50:]) < 0.5 * np.mean(losses[:50])
+    d.clear_pairs()
+    for sd in (77, 78):
+        rows = L.evaluate(ctx, d, ncell=(32, 32, 32), steps=16, every=8, seed=sd)
+        raw = np.mean([r["raw_low"] for r in rows])
+        den = np.mean([r["denoised_low"] for r in rows])
+        loop = np.mean([r["denoised_in_loop"] for r in rows])
+        mid = np.mean([r["mid"] for r in rows])
+        print("seed %d: J error vs 256 ppc: raw 4 ppc %.3f, denoised %.3f, in the loop %.3f, 64 ppc %.3f"
+              % (sd, raw, den, loop, mid), flush=True)
+        assert den < 0.6 * raw and loop < 0.8 * raw
+    print("untrained: raw %.3f, denoised %.3f" % (untrained["raw_low"], untrained["denoised_low"]), flush=True)
+    assert not untrained["denoised_low"] < 0.6 * untrained["raw_low"]
+    d.close()

@@ -58,21 +58,43 @@ def test_bf16_rounding_matches_torch():
     assert np.array_equal(D.bf16(v), ref)
 
 
-def test_net_gradient_matches_finite_differences():
+@pytest.mark.parametrize("margin,n", [(0, (4, 4, 4)), (1, (5, 4, 6))])
+def test_net_gradient_matches_finite_differences(margin, n):
+    """central differences of the loss, over all voxels and over the interior of a crop"""
     rng = np.random.default_rng(5)
     p = D.init_params(7).astype(np.float64)
     p += 0.05 * rng.standard_normal(p.shape)          # non-zero biases too
-    x = _rand(rng, 3, 4, 4, 4)
-    y = _rand(rng, 3, 4, 4, 4)
-    loss, g = D.net_grad(p, x, y, emulate=False)
+    x = _rand(rng, 3, *n)
+    y = _rand(rng, 3, *n)
+    loss, g = D.net_grad(p, x, y, emulate=False, margin=margin)
+    assert abs(loss - D.net_loss(p, x, y, False, margin)) < 1e-12 * loss
     idx = rng.choice(D.NPARAMS, 40, replace=False)
     for i in idx:
         h = 1e-6
         pp, pm = p.copy(), p.copy()
         pp[i] += h
         pm[i] -= h
-        fd = (D.net_loss(pp, x, y, False) - D.net_loss(pm, x, y, False)) / (2 * h)
+        fd = (D.net_loss(pp, x, y, False, margin) - D.net_loss(pm, x, y, False, margin)) / (2 * h)
         assert abs(fd - g[i]) < 1e-6 * max(1.0, abs(g).max()), (i, fd, g[i])
+
+
+def test_margin_loss_counts_only_the_interior():
+    """margin 0 is the plain mean over every voxel; with a margin the target outside the interior
+    does not enter the loss or the gradient, and the loss is the mean over the interior alone"""
+    rng = np.random.default_rng(8)
+    p = D.init_params(3).astype(np.float64)
+    x = _rand(rng, 3, 7, 8, 9)
+    y = _rand(rng, 3, 7, 8, 9)
+    out, _ = D.net_forward(p, x, False)
+    assert abs(D.net_loss(p, x, y, False) - np.mean((out - y) ** 2)) < 1e-12
+    l1, g1 = D.net_grad(p, x, y, False, margin=2)
+    y2 = y.copy()
+    y2[:, :2] += 5.0
+    y2[:, :, -2:] -= 3.0
+    l2, g2 = D.net_grad(p, x, y2, False, margin=2)
+    assert l1 == l2 and np.array_equal(g1, g2)
+    inner = (out - y)[:, 2:5, 2:6, 2:7]
+    assert abs(l1 - np.mean(inner ** 2)) < 1e-12 * l1
 
 
 def test_projection_restores_continuity():
