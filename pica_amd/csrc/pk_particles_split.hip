@@ -403,16 +403,46 @@ __device__ __forceinline__ void particles_split_sc(const SplitArgs& A, long s)
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const uint64_t below = (1ull << lane) - 1ull;
-    // field tile: a lane holds one (j, k) node of a slice, a wavefront walks the slices (comp, i)
+    // field tile: a lane holds one (j, k) node of a slice, a wavefront walks the slices (comp, i).
+    // The grid's components are equally spaced in one allocation, so a slice's address is
+    // E_x + comp * pitch + offset, with no per-slice load of a component pointer.
     const int uwave = __builtin_amdgcn_readfirstlane(wave);
-    for (int jk0 = 0; jk0 < its[1] * its[2]; jk0 += 64) {
-        const int jk = jk0 + lane, j = jk / its[2], k = jk - j * its[2];
-        if (jk < its[1] * its[2]) {
-            const long gofs = (long)(ibase[1] + j) * g.s2 + (ibase[2] + k);
-            float* const lt = itile + j * isy + k;
-            for (int sl = uwave; sl < 6 * its[0]; sl += WAVES) {
-                const int comp = sl / its[0], i = sl - comp * its[0];
-                lt[comp * itn + i * isx] = g.f[PK_EX + comp][(long)(ibase[0] + i) * g.s1 * g.s2 + gofs];
+    {
+        const float* __restrict__ f0 = g.f[PK_EX];
+        const long pitch = g.f[PK_EY] - g.f[PK_EX];
+        const long plane = (long)g.s1 * g.s2;
+        if constexpr (NC > 0) {
+            // every load of a lane is issued before its first LDS store: one memory latency per
+            // workgroup instead of one per slice
+            constexpr int IT = NC + (SHAPE == PK_TSC ? 3 : 2), NSL = 6 * IT, PER = (NSL + WAVES - 1) / WAVES;
+            for (int jk0 = 0; jk0 < IT * IT; jk0 += 64) {
+                const int jk = jk0 + lane, j = jk / IT, k = jk - j * IT;
+                const bool on = jk < IT * IT;
+                const long gofs = (long)(ibase[0]) * plane + (long)(ibase[1] + j) * g.s2 + (ibase[2] + k);
+                float v[PER];
+#pragma unroll
+                for (int q = 0; q < PER; q++) {
+                    const int sl = uwave + q * WAVES, comp = sl / IT, i = sl - comp * IT;
+                    v[q] = on && sl < NSL ? f0[comp * pitch + i * plane + gofs] : 0.0f;
+                }
+                float* const lt = itile + j * isy + k;
+#pragma unroll
+                for (int q = 0; q < PER; q++) {
+                    const int sl = uwave + q * WAVES, comp = sl / IT, i = sl - comp * IT;
+                    if (on && sl < NSL) lt[comp * itn + i * isx] = v[q];
+                }
+            }
+        } else {
+            for (int jk0 = 0; jk0 < its[1] * its[2]; jk0 += 64) {
+                const int jk = jk0 + lane, j = jk / its[2], k = jk - j * its[2];
+                if (jk < its[1] * its[2]) {
+                    const long gofs = (long)(ibase[1] + j) * g.s2 + (ibase[2] + k);
+                    float* const lt = itile + j * isy + k;
+                    for (int sl = uwave; sl < 6 * its[0]; sl += WAVES) {
+                        const int comp = sl / its[0], i = sl - comp * its[0];
+                        lt[comp * itn + i * isx] = f0[comp * pitch + (long)(ibase[0] + i) * plane + gofs];
+                    }
+                }
             }
         }
     }
@@ -731,16 +761,19 @@ __device__ __forceinline__ void particles_split_sc(const SplitArgs& A, long s)
     }
     __syncthreads();
     // J flush: a lane holds one (j, k) node of a slice, a wavefront walks the slices (comp, i)
-    for (int jk0 = 0; jk0 < dts[1] * dts[2]; jk0 += 64) {
-        const int jk = jk0 + lane, j = jk / dts[2], k = jk - j * dts[2];
-        if (jk < dts[1] * dts[2]) {
-            const long gofs = (long)(dbase[1] + j) * g.s2 + (dbase[2] + k);
-            const double* const lt = jtile + j * dsy + k;
-            for (int sl = uwave; sl < 3 * dts[0]; sl += WAVES) {
-                const int comp = sl / dts[0], i = sl - comp * dts[0];
-                const double v = lt[comp * dtn + i * dsx];
-                if (v != 0.0)
-                    atomicAdd(&g.f[PK_JX + comp][(long)(dbase[0] + i) * g.s1 * g.s2 + gofs], (float)(v * (double)sp.norm));
+    {
+        float* const j0 = g.f[PK_JX];
+        const long pitch = g.f[PK_JY] - g.f[PK_JX], plane = (long)g.s1 * g.s2;
+        for (int jk0 = 0; jk0 < dts[1] * dts[2]; jk0 += 64) {
+            const int jk = jk0 + lane, j = jk / dts[2], k = jk - j * dts[2];
+            if (jk < dts[1] * dts[2]) {
+                const long gofs = (long)dbase[0] * plane + (long)(dbase[1] + j) * g.s2 + (dbase[2] + k);
+                const double* const lt = jtile + j * dsy + k;
+                for (int sl = uwave; sl < 3 * dts[0]; sl += WAVES) {
+                    const int comp = sl / dts[0], i = sl - comp * dts[0];
+                    const double v = lt[comp * dtn + i * dsx];
+                    if (v != 0.0) atomicAdd(&j0[comp * pitch + i * plane + gofs], (float)(v * (double)sp.norm));
+                }
             }
         }
     }

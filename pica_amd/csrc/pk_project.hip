@@ -73,12 +73,15 @@ __device__ __forceinline__ double div_j(const GridView<T>& g, const Phys& P, lon
 
 // w[v] = (D J - d0[v], 0) = (D (J - Jref), 0): the residual when the current Jref, whose
 // divergence d0 was saved, satisfies continuity exactly, as the Esirkepov current does
+// pack_ref: the imaginary part carries d0 itself, so that one complex transform gives the
+// transforms of both real fields (the charge filter needs both)
 template <class T>
-__global__ void k_residual_ref(GridView<T> g, Phys P, const double* __restrict__ d0, Cx<T>* __restrict__ w)
+__global__ void k_residual_ref(GridView<T> g, Phys P, const double* __restrict__ d0, Cx<T>* __restrict__ w,
+                               int pack_ref)
 {
     const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= P.nv) return;
-    w[v] = Cx<T>{(T)(div_j(g, P, v) - d0[v]), (T)0};
+    w[v] = Cx<T>{(T)(div_j(g, P, v) - d0[v]), pack_ref ? (T)d0[v] : (T)0};
 }
 
 template <class T>
@@ -186,30 +189,26 @@ __global__ void k_correct(GridView<T> g, Phys P, const Cx<T>* __restrict__ w)
     g.f[PK_JZ][c] = (T)((double)g.f[PK_JZ][c] - (ph - pz) / g.step[2]);
 }
 
-// w2[v] = (d0[v], 0): the reference divergence as a complex volume
+// Z = transform of a + i b for real volumes a (the residual D J - d0) and b (d0), whose
+// transforms are A(k) = (Z(k) + conj Z(-k)) / 2 and B(k) = (Z(k) - conj Z(-k)) / 2i.  Writes
+// out = A + (1 - g(|k|)) B, the transform of D J - S d0 with S the radial filter of gain g (linear
+// interpolation of nb values over [0, kmax], the last value beyond); |k| in radians per cell,
+// k_d = 2 pi f_d / n_d with f_d the signed frequency
 template <class R>
-__global__ void k_pack(const double* __restrict__ d0, Cx<R>* __restrict__ w2, long nv)
-{
-    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v < nv) w2[v] = Cx<R>{(R)d0[v], (R)0};
-}
-
-// in Fourier space: w += (1 - g(|k|)) w2, so that the residual D J - d0 becomes D J - S d0 with S
-// the radial filter of gain g (linear interpolation of nb values over [0, kmax], the last value
-// beyond); |k| in radians per cell, k_d = 2 pi f_d / n_d with f_d the signed frequency
-template <class R>
-__global__ void k_gain_combine(Cx<R>* __restrict__ w, const Cx<R>* __restrict__ w2, Phys P,
+__global__ void k_gain_combine(const Cx<R>* __restrict__ z, Cx<R>* __restrict__ out, Phys P,
                                const double* __restrict__ gain, int nb, double kmax)
 {
     const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= P.nv) return;
     const int c[3] = {(int)(v / ((long)P.n[1] * P.n[2])), (int)((v / P.n[2]) % P.n[1]), (int)(v % P.n[2])};
     double k2 = 0.0;
+    int m[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         const int f = c[d] <= P.n[d] / 2 ? c[d] : c[d] - P.n[d];
         const double kd = 2.0 * M_PI * (double)f / (double)P.n[d];
         k2 += kd * kd;
+        m[d] = c[d] == 0 ? 0 : P.n[d] - c[d];
     }
     const double x = sqrt(k2) / kmax * (double)(nb - 1);
     double gk;
@@ -220,8 +219,11 @@ __global__ void k_gain_combine(Cx<R>* __restrict__ w, const Cx<R>* __restrict__ 
         const double t = x - (double)b;
         gk = gain[b] + t * (gain[b + 1] - gain[b]);
     }
+    const Cx<R> zp = z[v], zm = z[((long)m[0] * P.n[1] + m[1]) * P.n[2] + m[2]];
+    const R ax = (R)0.5 * (zp.x + zm.x), ay = (R)0.5 * (zp.y - zm.y);     // A(k)
+    const R bx = (R)0.5 * (zp.y + zm.y), by = (R)0.5 * (zm.x - zp.x);     // B(k)
     const R f = (R)(1.0 - gk);
-    w[v] = Cx<R>{w[v].x + f * w2[v].x, w[v].y + f * w2[v].y};
+    out[v] = Cx<R>{ax + f * bx, ay + f * by};
 }
 
 __global__ void k_copy_d(const double* __restrict__ a, double* __restrict__ b, long n)
@@ -330,14 +332,15 @@ int project(pk_grid* g, const Phys& P, const double* rho1, double dt, ChargeGain
                            g->rho_prev, 1.0 / dt, w);
     else
         hipLaunchKernelGGL(k_residual_ref<T>, dim3(nblk(P.nv, 256)), dim3(256), 0, s, grid_view<T>(g), P,
-                           jref_of(g, P), w);
+                           jref_of(g, P), w, cg.n > 0 ? 1 : 0);
     int r;
     if ((r = fft3(g, P, w, tw, 0, nullptr))) return r;
     if (!rho1 && cg.n > 0) {
+        // both real fields went through the one transform; their filtered combination goes to a
+        // second volume (the combination reads Z at k and -k)
         Cx<T>* w2 = (Cx<T>*)(jref_of(g, P) + P.nv);
-        hipLaunchKernelGGL(k_pack<T>, dim3(nblk(P.nv, 256)), dim3(256), 0, s, jref_of(g, P), w2, P.nv);
-        if ((r = fft3(g, P, w2, tw, 0, nullptr))) return r;
         hipLaunchKernelGGL(k_gain_combine<T>, dim3(nblk(P.nv, 256)), dim3(256), 0, s, w, w2, P, cg.g, cg.n, cg.kmax);
+        w = w2;
     }
     if ((r = fft3(g, P, w, tw, 1, lam))) return r;
     hipLaunchKernelGGL(k_correct<T>, dim3(nblk(P.nv, 256)), dim3(256), 0, s, grid_view<T>(g), P, w);
