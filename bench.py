@@ -228,7 +228,7 @@ def run_batched(args, specs, nsteps):
     tuned_ms = b.autotune(dt, P.SHAPE_PER_INSTANCE, P.ESIRKEPOV, trial_steps=2)
     knobs = {k: b.get(v) for k, v in (("wg_per_cu", P.BATCH_WG_PER_CU), ("order", P.BATCH_ORDER),
                                          ("graph", P.BATCH_GRAPH), ("supercell", P.BATCH_SUPERCELL),
-                                         ("interleave", P.BATCH_INTERLEAVE))}
+                                         ("interleave", P.BATCH_INTERLEAVE), ("block", P.BATCH_BLOCK))}
     b.run(dt, 1, P.SHAPE_PER_INSTANCE, P.ESIRKEPOV)
     ctx.sync()
     ctx.timer_reset()

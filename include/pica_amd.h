@@ -180,6 +180,8 @@ enum {
                                  when set; 0 = the supercells the ensembles had when the batch was created */
     PK_BATCH_INTERLEAVE = 4,  /* FP32 interleave period of the batched step: -1 = the context's
                                  PK_OPT_INTERLEAVE, 0 = off, k = every k-th re-sort */
+    PK_BATCH_BLOCK = 5,       /* FP32 particle kernel workgroup: 256 threads (four wavefronts take the
+                                 128-slot groups of a supercell in turn) or 128 (two wavefronts) */
     PK_BATCH_NKNOB
 };
 /* shape argument of pk_batch_run / pk_batch_autotune: every ensemble with its own form factor, set

@@ -387,8 +387,11 @@ int pk_batch_set(pk_batch* b, int knob, int value)
 {
     if (!b) return PK_E_ARG;
     if (knob < 0 || knob >= PK_BATCH_NKNOB) return pk_fail(b->ctx, PK_E_ARG, "pk_batch_set: unknown knob");
-    const int lo = knob == PK_BATCH_INTERLEAVE ? -1 : 0;
-    const int hi = knob == PK_BATCH_WG_PER_CU || knob == PK_BATCH_INTERLEAVE ? 64 : knob == PK_BATCH_SUPERCELL ? 12 : 1;
+    const int lo = knob == PK_BATCH_INTERLEAVE ? -1 : knob == PK_BATCH_BLOCK ? 128 : 0;
+    const int hi = knob == PK_BATCH_WG_PER_CU || knob == PK_BATCH_INTERLEAVE ? 64
+                   : knob == PK_BATCH_SUPERCELL ? 12 : knob == PK_BATCH_BLOCK ? 256 : 1;
+    if (knob == PK_BATCH_BLOCK && value != 128 && value != 256)
+        return pk_fail(b->ctx, PK_E_ARG, "pk_batch_set: block must be 128 or 256 threads");
     if (value < lo || value > hi) return pk_fail(b->ctx, PK_E_ARG, "pk_batch_set: value out of range");
     if (knob == PK_BATCH_SUPERCELL) {
         const int ncps[3] = {value ? value : b->ncps0[0], value ? value : b->ncps0[1], value ? value : b->ncps0[2]};
@@ -521,7 +524,14 @@ int pk_batch_autotune(pk_batch* b, double dt, int shape, int dep, int trial_step
             if ((r = consider(sc, kn))) break;
         }
     }
-    // (3) launch: graph, item order, persistence
+    // (3) block shape: two or four wavefronts per workgroup (FP32 split ensembles)
+    if (!r && b->prec == PK_F32) {
+        int kn[PK_BATCH_NKNOB];
+        for (int k = 0; k < PK_BATCH_NKNOB; k++) kn[k] = best[k];
+        kn[PK_BATCH_BLOCK] = best[PK_BATCH_BLOCK] == 128 ? PK_BLOCK : 128;
+        r = consider(best_sc, kn);
+    }
+    // (4) launch: graph, item order, persistence
     if (!r) {
         const int wgs[] = {0, 1, 2, 4, 8};
         const int sc = best_sc;

@@ -219,7 +219,7 @@ struct pk_batch {
     int prec = PK_F64;
     std::vector<pk_ens*> ens;
     std::vector<pk_grid*> grids;
-    int knob[PK_BATCH_NKNOB] = {0, 1, 1, 0, -1};
+    int knob[PK_BATCH_NKNOB] = {0, 1, 1, 0, -1, PK_BLOCK};
     int ncps0[3] = {0, 0, 0};  // supercell shape the ensembles had when the batch was created
     std::vector<int> shapes;   // per-ensemble form factor for PK_SHAPE_PER_INSTANCE (-1: unset)
     // prepared launch state, valid while !dirty and (dt, shape, dep) are unchanged

@@ -354,9 +354,10 @@ struct SplitArgs {
 
 // One supercell s of one ensemble.  NC > 0: cubic supercells of NC cells (compile-time tile
 // strides); NC == 0: any shape.
-template <int SHAPE, int DEP, int NC>
+template <int SHAPE, int DEP, int NC, int BS = PK_BLOCK>
 __device__ __forceinline__ void particles_split_sc(const SplitArgs& A, long s)
 {
+    constexpr int WV = BS / 64;      // wavefronts of the workgroup
     const GridView<float>& g = A.g;
     const EnsGeom& eg = A.eg;
     const PartSplit& pm = A.pm;
@@ -414,7 +415,7 @@ __device__ __forceinline__ void particles_split_sc(const SplitArgs& A, long s)
         if constexpr (NC > 0) {
             // every load of a lane is issued before its first LDS store: one memory latency per
             // workgroup instead of one per slice
-            constexpr int IT = NC + (SHAPE == PK_TSC ? 3 : 2), NSL = 6 * IT, PER = (NSL + WAVES - 1) / WAVES;
+            constexpr int IT = NC + (SHAPE == PK_TSC ? 3 : 2), NSL = 6 * IT, PER = (NSL + WV - 1) / WV;
             for (int jk0 = 0; jk0 < IT * IT; jk0 += 64) {
                 const int jk = jk0 + lane, j = jk / IT, k = jk - j * IT;
                 const bool on = jk < IT * IT;
@@ -422,13 +423,13 @@ __device__ __forceinline__ void particles_split_sc(const SplitArgs& A, long s)
                 float v[PER];
 #pragma unroll
                 for (int q = 0; q < PER; q++) {
-                    const int sl = uwave + q * WAVES, comp = sl / IT, i = sl - comp * IT;
+                    const int sl = uwave + q * WV, comp = sl / IT, i = sl - comp * IT;
                     v[q] = on && sl < NSL ? f0[comp * pitch + i * plane + gofs] : 0.0f;
                 }
                 float* const lt = itile + j * isy + k;
 #pragma unroll
                 for (int q = 0; q < PER; q++) {
-                    const int sl = uwave + q * WAVES, comp = sl / IT, i = sl - comp * IT;
+                    const int sl = uwave + q * WV, comp = sl / IT, i = sl - comp * IT;
                     if (on && sl < NSL) lt[comp * itn + i * isx] = v[q];
                 }
             }
@@ -438,7 +439,7 @@ __device__ __forceinline__ void particles_split_sc(const SplitArgs& A, long s)
                 if (jk < its[1] * its[2]) {
                     const long gofs = (long)(ibase[1] + j) * g.s2 + (ibase[2] + k);
                     float* const lt = itile + j * isy + k;
-                    for (int sl = uwave; sl < 6 * its[0]; sl += WAVES) {
+                    for (int sl = uwave; sl < 6 * its[0]; sl += WV) {
                         const int comp = sl / its[0], i = sl - comp * its[0];
                         lt[comp * itn + i * isx] = f0[comp * pitch + (long)(ibase[0] + i) * plane + gofs];
                     }
@@ -446,8 +447,8 @@ __device__ __forceinline__ void particles_split_sc(const SplitArgs& A, long s)
             }
         }
     }
-    for (int t = tid; t < 3 * dtn; t += PK_BLOCK) jtile[t] = 0.0;
-    for (int t = tid; t < 27; t += PK_BLOCK) dircnt[t] = 0;
+    for (int t = tid; t < 3 * dtn; t += BS) jtile[t] = 0.0;
+    for (int t = tid; t < 27; t += BS) dircnt[t] = 0;
     if (tid == 0) {
         bad[0] = 0;
         qlen[0] = 0;
@@ -457,7 +458,7 @@ __device__ __forceinline__ void particles_split_sc(const SplitArgs& A, long s)
     const long base = off[s];
     const int n = cnt[s];
     const int N1N2 = eg.ncell[1] * eg.ncell[2];
-    // Each wavefront owns the 128-slot groups wave, wave + WAVES, ...; a lane pushes the two
+    // Each wavefront owns the 128-slot groups wave, wave + WV, ...; a lane pushes the two
     // adjacent slots c0 + 2 lane and c0 + 2 lane + 1.  No workgroup barrier until every particle
     // is pushed, so the wavefronts drift apart and one's LDS atomics overlap another's
     // arithmetic.  Every pushed particle is written back in place; movers are copied to the
@@ -618,7 +619,7 @@ __device__ __forceinline__ void particles_split_sc(const SplitArgs& A, long s)
             if (direct) esirkepov_window<SHAPE>(jtile, dtn, dsx, dsy, P.l, P.sh, P.o, P.on, big, P.qw, sp.hdt);
         }
     };
-    for (int c0 = 128 * wave; c0 < n; c0 += 128 * WAVES) {
+    for (int c0 = 128 * wave; c0 < n; c0 += 128 * WV) {
         PState A, B;
         push(c0 + 2 * lane, A);
         push(c0 + 2 * lane + 1, B);
@@ -668,7 +669,7 @@ __device__ __forceinline__ void particles_split_sc(const SplitArgs& A, long s)
     if (DEP == PK_ESIRKEPOV) {
         // the change queue, by full wavefronts
         const int nq = qlen[0] < QCAP ? qlen[0] : QCAP;
-        for (int t = tid; t - lane < nq; t += PK_BLOCK) {
+        for (int t = tid; t - lane < nq; t += BS) {
             const bool has = t < nq;
             int dl[3] = {0, 0, 0}, dsh[3] = {0, 0, 0}, dcls = 0;
             float d_o[3] = {0.5f, 0.5f, 0.5f}, d_on[3] = {0.5f, 0.5f, 0.5f}, dqw = 0.0f;
@@ -721,7 +722,7 @@ __device__ __forceinline__ void particles_split_sc(const SplitArgs& A, long s)
             return (lo << 6) + pk_select_bit(mask[lo], (int)(r - wpre[lo]));
         };
         // one lane per mover (not per slot): the t-th mover goes to outbox slot t
-        for (int t = tid; t < nmv; t += PK_BLOCK) {
+        for (int t = tid; t < nmv; t += BS) {
             const int i = mover_slot(t);
             const long src = base + i, q = base + t;
             const int cl = pm.cell[src];
@@ -744,7 +745,7 @@ __device__ __forceinline__ void particles_split_sc(const SplitArgs& A, long s)
         __syncthreads();
         // tail stayers fill the holes the movers left
         const long below_final = movers_below(nfinal);
-        for (int t = tid; t < nmv; t += PK_BLOCK) {
+        for (int t = tid; t < nmv; t += BS) {
             const int j = nfinal + t;
             if ((mask[j >> 6] >> (j & 63)) & 1ull) continue;
             const long r = (long)t - (movers_below(j) - below_final);   // its rank among the tail stayers
@@ -769,7 +770,7 @@ __device__ __forceinline__ void particles_split_sc(const SplitArgs& A, long s)
             if (jk < dts[1] * dts[2]) {
                 const long gofs = (long)dbase[0] * plane + (long)(dbase[1] + j) * g.s2 + (dbase[2] + k);
                 const double* const lt = jtile + j * dsy + k;
-                for (int sl = uwave; sl < 3 * dts[0]; sl += WAVES) {
+                for (int sl = uwave; sl < 3 * dts[0]; sl += WV) {
                     const int comp = sl / dts[0], i = sl - comp * dts[0];
                     const double v = lt[comp * dtn + i * dsx];
                     if (v != 0.0) atomicAdd(&j0[comp * pitch + i * plane + gofs], (float)(v * (double)sp.norm));
@@ -788,17 +789,18 @@ __device__ __forceinline__ void particles_split_sc(const SplitArgs& A, long s)
 // At least 5 wavefronts per SIMD: the LDS tiles allow 5 workgroups per CU, and the TSC
 // Esirkepov body (two particles per lane) needs ~111 VGPRs, i.e. 4 wavefronts, unless the
 // compiler keeps it within 96 (a few spilled dwords; 14.5 -> 14.1 ms per step at 256^3 x 16 ppc).
-template <int SHAPE, int DEP, int NC>
-__global__ void __launch_bounds__(PK_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) k_particles_split(SplitArgs A)
+// BS threads per workgroup (256, or 128 for the batched form's block-shape knob)
+template <int SHAPE, int DEP, int NC, int BS>
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(5))) k_particles_split(SplitArgs A)
 {
     const long s = pk_xcd_supercell(blockIdx.x, A.eg.xcd_chunk);
     if (s >= A.eg.nsct) return;
-    particles_split_sc<SHAPE, DEP, NC>(A, s);
+    particles_split_sc<SHAPE, DEP, NC, BS>(A, s);
 }
 
 // many ensembles: a list of (ensemble, supercell) items; gridDim.x < nitems makes it persistent
-template <int SHAPE, int DEP, int NC>
-__global__ void __launch_bounds__(PK_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) k_particles_split_batch(const SplitArgs* __restrict__ jobs,
+template <int SHAPE, int DEP, int NC, int BS>
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(5))) k_particles_split_batch(const SplitArgs* __restrict__ jobs,
                                                                     const int2* __restrict__ items, long nitems,
                                                                     const int* __restrict__ halt)
 {
@@ -806,7 +808,7 @@ __global__ void __launch_bounds__(PK_BLOCK) __attribute__((amdgpu_waves_per_eu(5
     for (long w = blockIdx.x; w < nitems; w += gridDim.x) {
         __syncthreads();
         const int2 it = items[w];
-        particles_split_sc<SHAPE, DEP, NC>(jobs[it.x], it.y);     // arguments read in place (a copy would live in scratch)
+        particles_split_sc<SHAPE, DEP, NC, BS>(jobs[it.x], it.y);     // arguments read in place (a copy would live in scratch)
     }
 }
 
@@ -1306,7 +1308,8 @@ int launch(pk_ens* e, pk_grid* g, double dt)
     const unsigned nblocks = (unsigned)(PK_XCD * (long)eg.xcd_chunk);
     pk_ktimer tm(e->ctx, "particles");
     const SplitArgs A = split_args(e, g, eg, sp);
-    hipLaunchKernelGGL((k_particles_split<SHAPE, DEP, NC>), dim3(nblocks), dim3(PK_BLOCK), lds, e->ctx->stream, A);
+    hipLaunchKernelGGL((k_particles_split<SHAPE, DEP, NC, PK_BLOCK>), dim3(nblocks), dim3(PK_BLOCK), lds,
+                       e->ctx->stream, A);
     PK_HIP(e->ctx, hipGetLastError());
     return PK_OK;
 }
@@ -1386,7 +1389,9 @@ int launch_nc(pk_ens* e, pk_grid* g, double dt)
 template <int SHAPE, int DEP, int NC>
 int batch_go(pk_batch* b, bool prepare)
 {
-    const void* fn = (const void*)k_particles_split_batch<SHAPE, DEP, NC>;
+    const bool small = b->knob[PK_BATCH_BLOCK] == 128;
+    const void* fn = small ? (const void*)k_particles_split_batch<SHAPE, DEP, NC, 128>
+                           : (const void*)k_particles_split_batch<SHAPE, DEP, NC, PK_BLOCK>;
     const size_t lds = b->cplds[SHAPE];
     const long nitems = b->cnitems[SHAPE];
     if (nitems == 0) return PK_OK;
@@ -1396,8 +1401,12 @@ int batch_go(pk_batch* b, bool prepare)
     }
     const long wg = b->knob[PK_BATCH_WG_PER_CU];
     const long nb = wg > 0 && wg * 256 < nitems ? wg * 256 : nitems;
-    hipLaunchKernelGGL((k_particles_split_batch<SHAPE, DEP, NC>), dim3((unsigned)nb), dim3(PK_BLOCK), lds,
-                       b->ctx->stream, (const SplitArgs*)b->pjobs, b->citems[SHAPE], nitems, b->halt);
+    if (small)
+        hipLaunchKernelGGL((k_particles_split_batch<SHAPE, DEP, NC, 128>), dim3((unsigned)nb), dim3(128), lds,
+                           b->ctx->stream, (const SplitArgs*)b->pjobs, b->citems[SHAPE], nitems, b->halt);
+    else
+        hipLaunchKernelGGL((k_particles_split_batch<SHAPE, DEP, NC, PK_BLOCK>), dim3((unsigned)nb), dim3(PK_BLOCK),
+                           lds, b->ctx->stream, (const SplitArgs*)b->pjobs, b->citems[SHAPE], nitems, b->halt);
     PK_HIP(b->ctx, hipGetLastError());
     return PK_OK;
 }

@@ -94,11 +94,14 @@ def test_batch_matches_one_by_one(ctx, prec, shape, dep):
     _check(ctx, prec, shape, dep, 6)
 
 
-@pytest.mark.parametrize("wg,order,graph", [(0, 0, 0), (0, 1, 1), (1, 0, 1), (2, 1, 0)])
-def test_batch_knobs(ctx, wg, order, graph):
-    """persistent or one-workgroup-per-item launches, both item orders, graph or eager"""
-    _say("batch knobs wg=%d order=%d graph=%d" % (wg, order, graph))
-    _check(ctx, P.F32, P.TSC, P.ESIRKEPOV, 5, {P.BATCH_WG_PER_CU: wg, P.BATCH_ORDER: order, P.BATCH_GRAPH: graph})
+@pytest.mark.parametrize("wg,order,graph,block", [(0, 0, 0, 256), (0, 1, 1, 256), (1, 0, 1, 256), (2, 1, 0, 256),
+                                                  (0, 1, 1, 128), (2, 0, 1, 128)])
+def test_batch_knobs(ctx, wg, order, graph, block):
+    """persistent or one-workgroup-per-item launches, both item orders, graph or eager, workgroups of
+    four or two wavefronts"""
+    _say("batch knobs wg=%d order=%d graph=%d block=%d" % (wg, order, graph, block))
+    _check(ctx, P.F32, P.TSC, P.ESIRKEPOV, 5, {P.BATCH_WG_PER_CU: wg, P.BATCH_ORDER: order, P.BATCH_GRAPH: graph,
+                                               P.BATCH_BLOCK: block})
 
 
 def test_batch_graph_replays(ctx):
@@ -152,11 +155,12 @@ def test_batch_autotune_restores_state(ctx, prec):
     b = P.Batch(ctx, mine)
     ms = b.autotune(dt, P.TSC, P.ESIRKEPOV, trial_steps=2)
     knobs = {k: b.get(k) for k in (P.BATCH_WG_PER_CU, P.BATCH_ORDER, P.BATCH_GRAPH, P.BATCH_SUPERCELL,
-                                   P.BATCH_INTERLEAVE)}
+                                   P.BATCH_INTERLEAVE, P.BATCH_BLOCK)}
     print("autotune: %.4f ms/step with knobs %s" % (ms, knobs), flush=True)
     assert ms > 0 and knobs[P.BATCH_WG_PER_CU] in (0, 1, 2, 4, 8) and knobs[P.BATCH_ORDER] in (0, 1)
     assert knobs[P.BATCH_GRAPH] in (0, 1) and knobs[P.BATCH_SUPERCELL] in (2, 4)
     assert knobs[P.BATCH_INTERLEAVE] in ((-1,) if prec == P.F64 else (0, 2, 3, 4, 6, 8))
+    assert knobs[P.BATCH_BLOCK] in ((256,) if prec == P.F64 else (128, 256))
     sc = knobs[P.BATCH_SUPERCELL]
     for e, g in ref:
         e.set_supercells((sc, sc, sc))
