@@ -356,7 +356,7 @@ def main():
     ap.add_argument("--no-events", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-plasma batched workload")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-denoiser workload")
-    ap.add_argument("--learned-runs", type=int, default=12,
+    ap.add_argument("--learned-runs", type=int, default=48,
                     help="128^3 run pairs the denoiser trains on (the learned form's full set is 200)")
     ap.add_argument("--batch-n", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=100)

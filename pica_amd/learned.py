@@ -250,7 +250,7 @@ def costs(ctx, dn, ncell=(128, 128, 128), ppc_low=4, ppc_mid=64, low_ncps=(4, 4,
     return cost
 
 
-def run(ctx, train_runs=12, train_grid=(128, 128, 128), eval_grid=(128, 128, 128), schedule=None, seed=0,
+def run(ctx, train_runs=48, train_grid=(128, 128, 128), eval_grid=(128, 128, 128), schedule=None, seed=0,
         ppc_target=256, eval_seeds=(99, 199, 299), snapshots=32, crops=4, edge=32, log=None):
     """the whole learned-form pipeline; returns (report dict, trained Denoiser)"""
     dn = Denoiser(ctx)
