@@ -196,11 +196,13 @@ int pk_batch_set_shape(pk_batch* b, int index, int shape);
 int pk_batch_get(pk_batch* b, int knob, int* value);
 /* nsteps periodic PIC steps (pk_step's sequence) of every ensemble of the batch */
 int pk_batch_run(pk_batch* b, double dt, int shape, int dep, int nsteps);
-/* autotuner: from the current state, restored after every trial, times trial_steps steps of
- * (1) every supercell size 2^3, 4^3, 8^3 that divides all grids (and the batch's original one),
- * (2) FP32: interleave periods 0, 2, 3, 4, 6, 8, (3) every (graph, order, workgroups per CU) of
- * {0,1} x {0,1} x {0,1,2,4,8}, each stage with the best of the previous ones, and keeps the fastest
- * setting (its ms per step in *best_ms; the ensembles end re-bucketed to the chosen supercells) */
+/* autotuner: from the current state, restored after every trial, times trial_steps steps (after
+ * one warm-up step) of (1) every supercell size 2^3, 4^3, 8^3 that divides all grids (and the
+ * batch's original one), (2) FP32: interleave periods 0, 2, 3, 4, 6, 8 (over at least 24 steps:
+ * its gain builds up over tens of steps), (3) FP32: workgroups of 256 and 128 threads, (4) every
+ * (graph, order, workgroups per CU) of {0,1} x {0,1} x {0,1,2,4,8}; each stage starts from the
+ * best setting of the previous ones and keeps its fastest candidate (the last stage's ms per step
+ * in *best_ms; the ensembles end re-bucketed to the chosen supercells) */
 int pk_batch_autotune(pk_batch* b, double dt, int shape, int dep, int trial_steps, double* best_ms);
 
 /* ---- learned form (no counterpart in pica) ----

@@ -457,8 +457,9 @@ int pk_batch_autotune(pk_batch* b, double dt, int shape, int dep, int trial_step
     // supercell edge of the state the autotuner starts from (0 when not cubic)
     const int* n0 = S.ens[0].ncps;
     const int edge0 = n0[0] == n0[1] && n0[1] == n0[2] ? n0[0] : 0;
-    // one trial from the saved state: supercell edge sc (0 = the saved one), then the knobs
-    auto trial = [&](int sc, const int knobs[PK_BATCH_NKNOB], double* ms) -> int {
+    // one trial from the saved state: supercell edge sc (0 = the saved one), then the knobs, timed
+    // over `steps` steps after one warm-up step; *ms = ms per step
+    auto trial = [&](int sc, const int knobs[PK_BATCH_NKNOB], int steps, double* ms) -> int {
         int rr;
         if ((rr = snapshot_restore(b, S))) return rr;
         if (sc > 0 && sc != edge0) {
@@ -471,11 +472,11 @@ int pk_batch_autotune(pk_batch* b, double dt, int shape, int dep, int trial_step
         b->dirty = true;
         if ((rr = run(b, dt, shape, dep, 1))) return rr;     // warm-up, prepares and captures
         if (hipEventRecord(t0, c->stream) != hipSuccess) return PK_E_HIP;
-        if ((rr = run(b, dt, shape, dep, trial_steps))) return rr;
+        if ((rr = run(b, dt, shape, dep, steps))) return rr;
         if (hipEventRecord(t1, c->stream) != hipSuccess || hipEventSynchronize(t1) != hipSuccess) return PK_E_HIP;
         float t = 0.f;
         (void)hipEventElapsedTime(&t, t0, t1);
-        *ms = t;
+        *ms = t / steps;
         return PK_OK;
     };
     int best[PK_BATCH_NKNOB];
@@ -483,10 +484,13 @@ int pk_batch_autotune(pk_batch* b, double dt, int shape, int dep, int trial_step
     best[PK_BATCH_GRAPH] = 1;
     int best_sc = 0;
     double best_t = 1e300;
-    // a candidate that does not fit (supercell past 65,536 particles or the LDS) is skipped
+    // a candidate that does not fit (supercell past 65,536 particles or the LDS) is skipped.  Each
+    // stage compares its own candidates (the current best setting among them) over trials of one
+    // length: best_t is reset when a stage starts.
+    int steps = trial_steps;
     auto consider = [&](int sc, const int knobs[PK_BATCH_NKNOB]) -> int {
         double ms = 0.0;
-        const int rr = trial(sc, knobs, &ms);
+        const int rr = trial(sc, knobs, steps, &ms);
         if (rr == PK_E_CAPACITY) return PK_OK;
         if (rr) return rr;
         if (ms < best_t) {
@@ -511,9 +515,13 @@ int pk_batch_autotune(pk_batch* b, double dt, int shape, int dep, int trial_step
             if ((r = consider(sc, kn))) break;
         }
     }
-    // (2) interleave period (FP32 split ensembles)
+    // (2) interleave period (FP32 split ensembles).  The interleave restores an order that decays
+    // over tens of steps, so these trials run at least 24 steps (a short trial would see only its
+    // cost)
     if (!r && b->prec == PK_F32) {
         const int per[] = {0, 2, 3, 4, 6, 8};
+        steps = trial_steps > 24 ? trial_steps : 24;
+        best_t = 1e300;
         const int sc = best_sc;
         int kn0[PK_BATCH_NKNOB];
         for (int k = 0; k < PK_BATCH_NKNOB; k++) kn0[k] = best[k];
@@ -523,16 +531,23 @@ int pk_batch_autotune(pk_batch* b, double dt, int shape, int dep, int trial_step
             kn[PK_BATCH_INTERLEAVE] = p;
             if ((r = consider(sc, kn))) break;
         }
+        steps = trial_steps;
     }
-    // (3) block shape: two or four wavefronts per workgroup (FP32 split ensembles)
+    // (3) block shape: four or two wavefronts per workgroup (FP32 split ensembles)
     if (!r && b->prec == PK_F32) {
-        int kn[PK_BATCH_NKNOB];
-        for (int k = 0; k < PK_BATCH_NKNOB; k++) kn[k] = best[k];
-        kn[PK_BATCH_BLOCK] = best[PK_BATCH_BLOCK] == 128 ? PK_BLOCK : 128;
-        r = consider(best_sc, kn);
+        best_t = 1e300;
+        int kn0[PK_BATCH_NKNOB];
+        for (int k = 0; k < PK_BATCH_NKNOB; k++) kn0[k] = best[k];
+        for (int bs : {PK_BLOCK, 128}) {
+            int kn[PK_BATCH_NKNOB];
+            for (int k = 0; k < PK_BATCH_NKNOB; k++) kn[k] = kn0[k];
+            kn[PK_BATCH_BLOCK] = bs;
+            if ((r = consider(best_sc, kn))) break;
+        }
     }
     // (4) launch: graph, item order, persistence
     if (!r) {
+        best_t = 1e300;
         const int wgs[] = {0, 1, 2, 4, 8};
         const int sc = best_sc;
         int kn0[PK_BATCH_NKNOB];
@@ -562,7 +577,7 @@ int pk_batch_autotune(pk_batch* b, double dt, int shape, int dep, int trial_step
     if (r) return r == PK_E_HIP ? pk_fail(c, PK_E_HIP, "pk_batch_autotune: event timing failed") : r;
     if (rr) return rr;
     if (best_t >= 1e300) return pk_fail(c, PK_E_CAPACITY, "pk_batch_autotune: no setting fits");
-    if (best_ms) *best_ms = best_t / trial_steps;
+    if (best_ms) *best_ms = best_t;
     return PK_OK;
 }
 
