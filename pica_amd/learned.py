@@ -28,7 +28,7 @@ import time
 
 import numpy as np
 
-from . import (C, DIAG_FIELD_ENERGY, ELECTRON_CHARGE, ESIRKEPOV, F32, JX, JY, JZ, TSC, Denoiser,
+from . import (C, DIAG_FIELD_ENERGY, ELECTRON_CHARGE, ESIRKEPOV, F32, F64, JX, JY, JZ, TSC, Denoiser,
                EnsembleSupercells, YeeGrid, diag, plasma_parameters, step, step_denoised)
 
 UTH = 0.01        # thermal spread of u = p/(mc)
@@ -58,10 +58,10 @@ def network_scale():
     return 1.0 / (abs(ELECTRON_CHARGE) * n0 * C * U_AMP)
 
 
-def plasma(ctx, ncell, ppc, modes, seed, ncps=(4, 4, 4)):
+def plasma(ctx, ncell, ppc, modes, seed, ncps=(4, 4, 4), prec=F32):
     _, w = plasma_parameters(DT, ppc)
-    g = YeeGrid.periodic(ctx, ncell, prec=F32)
-    e = EnsembleSupercells(ctx, ncell, (1.0, 1.0, 1.0), ncps, prec=F32)
+    g = YeeGrid.periodic(ctx, ncell, prec=prec)
+    e = EnsembleSupercells(ctx, ncell, (1.0, 1.0, 1.0), ncps, prec=prec)
     e.load_modes(ppc, UTH, U_DRIFT, w, seed, modes)
     return e, g
 
@@ -222,12 +222,17 @@ def evaluate(ctx, dn, ncell=(128, 128, 128), ppc_low=4, ppc_mid=64, ppc_ref=256,
 
 
 def costs(ctx, dn, ncell=(128, 128, 128), ppc_low=4, ppc_mid=64, low_ncps=(4, 4, 4), seed=5):
-    """ms per step of a mid-ppc plain step and of a low-ppc denoised step (fresh plasmas), with the
-    per-kernel HIP-event times of the denoised step"""
+    """ms per step of a mid-ppc plain step (FP32 split, as the denoised step, and FP64, pica's
+    precision) and of a low-ppc denoised step (fresh plasmas), with the per-kernel HIP-event times of
+    the denoised step"""
     modes = random_modes(np.random.default_rng(seed), ncell)
     cost = {}
     mid = plasma(ctx, ncell, ppc_mid, modes, 1)
     cost["mid_plain_ms"] = 1e3 * timed_steps(lambda: step(*mid, DT, TSC, ESIRKEPOV), 10, ctx.sync)
+    for obj in mid:
+        obj.close()
+    mid = plasma(ctx, ncell, ppc_mid, modes, 1, prec=F64)
+    cost["mid_plain_fp64_ms"] = 1e3 * timed_steps(lambda: step(*mid, DT, TSC, ESIRKEPOV), 10, ctx.sync)
     for obj in mid:
         obj.close()
     low = plasma(ctx, ncell, ppc_low, modes, 2, low_ncps)
@@ -246,6 +251,7 @@ def costs(ctx, dn, ncell=(128, 128, 128), ppc_low=4, ppc_mid=64, low_ncps=(4, 4,
     for obj in low:
         obj.close()
     cost["mid_over_low_denoised"] = cost["mid_plain_ms"] / cost["low_denoised_ms"]
+    cost["mid_fp64_over_low_denoised"] = cost["mid_plain_fp64_ms"] / cost["low_denoised_ms"]
     cost["low_supercells"] = list(low_ncps)
     return cost
 
