@@ -67,7 +67,7 @@ def test_direct_deposition_is_not_charge_conserving(ctx):
 
 
 @pytest.mark.parametrize("prec,n,steps,tol", [(P.F64, 64, 20, 1e-11), (P.F32, 64, 20, 1e-4),
-                                              (P.F32, 256, 5, 1e-4)])
+                                              (P.F32, 256, 5, 1e-4), (P.F64, 256, 5, 1e-11)])
 def test_gauss_law_drift(ctx, prec, n, steps, tol):
     _say("gauss drift prec=%d n=%d" % (prec, n))
     dt, g, ens = _run(ctx, prec, n, 16 if n == 256 else 8, P.TSC, P.ESIRKEPOV, 0)
@@ -112,3 +112,21 @@ def test_device_continuity_residual(ctx, prec, tol, shape):
             assert d[P.DIAG_CONTINUITY_MAX] == pytest.approx(host, rel=1e-3)
         ens.close()
         g.close()
+
+
+@pytest.mark.parametrize("prec,tol", [(P.F64, 1e-11), (P.F32, 2e-4)])
+def test_device_continuity_at_256cube(ctx, prec, tol):
+    """the headline shape (256^3, 16 ppc, two-stream, TSC, Esirkepov): after three steps the device
+    continuity residual of the next step, max |(rho1 - rho0)/dt + div J| over max |div J|, stays
+    at round-off (a size-independent property; the oracle cannot run this size)"""
+    _say("device continuity 256^3 prec=%d" % prec)
+    dt, g, ens = _run(ctx, prec, 256, 16, P.TSC, P.ESIRKEPOV, 0)
+    P.run(ens, g, dt, 3, P.TSC, P.ESIRKEPOV)
+    P.rho_save(ens, g, P.TSC)
+    P.step(ens, g, dt, P.TSC, P.ESIRKEPOV)
+    d = P.diag(ens, g, P.TSC)
+    r = d[P.DIAG_CONTINUITY_MAX] / d[P.DIAG_DIVJ_MAX]
+    print("256^3 prec=%d: continuity residual %.3g of max |div J| %.3g" % (prec, r, d[P.DIAG_DIVJ_MAX]), flush=True)
+    assert d[P.DIAG_DIVJ_MAX] > 0 and r < tol
+    ens.close()
+    g.close()
