@@ -257,7 +257,11 @@ def run_learned(runs):
     import pica_amd as P
     from pica_amd import learned
     ctx = P.Context(0)
-    report, dn = learned.run(ctx, train_runs=runs)
+    def log(msg):                      # every 8th training run pair, so that a long leg shows progress
+        done = int(msg.split()[2].split("/")[0]) if msg.startswith("training run ") else 0
+        if done % 8 == 0:
+            print(msg, file=sys.stderr, flush=True)
+    report, dn = learned.run(ctx, train_runs=runs, log=log)
     dn.close()
     ctx.close()
     ev = report["eval"]
@@ -351,8 +355,8 @@ def main():
     ap.add_argument("--dep", default="esirkepov", choices=["direct", "esirkepov"])
     ap.add_argument("--prec", default="both", choices=["f64", "f32", "both"],
                     help="f32: the split-position FP32 form (the headline value); f64: the plain FP64 path")
-    ap.add_argument("--cpu-n", type=int, default=128)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-n", type=int, default=256, help="CPU baseline grid edge (the workload's own: 256)")
+    ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-events", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-plasma batched workload")
