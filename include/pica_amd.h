@@ -279,6 +279,12 @@ int pk_step(pk_ens* e, pk_grid* g, double dt, int shape, int dep);
  * one field update; ensembles may be electrons and mobile ions alike */
 int pk_step_species(pk_ens* const* ens, int n, pk_grid* g, double dt, int shape, int dep);
 int pk_run(pk_ens* e, pk_grid* g, double dt, int shape, int dep, int nsteps);
+/* pk_run without a host synchronisation between steps (no pica counterpart: pica's loop is the
+ * caller's): the step is captured once into a HIP graph and replayed nsteps times, a slack overflow
+ * stops the replay on the device and is rebuilt once the run returns (pk_batch_run on a batch of
+ * this one ensemble and grid).  For grids small enough that launches and per-step synchronisation
+ * dominate; results equal pk_run's to the batched form's tolerance (atomic order). */
+int pk_run_captured(pk_ens* e, pk_grid* g, double dt, int shape, int dep, int nsteps);
 
 /* ---- diagnostics (FP64 reductions on the device) ----
  * rho of the ensemble with the given shape at cell centres into the grid's PK_RHO component

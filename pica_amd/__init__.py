@@ -59,6 +59,7 @@ ABI_SYMBOLS = (
     "pk_dn_clear_pairs", "pk_dn_train", "pk_dn_apply", "pk_rho_save", "pk_project_continuity", "pk_step_denoised", "pk_grid_save", "pk_grid_load",
     "pk_ens_save", "pk_ens_load", "pk_step_species", "pk_ens_interleave", "pk_diag_energies_static",
     "pk_ens_set_supercells", "pk_batch_set_shape", "pk_dn_set_charge_gain", "pk_dn_add_pair_crops",
+    "pk_run_captured",
 )
 
 _LIB = None
@@ -121,6 +122,7 @@ def lib():
         L.pk_ens_resort.argtypes = [vp]
         L.pk_step.argtypes = [vp, vp, ctypes.c_double, ctypes.c_int, ctypes.c_int]
         L.pk_run.argtypes = [vp, vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.pk_run_captured.argtypes = [vp, vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.pk_deposit_rho.argtypes = [vp, vp, ctypes.c_int]
         L.pk_diag.argtypes = [vp, vp, ctypes.c_int, vp]
         L.pk_diag_set_reference.argtypes = [vp, vp, ctypes.c_int]
@@ -616,6 +618,11 @@ def step_species(ensembles, grid, dt, shape=CIC, dep=DIRECT):
 
 def run(ens, grid, dt, nsteps, shape=CIC, dep=DIRECT):
     ens.ctx.check(lib().pk_run(ens.h, grid.h, float(dt), shape, dep, int(nsteps)))
+
+
+def run_captured(ens, grid, dt, nsteps, shape=CIC, dep=DIRECT):
+    """run() with the step captured into a HIP graph and no host synchronisation between steps"""
+    ens.ctx.check(lib().pk_run_captured(ens.h, grid.h, float(dt), shape, dep, int(nsteps)))
 
 
 def deposit_rho(ens, grid, shape=CIC):

@@ -546,6 +546,21 @@ int pk_run(pk_ens* e, pk_grid* g, double dt, int shape, int dep, int nsteps)
     return PK_OK;
 }
 
+int pk_run_captured(pk_ens* e, pk_grid* g, double dt, int shape, int dep, int nsteps)
+{
+    int r = pk_check_pair(e, g);
+    if (r) return r;
+    if (nsteps < 0) return pk_fail(e->ctx, PK_E_ARG, "pk_run_captured: negative step count");
+    if (nsteps == 0) return PK_OK;
+    pk_batch* b = nullptr;
+    pk_ens* const es[1] = {e};
+    pk_grid* const gs[1] = {g};
+    if ((r = pk_batch_create(e->ctx, 1, es, gs, &b))) return r;
+    if (!(r = pk_batch_set(b, PK_BATCH_GRAPH, 1))) r = pk_batch_run(b, dt, shape, dep, nsteps);
+    pk_batch_destroy(b);
+    return r;
+}
+
 // ---------------------------------------------------------------- diagnostics
 int pk_deposit_rho(pk_ens* e, pk_grid* g, int shape)
 {

@@ -279,3 +279,44 @@ def test_batch_interleave_matches_one_by_one(ctx, period, graph, uth, slack, spe
     if slack is not None:
         assert rebuilds > 0
     b.close()
+
+
+@pytest.mark.parametrize("prec,shape,dep,period,slack", [(P.F64, P.TSC, P.ESIRKEPOV, 0, None),
+                                                         (P.F32, P.TSC, P.ESIRKEPOV, 3, None),
+                                                         (P.F64, P.CIC, P.DIRECT, 0, (0.0, 2)),
+                                                         (P.F32, P.CIC, P.ESIRKEPOV, 2, (0.0, 2))])
+def test_run_captured_matches_run(ctx, prec, shape, dep, period, slack):
+    """pk_run_captured (the step captured once, no host synchronisation between steps) gives
+    pk_run's particles, order and fields, with the interleave at the same re-sort counts and with
+    slack overflows rebuilt after the replay stops on the device"""
+    _say("run captured prec=%d shape=%d dep=%d period=%d slack=%s" % (prec, shape, dep, period, slack))
+    ctx.set_option(P.OPT_INTERLEAVE, period)
+    dt = courant_dt()
+    for spec in SPECS[:2]:
+        (e, g), (re_, rg) = _pair(ctx, prec, spec, slack=slack), _pair(ctx, prec, spec, slack=slack)
+        P.run_captured(e, g, dt, 9, shape, dep)
+        P.run(re_, rg, dt, 9, shape, dep)
+        _compare(prec, _state(e, g), _state(re_, rg), "ensemble %s" % (spec,))
+        P.run_captured(e, g, dt, 0, shape, dep)              # no steps: nothing changes
+        _compare(prec, _state(e, g), _state(re_, rg), "ensemble %s after 0 steps" % (spec,))
+
+
+def test_run_captured_config0_timing(ctx):
+    """the 32^3 x 8 ppc plasma of config 0 (CIC, direct, FP64), 100 steps each way from the same
+    particles: the captured run matches pk_run and is timed beside it"""
+    _say("run captured timing, config 0 shape")
+    import time
+    dt = courant_dt()
+    spec = ((32, 32, 32), 8, 21)
+    runs = []
+    for fn in (P.run, P.run_captured):
+        e, g = _pair(ctx, P.F64, spec, uth=0.01)
+        fn(e, g, dt, 2, P.CIC, P.DIRECT)
+        ctx.sync()
+        t0 = time.perf_counter()
+        fn(e, g, dt, 100, P.CIC, P.DIRECT)
+        ctx.sync()
+        runs.append((time.perf_counter() - t0, _state(e, g)))
+    print("config-0 shape, 100 steps: pk_run %.2f ms per step, pk_run_captured %.2f ms per step"
+          % (10.0 * runs[0][0], 10.0 * runs[1][0]), flush=True)
+    _compare(P.F64, runs[1][1], runs[0][1], "config-0 shape")
