@@ -500,8 +500,7 @@ int pk_step(pk_ens* e, pk_grid* g, double dt, int shape, int dep)
 {
     int r = pk_check_pair(e, g);
     if (r) return r;
-    for (int comp = PK_JX; comp <= PK_JZ; comp++)
-        if ((r = pk_field_zero(g, comp))) return r;
+    if ((r = pk_field_zero_j(g))) return r;
     if ((r = pk_particle_step(e, g, dt, shape, dep))) return r;
     if ((r = pk_field_ghost(g, PK_FOLD_J | PK_FILL_J))) return r;
     // B(dt/2), E(dt), B(dt/2) with ghost fills (the leapfrog around YeeSolver.h:28-38)
@@ -520,8 +519,7 @@ int pk_step_species(pk_ens* const* ens, int n, pk_grid* g, double dt, int shape,
     int r;
     for (int k = 0; k < n; k++)
         if ((r = pk_check_pair(ens[k], g))) return r;
-    for (int comp = PK_JX; comp <= PK_JZ; comp++)
-        if ((r = pk_field_zero(g, comp))) return r;
+    if ((r = pk_field_zero_j(g))) return r;
     // every species deposits into the same J (the particle stage adds to it)
     for (int k = 0; k < n; k++)
         if ((r = pk_particle_step(ens[k], g, dt, shape, dep))) return r;

@@ -1087,7 +1087,13 @@ int pk_dn_train(pk_dn* d, int iters, int batch, double lr, uint64_t seed, float*
     return r;
 }
 
-int pk_dn_apply(pk_dn* d, pk_grid* g)
+int pk_dn_apply(pk_dn* d, pk_grid* g) { return pk_dn_apply_impl(d, g, true); }
+
+}  // extern "C"
+
+// pk_dn_apply; fill = false leaves J's ghosts as they were (for a caller that reads J only through
+// the physical cells until it fills them itself)
+int pk_dn_apply_impl(pk_dn* d, pk_grid* g, bool fill)
 {
     if (!d || !g) return PK_E_ARG;
     if (g->ctx != d->ctx) return pk_fail(d->ctx, PK_E_ARG, "pk_dn_apply: other context");
@@ -1114,11 +1120,9 @@ int pk_dn_apply(pk_dn* d, pk_grid* g)
         hipLaunchKernelGGL(k_f32_to_grid<float>, dim3(nblk(V.nv, 256)), dim3(256), 0, s, grid_view<float>(g), V,
                            1.0 / d->scale, d->out);
     PK_HIP(d->ctx, hipGetLastError());
-    if (g->ghost > 0) return pk_field_ghost(g, PK_FILL_J);
+    if (fill && g->ghost > 0) return pk_field_ghost(g, PK_FILL_J);
     return PK_OK;
 }
-
-}  // extern "C"
 
 // the device gain table of pk_step_denoised's projection (n = 0: none)
 void pk_dn_charge_gain(const pk_dn* d, const double** gain, int* n, double* kmax)

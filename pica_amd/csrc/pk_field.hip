@@ -332,6 +332,20 @@ int batch_launch(pk_batch* b, int op, unsigned mask, int d)
 
 }  // namespace
 
+int pk_field_zero_j(pk_grid* g)
+{
+    const long n = (long)(2 * g->pitch + g->n);
+    pk_ktimer tm(g->ctx, "zero");
+    if (g->prec == PK_F64)
+        hipLaunchKernelGGL(k_fill_zero<double>, dim3(nblk(n, 256)), dim3(256), 0, g->ctx->stream,
+                           (double*)g->comp(PK_JX), n);
+    else
+        hipLaunchKernelGGL(k_fill_zero<float>, dim3(nblk(n, 256)), dim3(256), 0, g->ctx->stream,
+                           (float*)g->comp(PK_JX), n);
+    PK_HIP(g->ctx, hipGetLastError());
+    return PK_OK;
+}
+
 int pk_field_zero(pk_grid* g, int comp)
 {
     const long n = (long)g->n;

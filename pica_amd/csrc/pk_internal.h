@@ -269,6 +269,10 @@ int pk_ens_rebuild_after_overflow(pk_ens* e);   // rebuild of an overflowed re-s
 
 // ---- launchers implemented in the .hip files ----
 int pk_field_zero(pk_grid* g, int comp);
+// Jx, Jy, Jz zeroed in one launch (they are consecutive components of the grid's block)
+int pk_field_zero_j(pk_grid* g);
+// pk_dn_apply; with fill = false J's ghosts are left as they were
+int pk_dn_apply_impl(pk_dn* d, pk_grid* g, bool fill);
 int pk_field_yee_e(pk_grid* g, double dt);
 int pk_field_yee_b(pk_grid* g, double dt);
 int pk_field_ghost(pk_grid* g, int ops);

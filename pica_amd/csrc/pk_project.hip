@@ -58,16 +58,17 @@ __global__ void k_residual(GridView<T> g, Phys P, const double* __restrict__ rho
     w[v] = Cx<T>{(T)r, (T)0};
 }
 
-// D J at physical node v, FP64; J ghosts must be filled
+// D J at physical node v, FP64, from the physical cells alone (the periodic neighbour of the last
+// cell is the first one: the value a filled ghost holds, so J ghosts need not be filled)
 template <class T>
 __device__ __forceinline__ double div_j(const GridView<T>& g, const Phys& P, long v)
 {
     const int i = (int)(v / ((long)P.n[1] * P.n[2])), j = (int)((v / P.n[2]) % P.n[1]), k = (int)(v % P.n[2]);
+    const int ip = i + 1 == P.n[0] ? 0 : i + 1, jp = j + 1 == P.n[1] ? 0 : j + 1, kp = k + 1 == P.n[2] ? 0 : k + 1;
     const long c = grid_index(g.s1, g.s2, g.G, i, j, k);
-    const long s12 = (long)g.s1 * g.s2;
-    double r = ((double)g.f[PK_JX][c + s12] - (double)g.f[PK_JX][c]) / g.step[0];
-    r += ((double)g.f[PK_JY][c + g.s2] - (double)g.f[PK_JY][c]) / g.step[1];
-    r += ((double)g.f[PK_JZ][c + 1] - (double)g.f[PK_JZ][c]) / g.step[2];
+    double r = ((double)g.f[PK_JX][grid_index(g.s1, g.s2, g.G, ip, j, k)] - (double)g.f[PK_JX][c]) / g.step[0];
+    r += ((double)g.f[PK_JY][grid_index(g.s1, g.s2, g.G, i, jp, k)] - (double)g.f[PK_JY][c]) / g.step[1];
+    r += ((double)g.f[PK_JZ][grid_index(g.s1, g.s2, g.G, i, j, kp)] - (double)g.f[PK_JZ][c]) / g.step[2];
     return r;
 }
 
@@ -422,10 +423,11 @@ int pk_step_denoised(pk_ens* e, pk_grid* g, pk_dn* d, double dt, int shape, int 
     // the direct current needs the charge density before the push; the Esirkepov current is its
     // own continuity reference (D J_esirkepov = -(rho1 - rho0) / dt exactly)
     if (dep != PK_ESIRKEPOV && (r = save_rho(e, g, shape))) return r;
-    for (int comp = PK_JX; comp <= PK_JZ; comp++)
-        if ((r = pk_field_zero(g, comp))) return r;
+    if ((r = pk_field_zero_j(g))) return r;
     if ((r = pk_particle_step(e, g, dt, shape, dep))) return r;
-    if ((r = pk_field_ghost(g, PK_FOLD_J | PK_FILL_J))) return r;
+    // Esirkepov: the saved divergence, the denoiser and the residual read J's physical cells only,
+    // so its ghosts are filled once, after the projection
+    if ((r = pk_field_ghost(g, dep == PK_ESIRKEPOV ? PK_FOLD_J : PK_FOLD_J | PK_FILL_J))) return r;
     if (dep == PK_ESIRKEPOV) {
         if ((r = ensure_proj(g, P))) return r;
         if (g->prec == PK_F64)
@@ -435,7 +437,7 @@ int pk_step_denoised(pk_ens* e, pk_grid* g, pk_dn* d, double dt, int shape, int 
             hipLaunchKernelGGL(k_save_div<float>, dim3(nblk(P.nv, 256)), dim3(256), 0, g->ctx->stream,
                                grid_view<float>(g), P, jref_of(g, P));
         PK_HIP(e->ctx, hipGetLastError());
-        if ((r = pk_dn_apply(d, g))) return r;
+        if ((r = pk_dn_apply_impl(d, g, false))) return r;
         {
             pk_ktimer tm(e->ctx, "project");
             ChargeGain cg;
