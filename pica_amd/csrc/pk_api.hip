@@ -504,12 +504,9 @@ int pk_step(pk_ens* e, pk_grid* g, double dt, int shape, int dep)
     if ((r = pk_particle_step(e, g, dt, shape, dep))) return r;
     if ((r = pk_field_ghost(g, PK_FOLD_J | PK_FILL_J))) return r;
     // B(dt/2), E(dt), B(dt/2) with ghost fills (the leapfrog around YeeSolver.h:28-38)
-    if ((r = pk_field_yee_b(g, dt * 0.5))) return r;
-    if ((r = pk_field_ghost(g, PK_FILL_B))) return r;
-    if ((r = pk_field_yee_e(g, dt))) return r;
-    if ((r = pk_field_ghost(g, PK_FILL_E))) return r;
-    if ((r = pk_field_yee_b(g, dt * 0.5))) return r;
-    if ((r = pk_field_ghost(g, PK_FILL_B))) return r;
+    if ((r = pk_field_yee_periodic(g, dt * 0.5, false))) return r;
+    if ((r = pk_field_yee_periodic(g, dt, true))) return r;
+    if ((r = pk_field_yee_periodic(g, dt * 0.5, false))) return r;
     return pk_ens_resort(e);
 }
 
@@ -524,12 +521,9 @@ int pk_step_species(pk_ens* const* ens, int n, pk_grid* g, double dt, int shape,
     for (int k = 0; k < n; k++)
         if ((r = pk_particle_step(ens[k], g, dt, shape, dep))) return r;
     if ((r = pk_field_ghost(g, PK_FOLD_J | PK_FILL_J))) return r;
-    if ((r = pk_field_yee_b(g, dt * 0.5))) return r;
-    if ((r = pk_field_ghost(g, PK_FILL_B))) return r;
-    if ((r = pk_field_yee_e(g, dt))) return r;
-    if ((r = pk_field_ghost(g, PK_FILL_E))) return r;
-    if ((r = pk_field_yee_b(g, dt * 0.5))) return r;
-    if ((r = pk_field_ghost(g, PK_FILL_B))) return r;
+    if ((r = pk_field_yee_periodic(g, dt * 0.5, false))) return r;
+    if ((r = pk_field_yee_periodic(g, dt, true))) return r;
+    if ((r = pk_field_yee_periodic(g, dt * 0.5, false))) return r;
     for (int k = 0; k < n; k++)
         if ((r = pk_ens_resort(ens[k]))) return r;
     return PK_OK;

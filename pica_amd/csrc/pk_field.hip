@@ -70,6 +70,76 @@ __global__ void __launch_bounds__(256) k_yee_b(GridView<T> g, YeeCoef<T> k, long
     if (c < n) yee_b_at(g, k, c);
 }
 
+// ---- the periodic step's Yee updates: each thread updates one physical cell with the expressions
+// above (at a physical cell every stencil is inside the array) and writes the new value into the
+// cell and into every ghost cell that images it, which is what the whole-array update followed by
+// a ghost fill leaves there; so the fill launch is not needed.  The update reads only its own cell
+// of the component it writes, and neighbours of the other field, whose ghosts are filled.
+constexpr int MAX_IMAGES = 1 + 2 * PK_MAX_PERIODIC_GHOST;
+
+// array indices along one dimension whose periodic image is physical index p (n physical cells,
+// G ghost layers a side): p itself, then the low ghosts, then the high ghosts
+__device__ __forceinline__ int image_list(int p, int n, int G, int out[MAX_IMAGES])
+{
+    int m = 0;
+    out[m++] = G + p;
+    for (int a = p + G - n; a >= 0; a -= n) out[m++] = a;
+    for (int b = p; b < G; b += n) out[m++] = G + n + b;
+    return m;
+}
+
+template <class T>
+__device__ __forceinline__ void write_images(const GridView<T>& g, int n0, int n1, int n2, int i, int j, int k, int c0,
+                                             T v0, T v1, T v2)
+{
+    int li[MAX_IMAGES], lj[MAX_IMAGES], lk[MAX_IMAGES];
+    const int mi = image_list(i, n0, g.G, li), mj = image_list(j, n1, g.G, lj), mk = image_list(k, n2, g.G, lk);
+    for (int a = 0; a < mi; a++)
+        for (int b = 0; b < mj; b++)
+            for (int d = 0; d < mk; d++) {
+                const long idx = ((long)li[a] * g.s1 + lj[b]) * g.s2 + lk[d];
+                g.f[c0][idx] = v0;
+                g.f[c0 + 1][idx] = v1;
+                g.f[c0 + 2][idx] = v2;
+            }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_yee_e_periodic(GridView<T> g, YeeCoef<T> k, int n0, int n1, int n2)
+{
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= (long)n0 * n1 * n2) return;
+    const int i = (int)(v / ((long)n1 * n2)), j = (int)((v / n2) % n1), kz = (int)(v % n2);
+    const int s2 = g.s2;
+    const long s12 = (long)g.s1 * g.s2;
+    const long c = ((long)(i + g.G) * g.s1 + (j + g.G)) * s2 + (kz + g.G);
+    const T* bx = g.f[PK_BX];
+    const T* by = g.f[PK_BY];
+    const T* bz = g.f[PK_BZ];
+    const T ex = g.f[PK_EX][c] + (k.cc * g.f[PK_JX][c] + k.ky * (bz[c + s2] - bz[c]) - k.kz * (by[c + 1] - by[c]));
+    const T ey = g.f[PK_EY][c] + (k.cc * g.f[PK_JY][c] + k.kz * (bx[c + 1] - bx[c]) - k.kx * (bz[c + s12] - bz[c]));
+    const T ez = g.f[PK_EZ][c] + (k.cc * g.f[PK_JZ][c] + k.kx * (by[c + s12] - by[c]) - k.ky * (bx[c + s2] - bx[c]));
+    write_images(g, n0, n1, n2, i, j, kz, PK_EX, ex, ey, ez);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_yee_b_periodic(GridView<T> g, YeeCoef<T> k, int n0, int n1, int n2)
+{
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= (long)n0 * n1 * n2) return;
+    const int i = (int)(v / ((long)n1 * n2)), j = (int)((v / n2) % n1), kz = (int)(v % n2);
+    const int s2 = g.s2;
+    const long s12 = (long)g.s1 * g.s2;
+    const long c = ((long)(i + g.G) * g.s1 + (j + g.G)) * s2 + (kz + g.G);
+    const T* ex = g.f[PK_EX];
+    const T* ey = g.f[PK_EY];
+    const T* ez = g.f[PK_EZ];
+    const T bx = g.f[PK_BX][c] + (k.kz * (ey[c] - ey[c - 1]) - k.ky * (ez[c] - ez[c - s2]));
+    const T by = g.f[PK_BY][c] + (k.kx * (ez[c] - ez[c - s12]) - k.kz * (ex[c] - ex[c - 1]));
+    const T bz = g.f[PK_BZ][c] + (k.ky * (ex[c] - ex[c - s2]) - k.kx * (ey[c] - ey[c - s12]));
+    write_images(g, n0, n1, n2, i, j, kz, PK_BX, bx, by, bz);
+}
+
 // Periodic fill: every ghost cell takes the value of its wrapped physical image.  The ghost
 // cells are enumerated as x slabs (full y,z), then y slabs (physical x, full z), then z slabs
 // (physical x,y); the closed-form image equals the dimension-by-dimension copy.
@@ -361,6 +431,38 @@ int pk_field_zero(pk_grid* g, int comp)
 }
 
 int pk_field_yee_e(pk_grid* g, double dt) { return g->prec == PK_F64 ? yee_e<double>(g, dt) : yee_e<float>(g, dt); }
+
+namespace {
+template <class T>
+int yee_periodic(pk_grid* g, double dt, bool e_update)
+{
+    const int G = g->ghost;
+    const int n0 = g->s[0] - 2 * G, n1 = g->s[1] - 2 * G, n2 = g->s[2] - 2 * G;
+    const long n = (long)n0 * n1 * n2;
+    pk_ktimer tm(g->ctx, e_update ? "yee_e" : "yee_b");
+    if (e_update)
+        hipLaunchKernelGGL(k_yee_e_periodic<T>, dim3(nblk(n, 256)), dim3(256), 0, g->ctx->stream, grid_view<T>(g),
+                           yee_coef<T>(g, dt, true), n0, n1, n2);
+    else
+        hipLaunchKernelGGL(k_yee_b_periodic<T>, dim3(nblk(n, 256)), dim3(256), 0, g->ctx->stream, grid_view<T>(g),
+                           yee_coef<T>(g, dt, false), n0, n1, n2);
+    PK_HIP(g->ctx, hipGetLastError());
+    return PK_OK;
+}
+}  // namespace
+
+int pk_field_yee_periodic(pk_grid* g, double dt, bool e_update)
+{
+    const int G = g->ghost;
+    if (G < 1 || G > PK_MAX_PERIODIC_GHOST) {
+        // not the periodic step's layout: pica's whole-array update, then the fill
+        const int r = e_update ? pk_field_yee_e(g, dt) : pk_field_yee_b(g, dt);
+        return r ? r : pk_field_ghost(g, e_update ? PK_FILL_E : PK_FILL_B);
+    }
+    for (int d = 0; d < 3; d++)
+        if (g->s[d] - 2 * G < 1) return pk_fail(g->ctx, PK_E_ARG, "Yee update: no physical cells");
+    return g->prec == PK_F64 ? yee_periodic<double>(g, dt, e_update) : yee_periodic<float>(g, dt, e_update);
+}
 int pk_field_yee_b(pk_grid* g, double dt) { return g->prec == PK_F64 ? yee_b<double>(g, dt) : yee_b<float>(g, dt); }
 int pk_field_ghost(pk_grid* g, int ops) { return g->prec == PK_F64 ? ghost<double>(g, ops) : ghost<float>(g, ops); }
 

@@ -276,6 +276,10 @@ int pk_dn_apply_impl(pk_dn* d, pk_grid* g, bool fill);
 int pk_field_yee_e(pk_grid* g, double dt);
 int pk_field_yee_b(pk_grid* g, double dt);
 int pk_field_ghost(pk_grid* g, int ops);
+// the periodic step's Yee update (E when e_update, else B): the whole-array update followed by the
+// fill of that field's ghosts, in one launch over the physical cells
+#define PK_MAX_PERIODIC_GHOST 2
+int pk_field_yee_periodic(pk_grid* g, double dt, bool e_update);
 int pk_particles_launch(pk_ens* e, pk_grid* g, double dt, int shape, int dep);
 int pk_ens_insert_launch(pk_ens* e);
 int pk_ens_rebuild(pk_ens* e, int source);   // source: 0 = main+outbox, 1 = staging (upload)

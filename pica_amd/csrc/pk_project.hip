@@ -451,12 +451,9 @@ int pk_step_denoised(pk_ens* e, pk_grid* g, pk_dn* d, double dt, int shape, int 
         if ((r = pk_dn_apply(d, g))) return r;
         if ((r = pk_project_continuity(e, g, shape, dt))) return r;
     }
-    if ((r = pk_field_yee_b(g, dt * 0.5))) return r;
-    if ((r = pk_field_ghost(g, PK_FILL_B))) return r;
-    if ((r = pk_field_yee_e(g, dt))) return r;
-    if ((r = pk_field_ghost(g, PK_FILL_E))) return r;
-    if ((r = pk_field_yee_b(g, dt * 0.5))) return r;
-    return pk_field_ghost(g, PK_FILL_B);
+    if ((r = pk_field_yee_periodic(g, dt * 0.5, false))) return r;
+    if ((r = pk_field_yee_periodic(g, dt, true))) return r;
+    return pk_field_yee_periodic(g, dt * 0.5, false);
 }
 
 }  // extern "C"
