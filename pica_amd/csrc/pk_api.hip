@@ -502,7 +502,7 @@ int pk_step(pk_ens* e, pk_grid* g, double dt, int shape, int dep)
     if (r) return r;
     if ((r = pk_field_zero_j(g))) return r;
     if ((r = pk_particle_step(e, g, dt, shape, dep))) return r;
-    if ((r = pk_field_ghost(g, PK_FOLD_J | PK_FILL_J))) return r;
+    if ((r = pk_field_fold_fill_j(g, true))) return r;
     // B(dt/2), E(dt), B(dt/2) with ghost fills (the leapfrog around YeeSolver.h:28-38)
     if ((r = pk_field_yee_periodic(g, dt * 0.5, false))) return r;
     if ((r = pk_field_yee_periodic(g, dt, true))) return r;
@@ -520,7 +520,7 @@ int pk_step_species(pk_ens* const* ens, int n, pk_grid* g, double dt, int shape,
     // every species deposits into the same J (the particle stage adds to it)
     for (int k = 0; k < n; k++)
         if ((r = pk_particle_step(ens[k], g, dt, shape, dep))) return r;
-    if ((r = pk_field_ghost(g, PK_FOLD_J | PK_FILL_J))) return r;
+    if ((r = pk_field_fold_fill_j(g, true))) return r;
     if ((r = pk_field_yee_periodic(g, dt * 0.5, false))) return r;
     if ((r = pk_field_yee_periodic(g, dt, true))) return r;
     if ((r = pk_field_yee_periodic(g, dt * 0.5, false))) return r;

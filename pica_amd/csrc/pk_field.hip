@@ -140,6 +140,76 @@ __global__ void __launch_bounds__(256) k_yee_b_periodic(GridView<T> g, YeeCoef<T
     write_images(g, n0, n1, n2, i, j, kz, PK_BX, bx, by, bz);
 }
 
+// ---- the periodic step's J fold in one launch: every physical cell gathers the ghost cells that
+// image it, in the order and association of the fold dimension by dimension (x, then y, then z;
+// along a dimension the cell itself, then the low ghosts, then the high ghosts, ascending), so its
+// value is that fold's bit for bit.  Ghost cells keep their values (the step fills them next, or
+// reads only physical cells until it does); a cell with no image along any dimension is left alone.
+// ordered list of the array indices that fold into physical index p along one dimension
+__device__ __forceinline__ int fold_list(int p, int n, int G, int out[MAX_IMAGES])
+{
+    int m = 0;
+    out[m++] = G + p;
+    for (int a = 0; a < G; a++) {              // low ghost a images (a - G) mod n
+        int r = (a - G) % n;
+        if (r < 0) r += n;
+        if (r == p) out[m++] = a;
+    }
+    for (int b = 0; b < G; b++)                // high ghost G + n + b images b mod n
+        if (b % n == p) out[m++] = G + n + b;
+    return m;
+}
+
+// the physical cells within G of a face, each once: slab x (i near a face; any j, k), then slab y
+// (i inside; j near a face), then slab z (i, j inside; k near a face).  nb = cells near a face
+// along a dimension (all n when n <= 2G)
+__device__ __forceinline__ int near_face(int q, int n, int G) { return n <= 2 * G || q < G ? q : n - 2 * G + q; }
+__device__ __forceinline__ int inside(int r, int G) { return G + r; }   // (only when n > 2G)
+
+template <class T>
+__global__ void __launch_bounds__(256) k_fold_j_gather(GridView<T> g, int n0, int n1, int n2)
+{
+    const int G = g.G;
+    const int b0 = n0 <= 2 * G ? n0 : 2 * G, b1 = n1 <= 2 * G ? n1 : 2 * G, b2 = n2 <= 2 * G ? n2 : 2 * G;
+    const long sx = (long)b0 * n1 * n2, sy = (long)(n0 - b0) * b1 * n2, sz = (long)(n0 - b0) * (n1 - b1) * b2;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int i, j, k;
+    if (t < sx) {
+        i = near_face((int)(t / ((long)n1 * n2)), n0, G);
+        j = (int)((t / n2) % n1);
+        k = (int)(t % n2);
+    } else if ((t -= sx) < sy) {
+        i = inside((int)(t / ((long)b1 * n2)), G);
+        j = near_face((int)((t / n2) % b1), n1, G);
+        k = (int)(t % n2);
+    } else if ((t -= sy) < sz) {
+        i = inside((int)(t / ((long)(n1 - b1) * b2)), G);
+        j = inside((int)((t / b2) % (n1 - b1)), G);
+        k = near_face((int)(t % b2), n2, G);
+    } else {
+        return;
+    }
+    int li[MAX_IMAGES], lj[MAX_IMAGES], lk[MAX_IMAGES];
+    const int mi = fold_list(i, n0, G, li), mj = fold_list(j, n1, G, lj), mk = fold_list(k, n2, G, lk);
+    for (int comp = PK_JX; comp <= PK_JZ; comp++) {
+        T* f = g.f[comp];
+        T acc_z = T(0);
+        for (int c = 0; c < mk; c++) {
+            T acc_y = T(0);
+            for (int b = 0; b < mj; b++) {
+                T acc_x = T(0);
+                for (int a = 0; a < mi; a++) {
+                    const T x = f[((long)li[a] * g.s1 + lj[b]) * g.s2 + lk[c]];
+                    acc_x = a == 0 ? x : acc_x + x;
+                }
+                acc_y = b == 0 ? acc_x : acc_y + acc_x;
+            }
+            acc_z = c == 0 ? acc_y : acc_z + acc_y;
+        }
+        f[((long)(i + G) * g.s1 + (j + G)) * g.s2 + (k + G)] = acc_z;
+    }
+}
+
 // Periodic fill: every ghost cell takes the value of its wrapped physical image.  The ghost
 // cells are enumerated as x slabs (full y,z), then y slabs (physical x, full z), then z slabs
 // (physical x,y); the closed-form image equals the dimension-by-dimension copy.
@@ -450,6 +520,27 @@ int yee_periodic(pk_grid* g, double dt, bool e_update)
     return PK_OK;
 }
 }  // namespace
+
+int pk_field_fold_fill_j(pk_grid* g, bool fill)
+{
+    const int G = g->ghost;
+    if (G < 1 || G > PK_MAX_PERIODIC_GHOST) return pk_field_ghost(g, fill ? PK_FOLD_J | PK_FILL_J : PK_FOLD_J);
+    const int n0 = g->s[0] - 2 * G, n1 = g->s[1] - 2 * G, n2 = g->s[2] - 2 * G;
+    if (n0 < 1 || n1 < 1 || n2 < 1) return pk_fail(g->ctx, PK_E_ARG, "pk_ghost: no physical cells");
+    const int b0 = n0 <= 2 * G ? n0 : 2 * G, b1 = n1 <= 2 * G ? n1 : 2 * G, b2 = n2 <= 2 * G ? n2 : 2 * G;
+    const long n = (long)b0 * n1 * n2 + (long)(n0 - b0) * b1 * n2 + (long)(n0 - b0) * (n1 - b1) * b2;
+    {
+        pk_ktimer tm(g->ctx, "ghost");
+        if (g->prec == PK_F64)
+            hipLaunchKernelGGL(k_fold_j_gather<double>, dim3(nblk(n, 256)), dim3(256), 0, g->ctx->stream,
+                               grid_view<double>(g), n0, n1, n2);
+        else
+            hipLaunchKernelGGL(k_fold_j_gather<float>, dim3(nblk(n, 256)), dim3(256), 0, g->ctx->stream,
+                               grid_view<float>(g), n0, n1, n2);
+        PK_HIP(g->ctx, hipGetLastError());
+    }
+    return fill ? pk_field_ghost(g, PK_FILL_J) : PK_OK;
+}
 
 int pk_field_yee_periodic(pk_grid* g, double dt, bool e_update)
 {

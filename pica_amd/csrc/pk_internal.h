@@ -280,6 +280,9 @@ int pk_field_ghost(pk_grid* g, int ops);
 // fill of that field's ghosts, in one launch over the physical cells
 #define PK_MAX_PERIODIC_GHOST 2
 int pk_field_yee_periodic(pk_grid* g, double dt, bool e_update);
+// the periodic step's J fold (physical cells bit-identical to PK_FOLD_J, in one launch; the ghosts
+// then hold stale values), followed by PK_FILL_J when fill
+int pk_field_fold_fill_j(pk_grid* g, bool fill);
 int pk_particles_launch(pk_ens* e, pk_grid* g, double dt, int shape, int dep);
 int pk_ens_insert_launch(pk_ens* e);
 int pk_ens_rebuild(pk_ens* e, int source);   // source: 0 = main+outbox, 1 = staging (upload)

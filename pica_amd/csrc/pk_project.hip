@@ -427,7 +427,7 @@ int pk_step_denoised(pk_ens* e, pk_grid* g, pk_dn* d, double dt, int shape, int 
     if ((r = pk_particle_step(e, g, dt, shape, dep))) return r;
     // Esirkepov: the saved divergence, the denoiser and the residual read J's physical cells only,
     // so its ghosts are filled once, after the projection
-    if ((r = pk_field_ghost(g, dep == PK_ESIRKEPOV ? PK_FOLD_J : PK_FOLD_J | PK_FILL_J))) return r;
+    if ((r = pk_field_fold_fill_j(g, dep != PK_ESIRKEPOV))) return r;
     if (dep == PK_ESIRKEPOV) {
         if ((r = ensure_proj(g, P))) return r;
         if (g->prec == PK_F64)
