@@ -29,8 +29,6 @@ __global__ void k_batch_clear(int* halt)
     if (threadIdx.x < PK_HALT_N) halt[threadIdx.x] = 0;
 }
 
-constexpr unsigned MASK_E = (1u << PK_EX) | (1u << PK_EY) | (1u << PK_EZ);
-constexpr unsigned MASK_B = (1u << PK_BX) | (1u << PK_BY) | (1u << PK_BZ);
 constexpr unsigned MASK_J = (1u << PK_JX) | (1u << PK_JY) | (1u << PK_JZ);
 
 // (ensemble, supercell) items: ensemble by ensemble in the XCD-aware supercell order of the
@@ -80,15 +78,13 @@ int enqueue_step(pk_batch* b)
     if ((r = pk_batch_fields_launch(b, PK_BOP_ZERO_J, 0, 0))) return r;
     r = b->prec == PK_F64 ? pk_batch_f64_launch(b, b->dep) : pk_batch_split_launch(b, b->dep);
     if (r) return r;
-    for (int d = 0; d < 3; d++)
-        if ((r = pk_batch_fields_launch(b, PK_BOP_FOLD, MASK_J, d))) return r;
+    // the periodic step's field forms: J fold gathered per cell, then its fill; Yee updates that
+    // write their own ghost images (pk_step's sequence, bit for bit)
+    if ((r = pk_batch_fields_launch(b, PK_BOP_FOLD_J_P, 0, 0))) return r;
     if ((r = pk_batch_fields_launch(b, PK_BOP_FILL, MASK_J, 0))) return r;
-    if ((r = pk_batch_fields_launch(b, PK_BOP_YEE_B, 0, 0))) return r;
-    if ((r = pk_batch_fields_launch(b, PK_BOP_FILL, MASK_B, 0))) return r;
-    if ((r = pk_batch_fields_launch(b, PK_BOP_YEE_E, 0, 0))) return r;
-    if ((r = pk_batch_fields_launch(b, PK_BOP_FILL, MASK_E, 0))) return r;
-    if ((r = pk_batch_fields_launch(b, PK_BOP_YEE_B, 0, 0))) return r;
-    if ((r = pk_batch_fields_launch(b, PK_BOP_FILL, MASK_B, 0))) return r;
+    if ((r = pk_batch_fields_launch(b, PK_BOP_YEE_B_P, 0, 0))) return r;
+    if ((r = pk_batch_fields_launch(b, PK_BOP_YEE_E_P, 0, 0))) return r;
+    if ((r = pk_batch_fields_launch(b, PK_BOP_YEE_B_P, 0, 0))) return r;
     if ((r = pk_batch_insert_launch(b))) return r;
     hipLaunchKernelGGL(k_batch_step_done, dim3(1), dim3(64), 0, b->ctx->stream, b->halt);
     PK_HIP(b->ctx, hipGetLastError());

@@ -105,9 +105,8 @@ __device__ __forceinline__ void write_images(const GridView<T>& g, int n0, int n
 }
 
 template <class T>
-__global__ void __launch_bounds__(256) k_yee_e_periodic(GridView<T> g, YeeCoef<T> k, int n0, int n1, int n2)
+__device__ __forceinline__ void yee_e_periodic_at(const GridView<T>& g, const YeeCoef<T>& k, int n0, int n1, int n2, long v)
 {
-    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= (long)n0 * n1 * n2) return;
     const int i = (int)(v / ((long)n1 * n2)), j = (int)((v / n2) % n1), kz = (int)(v % n2);
     const int s2 = g.s2;
@@ -123,9 +122,8 @@ __global__ void __launch_bounds__(256) k_yee_e_periodic(GridView<T> g, YeeCoef<T
 }
 
 template <class T>
-__global__ void __launch_bounds__(256) k_yee_b_periodic(GridView<T> g, YeeCoef<T> k, int n0, int n1, int n2)
+__device__ __forceinline__ void yee_b_periodic_at(const GridView<T>& g, const YeeCoef<T>& k, int n0, int n1, int n2, long v)
 {
-    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= (long)n0 * n1 * n2) return;
     const int i = (int)(v / ((long)n1 * n2)), j = (int)((v / n2) % n1), kz = (int)(v % n2);
     const int s2 = g.s2;
@@ -138,6 +136,18 @@ __global__ void __launch_bounds__(256) k_yee_b_periodic(GridView<T> g, YeeCoef<T
     const T by = g.f[PK_BY][c] + (k.kx * (ez[c] - ez[c - s12]) - k.kz * (ex[c] - ex[c - 1]));
     const T bz = g.f[PK_BZ][c] + (k.ky * (ex[c] - ex[c - s2]) - k.kx * (ey[c] - ey[c - s12]));
     write_images(g, n0, n1, n2, i, j, kz, PK_BX, bx, by, bz);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_yee_e_periodic(GridView<T> g, YeeCoef<T> k, int n0, int n1, int n2)
+{
+    yee_e_periodic_at(g, k, n0, n1, n2, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_yee_b_periodic(GridView<T> g, YeeCoef<T> k, int n0, int n1, int n2)
+{
+    yee_b_periodic_at(g, k, n0, n1, n2, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ---- the periodic step's J fold in one launch: every physical cell gathers the ghost cells that
@@ -166,13 +176,19 @@ __device__ __forceinline__ int fold_list(int p, int n, int G, int out[MAX_IMAGES
 __device__ __forceinline__ int near_face(int q, int n, int G) { return n <= 2 * G || q < G ? q : n - 2 * G + q; }
 __device__ __forceinline__ int inside(int r, int G) { return G + r; }   // (only when n > 2G)
 
+// cells near a face of an n0 x n1 x n2 grid with G ghost layers
+__host__ __device__ __forceinline__ long near_face_cells(int n0, int n1, int n2, int G)
+{
+    const int b0 = n0 <= 2 * G ? n0 : 2 * G, b1 = n1 <= 2 * G ? n1 : 2 * G, b2 = n2 <= 2 * G ? n2 : 2 * G;
+    return (long)b0 * n1 * n2 + (long)(n0 - b0) * b1 * n2 + (long)(n0 - b0) * (n1 - b1) * b2;
+}
+
 template <class T>
-__global__ void __launch_bounds__(256) k_fold_j_gather(GridView<T> g, int n0, int n1, int n2)
+__device__ __forceinline__ void fold_j_at(const GridView<T>& g, int n0, int n1, int n2, long t)
 {
     const int G = g.G;
     const int b0 = n0 <= 2 * G ? n0 : 2 * G, b1 = n1 <= 2 * G ? n1 : 2 * G, b2 = n2 <= 2 * G ? n2 : 2 * G;
     const long sx = (long)b0 * n1 * n2, sy = (long)(n0 - b0) * b1 * n2, sz = (long)(n0 - b0) * (n1 - b1) * b2;
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     int i, j, k;
     if (t < sx) {
         i = near_face((int)(t / ((long)n1 * n2)), n0, G);
@@ -208,6 +224,12 @@ __global__ void __launch_bounds__(256) k_fold_j_gather(GridView<T> g, int n0, in
         }
         f[((long)(i + G) * g.s1 + (j + G)) * g.s2 + (k + G)] = acc_z;
     }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_fold_j_gather(GridView<T> g, int n0, int n1, int n2)
+{
+    fold_j_at(g, n0, n1, n2, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // Periodic fill: every ghost cell takes the value of its wrapped physical image.  The ghost
@@ -376,6 +398,7 @@ struct FieldJob {
     GridView<T> g;
     YeeCoef<T> ke, kb;      // updateE(dt), updateB(dt / 2)
     long ncell, nx, ny, nz;
+    int n0, n1, n2;         // physical cells
 };
 
 // ensemble of block b: the last i with start[i] <= b (empty ranges are skipped)
@@ -400,14 +423,14 @@ __global__ void __launch_bounds__(256) k_batch_field(const FieldJob<T>* __restri
     if (OP == PK_BOP_ZERO_J) {
         if (t < J.ncell)
             for (int c = PK_JX; c <= PK_JZ; c++) J.g.f[c][t] = T(0);
-    } else if (OP == PK_BOP_YEE_E) {
-        if (t < J.ncell) yee_e_at(J.g, J.ke, t);
-    } else if (OP == PK_BOP_YEE_B) {
-        if (t < J.ncell) yee_b_at(J.g, J.kb, t);
     } else if (OP == PK_BOP_FILL) {
         ghost_fill_at(J.g, mask, J.nx, J.ny, J.nz, t);
+    } else if (OP == PK_BOP_YEE_E_P) {
+        yee_e_periodic_at(J.g, J.ke, J.n0, J.n1, J.n2, t);
+    } else if (OP == PK_BOP_YEE_B_P) {
+        yee_b_periodic_at(J.g, J.kb, J.n0, J.n1, J.n2, t);
     } else {
-        ghost_fold_at(J.g, mask, d, t);
+        fold_j_at(J.g, J.n0, J.n1, J.n2, t);
     }
 }
 
@@ -424,12 +447,17 @@ int batch_prepare(pk_batch* b, double dt)
         J.ke = yee_coef<T>(g, dt, true);
         J.kb = yee_coef<T>(g, dt * 0.5, false);
         const int G = g->ghost;
+        if (G < 1 || G > PK_MAX_PERIODIC_GHOST)
+            return pk_fail(b->ctx, PK_E_ARG, "batch: grids need 1 or 2 periodic ghost layers");
         J.ncell = (long)g->n;
         J.nx = (long)2 * G * g->s[1] * g->s[2];
         J.ny = (long)(g->s[0] - 2 * G) * 2 * G * g->s[2];
         J.nz = (long)(g->s[0] - 2 * G) * (g->s[1] - 2 * G) * 2 * G;
-        const long work[PK_BSTAGES] = {J.ncell, J.nx + J.ny + J.nz, (long)g->s[1] * g->s[2], (long)g->s[2] * g->s[0],
-                                       (long)g->s[0] * g->s[1]};
+        J.n0 = g->s[0] - 2 * G;
+        J.n1 = g->s[1] - 2 * G;
+        J.n2 = g->s[2] - 2 * G;
+        const long work[PK_BSTAGES] = {J.ncell, J.nx + J.ny + J.nz, (long)J.n0 * J.n1 * J.n2,
+                                       near_face_cells(J.n0, J.n1, J.n2, G)};
         for (int st = 0; st < PK_BSTAGES; st++)
             start[(size_t)st * (n + 1) + i + 1] = start[(size_t)st * (n + 1) + i] + (long)nblk(work[st], 256);
     }
@@ -443,7 +471,10 @@ template <class T>
 int batch_launch(pk_batch* b, int op, unsigned mask, int d)
 {
     const int n = b->n();
-    const int st = op == PK_BOP_FILL ? PK_BSTAGE_FILL : (op == PK_BOP_FOLD ? PK_BSTAGE_FOLD + d : PK_BSTAGE_CELLS);
+    const int st = op == PK_BOP_FILL                             ? PK_BSTAGE_FILL
+                   : op == PK_BOP_YEE_E_P || op == PK_BOP_YEE_B_P ? PK_BSTAGE_PHYS
+                   : op == PK_BOP_FOLD_J_P                       ? PK_BSTAGE_FACE
+                                                                 : PK_BSTAGE_CELLS;
     const long nb = b->fblocks[st];
     if (nb == 0) return PK_OK;
     const FieldJob<T>* jobs = (const FieldJob<T>*)b->fjobs;
@@ -453,17 +484,17 @@ int batch_launch(pk_batch* b, int op, unsigned mask, int d)
     case PK_BOP_ZERO_J:
         hipLaunchKernelGGL((k_batch_field<T, PK_BOP_ZERO_J>), dim3(nb), dim3(256), 0, s, jobs, start, n, mask, d, b->halt);
         break;
-    case PK_BOP_YEE_E:
-        hipLaunchKernelGGL((k_batch_field<T, PK_BOP_YEE_E>), dim3(nb), dim3(256), 0, s, jobs, start, n, mask, d, b->halt);
-        break;
-    case PK_BOP_YEE_B:
-        hipLaunchKernelGGL((k_batch_field<T, PK_BOP_YEE_B>), dim3(nb), dim3(256), 0, s, jobs, start, n, mask, d, b->halt);
-        break;
     case PK_BOP_FILL:
         hipLaunchKernelGGL((k_batch_field<T, PK_BOP_FILL>), dim3(nb), dim3(256), 0, s, jobs, start, n, mask, d, b->halt);
         break;
+    case PK_BOP_YEE_E_P:
+        hipLaunchKernelGGL((k_batch_field<T, PK_BOP_YEE_E_P>), dim3(nb), dim3(256), 0, s, jobs, start, n, mask, d, b->halt);
+        break;
+    case PK_BOP_YEE_B_P:
+        hipLaunchKernelGGL((k_batch_field<T, PK_BOP_YEE_B_P>), dim3(nb), dim3(256), 0, s, jobs, start, n, mask, d, b->halt);
+        break;
     default:
-        hipLaunchKernelGGL((k_batch_field<T, PK_BOP_FOLD>), dim3(nb), dim3(256), 0, s, jobs, start, n, mask, d, b->halt);
+        hipLaunchKernelGGL((k_batch_field<T, PK_BOP_FOLD_J_P>), dim3(nb), dim3(256), 0, s, jobs, start, n, mask, d, b->halt);
         break;
     }
     PK_HIP(b->ctx, hipGetLastError());
@@ -527,8 +558,7 @@ int pk_field_fold_fill_j(pk_grid* g, bool fill)
     if (G < 1 || G > PK_MAX_PERIODIC_GHOST) return pk_field_ghost(g, fill ? PK_FOLD_J | PK_FILL_J : PK_FOLD_J);
     const int n0 = g->s[0] - 2 * G, n1 = g->s[1] - 2 * G, n2 = g->s[2] - 2 * G;
     if (n0 < 1 || n1 < 1 || n2 < 1) return pk_fail(g->ctx, PK_E_ARG, "pk_ghost: no physical cells");
-    const int b0 = n0 <= 2 * G ? n0 : 2 * G, b1 = n1 <= 2 * G ? n1 : 2 * G, b2 = n2 <= 2 * G ? n2 : 2 * G;
-    const long n = (long)b0 * n1 * n2 + (long)(n0 - b0) * b1 * n2 + (long)(n0 - b0) * (n1 - b1) * b2;
+    const long n = near_face_cells(n0, n1, n2, G);
     {
         pk_ktimer tm(g->ctx, "ghost");
         if (g->prec == PK_F64)

@@ -208,8 +208,12 @@ __device__ __forceinline__ long pk_xcd_supercell(long b, int chunk)
 // ---- many ensembles stepped together (pk_batch_*) ----
 // Work of one stage for every ensemble goes into one launch: the particle and insert kernels
 // walk a list of (ensemble, supercell) items, the field kernels a per-ensemble block prefix.
-enum { PK_BSTAGE_CELLS = 0, PK_BSTAGE_FILL = 1, PK_BSTAGE_FOLD = 2, PK_BSTAGES = 5 };   // FOLD + d
-enum { PK_BOP_ZERO_J = 0, PK_BOP_YEE_E, PK_BOP_YEE_B, PK_BOP_FILL, PK_BOP_FOLD };
+// work ranges of the batched field kernels: all cells, ghost cells, physical cells, physical cells
+// within the ghost width of a face
+enum { PK_BSTAGE_CELLS = 0, PK_BSTAGE_FILL = 1, PK_BSTAGE_PHYS = 2, PK_BSTAGE_FACE = 3, PK_BSTAGES = 4 };
+// the periodic step's field operations: J zero, ghost fill, Yee updates that write their own ghost
+// images, the J fold gathered per cell
+enum { PK_BOP_ZERO_J = 0, PK_BOP_FILL, PK_BOP_YEE_E_P, PK_BOP_YEE_B_P, PK_BOP_FOLD_J_P };
 // device halt words of a batch: every batched kernel returns at once while STOP is set; an insert
 // overflow sets PENDING, which the step's last kernel turns into STOP (else it counts the step in DONE)
 enum { PK_HALT_STOP = 0, PK_HALT_DONE = 1, PK_HALT_PENDING = 2, PK_HALT_N = 4 };
@@ -232,7 +236,7 @@ struct pk_batch {
     void* ijobs = nullptr;     // insert-kernel arguments
     void* fjobs = nullptr;     // field-kernel arguments
     long* fstart = nullptr;    // PK_BSTAGES x (n + 1) block prefixes of the field stages
-    long fblocks[PK_BSTAGES] = {0, 0, 0, 0, 0};
+    long fblocks[PK_BSTAGES] = {};
     // particle-kernel items of each form factor (PK_CIC, PK_TSC), one launch per non-empty class,
     // and the dynamic LDS of each class (its largest ensemble)
     int2* citems[2] = {nullptr, nullptr};
