@@ -171,7 +171,7 @@ def run_gpu(args, prec):
 def batch_specs(n=64, seed=3, edges=(32, 48, 64, 96, 128), ppcs=(4, 8, 16, 32)):
     """the batching form's plasmas: per plasma a cubic edge and a ppc drawn uniformly from the
     sets, a drift speed uniform in [0, 0.2]c (as u = v / sqrt(1 - v^2)), CIC or TSC with equal odds
-    and its own seed (the draw of tests/helpers.config3_specs)"""
+    and its own seed (the draw of tests/helpers.uneven_plasma_specs)"""
     rng = np.random.default_rng(seed)
     out = []
     for i in range(n):

@@ -20,7 +20,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 import oracle as O  # noqa: E402
 
-CONFIG0_SEED = 20240601
+THERMAL32_SEED = 20240601
 SMALL_SEED = 1234
 
 
@@ -36,7 +36,7 @@ def _dp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
-def config0_inputs(seed=CONFIG0_SEED):
+def thermal32_inputs(seed=THERMAL32_SEED):
     """32^3 thermal plasma, 8 ppc, sigma_u = 0.01, omega_p dt = 0.1, Courant 0.5 (CGS, h = 1 cm)."""
     rng = np.random.default_rng(seed)
     ncell, ppc = (32, 32, 32), 8
@@ -162,9 +162,9 @@ def main():
                run_f_out=gr.f)
     np.savez_compressed(os.path.join(HERE, "pica_units.npz"), **out)
 
-    # --- config 0: 32^3, 8 ppc, CIC, direct, 100 steps: a summary (inputs regenerate from the seed)
-    ncell, dt, x, u, w = config0_inputs()
-    summ = {"seed": np.array([CONFIG0_SEED])}
+    # --- the thermal 32^3 plasma, 8 ppc, CIC, direct, 100 steps: a summary (inputs regenerate from the seed)
+    ncell, dt, x, u, w = thermal32_inputs()
+    summ = {"seed": np.array([THERMAL32_SEED])}
     for nsteps in (50, 100):
         xr, ur, wr, offr, gr = O.ref_run_cic_direct(ncell, (1.0, 1.0, 1.0), (4, 4, 4), O.E_CHARGE, O.E_MASS, dt,
                                                     nsteps, x, u, w)
@@ -176,8 +176,8 @@ def main():
         summ.update({"off_%d" % nsteps: offr, "counts_%d" % nsteps: counts, "sample_x_%d" % nsteps: xr[sample],
                      "sample_u_%d" % nsteps: ur[sample], "sumsq_%d" % nsteps: (phys ** 2).sum(axis=(1, 2, 3)),
                      "plane_%d" % nsteps: phys[:, 5].copy()})
-    np.savez_compressed(os.path.join(HERE, "config0_summary.npz"), **summ)
-    print("wrote", os.path.join(HERE, "pica_units.npz"), "and config0_summary.npz")
+    np.savez_compressed(os.path.join(HERE, "thermal32_cic_direct_summary.npz"), **summ)
+    print("wrote", os.path.join(HERE, "pica_units.npz"), "and thermal32_cic_direct_summary.npz")
 
 
 if __name__ == "__main__":

@@ -49,7 +49,7 @@ def random_fields(rng, grid, e_scale, b_scale):
         O.ghost_fill(grid.f[c])
 
 
-def config3_specs(n=64, seed=3, edges=(32, 48, 64, 96, 128), ppcs=(4, 8, 16, 32)):
+def uneven_plasma_specs(n=64, seed=3, edges=(32, 48, 64, 96, 128), ppcs=(4, 8, 16, 32)):
     """independent plasmas of the batching form: per instance a cubic grid edge and a ppc drawn
     uniformly from the given sets, a drift speed uniform in [0, 0.2]c (returned as u = v / sqrt(1 -
     v^2)), CIC or TSC with equal odds, and its own seed"""

@@ -95,14 +95,14 @@ def test_small_periodic_run():
 
 
 @pytest.mark.parametrize("nsteps", [50, 100])
-def test_config0_summary(nsteps):
-    """config 0 (32^3, 8 ppc, CIC, direct, FP64): membership, per-cell counts, sampled particles
+def test_thermal32_summary(nsteps):
+    """the thermal 32^3 plasma (8 ppc, CIC, direct, FP64): membership, per-cell counts, sampled particles
     and field planes after 50 and 100 steps equal pica's"""
     import sys
     sys.path.insert(0, os.path.join(HERE, "golden"))
-    from make_golden import config0_inputs
-    S = np.load(os.path.join(HERE, "golden", "config0_summary.npz"))
-    ncell, dt, x, u, w = config0_inputs(int(S["seed"][0]))
+    from make_golden import thermal32_inputs
+    S = np.load(os.path.join(HERE, "golden", "thermal32_cic_direct_summary.npz"))
+    ncell, dt, x, u, w = thermal32_inputs(int(S["seed"][0]))
     ens = O.Ensemble(ncell, (4, 4, 4), (1.0, 1.0, 1.0), x, u, w)
     g = O.Grid.periodic(ncell)
     for _ in range(nsteps):

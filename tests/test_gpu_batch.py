@@ -301,10 +301,10 @@ def test_run_captured_matches_run(ctx, prec, shape, dep, period, slack):
         _compare(prec, _state(e, g), _state(re_, rg), "ensemble %s after 0 steps" % (spec,))
 
 
-def test_run_captured_config0_timing(ctx):
-    """the 32^3 x 8 ppc plasma of config 0 (CIC, direct, FP64), 100 steps each way from the same
+def test_run_captured_timing_32cube(ctx):
+    """a thermal 32^3 x 8 ppc plasma (CIC, direct, FP64), 100 steps each way from the same
     particles: the captured run matches pk_run and is timed beside it"""
-    _say("run captured timing, config 0 shape")
+    _say("run captured timing, 32^3 x 8 ppc")
     import time
     dt = courant_dt()
     spec = ((32, 32, 32), 8, 21)
@@ -317,6 +317,6 @@ def test_run_captured_config0_timing(ctx):
         fn(e, g, dt, 100, P.CIC, P.DIRECT)
         ctx.sync()
         runs.append((time.perf_counter() - t0, _state(e, g)))
-    print("config-0 shape, 100 steps: pk_run %.2f ms per step, pk_run_captured %.2f ms per step"
+    print("32^3 x 8 ppc, 100 steps: pk_run %.2f ms per step, pk_run_captured %.2f ms per step"
           % (10.0 * runs[0][0], 10.0 * runs[1][0]), flush=True)
-    _compare(P.F64, runs[1][1], runs[0][1], "config-0 shape")
+    _compare(P.F64, runs[1][1], runs[0][1], "32^3 x 8 ppc")

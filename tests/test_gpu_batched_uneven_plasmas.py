@@ -1,4 +1,4 @@
-"""The batching form at config 3's shape: independent periodic plasmas with cubic grid edges from
+"""The batching form on 64 uneven plasmas: independent periodic plasmas with cubic grid edges from
 {32, 48, 64, 96, 128}, ppc from {4, 8, 16, 32}, drift speeds uniform in [0, 0.2]c, CIC or TSC per
 instance (PK_SHAPE_PER_INSTANCE), Esirkepov deposition, stepped as one batch.
 
@@ -15,7 +15,7 @@ import pytest
 
 import oracle as O
 import pica_amd as P
-from helpers import config3_specs, courant_dt, plasma, rel_err
+from helpers import uneven_plasma_specs, courant_dt, plasma, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -34,10 +34,10 @@ def _load(ctx, prec, spec, ncps=(4, 4, 4)):
     return e, g
 
 
-def test_config3_fp64_instances_match_the_oracle(ctx):
-    specs = [s for s in config3_specs() if s["edge"] * s["edge"] * s["edge"] * s["ppc"] <= 48 ** 3 * 8][:4]
+def test_uneven_plasmas_fp64_instances_match_the_oracle(ctx):
+    specs = [s for s in uneven_plasma_specs() if s["edge"] * s["edge"] * s["edge"] * s["ppc"] <= 48 ** 3 * 8][:4]
     assert {s["shape"] for s in specs} == {P.CIC, P.TSC}, "the draw should hold both form factors"
-    _say("config-3 FP64 vs oracle: %s" % [(s["edge"], s["ppc"], round(s["drift"], 3), s["shape"]) for s in specs])
+    _say("uneven plasmas FP64 vs oracle: %s" % [(s["edge"], s["ppc"], round(s["drift"], 3), s["shape"]) for s in specs])
     dt = courant_dt()
     pairs = [_load(ctx, P.F64, s) for s in specs]
     oracles = []
@@ -64,9 +64,9 @@ def test_config3_fp64_instances_match_the_oracle(ctx):
     b.close()
 
 
-def test_config3_fp32_workload_100_steps(ctx):
-    specs = config3_specs()
-    _say("config-3 FP32 workload, %d instances, 100 steps" % len(specs))
+def test_uneven_plasmas_fp32_workload_100_steps(ctx):
+    specs = uneven_plasma_specs()
+    _say("uneven plasmas FP32 workload, %d instances, 100 steps" % len(specs))
     dt = courant_dt()
     pairs = [_load(ctx, P.F32, s) for s in specs]
     n0 = [e.size() for e, _ in pairs]
@@ -93,7 +93,7 @@ def test_config3_fp32_workload_100_steps(ctx):
         worst["charge"] = max(worst["charge"], q)
         worst["gauss"] = max(worst["gauss"], gd)
         energies32.append(d1[P.DIAG_FIELD_ENERGY] + d1[P.DIAG_KINETIC_ENERGY])
-    print("config-3 FP32 after 100 steps: worst charge change %.3g, worst Gauss drift %.3g" %
+    print("uneven plasmas FP32 after 100 steps: worst charge change %.3g, worst Gauss drift %.3g" %
           (worst["charge"], worst["gauss"]), flush=True)
     assert worst["charge"] < 1e-5 and worst["gauss"] < 1e-4
     b.close()
@@ -113,7 +113,7 @@ def test_config3_fp32_workload_100_steps(ctx):
         d = P.diag(e, g, specs[i]["shape"])
         e64 = d[P.DIAG_FIELD_ENERGY] + d[P.DIAG_KINETIC_ENERGY]
         worst_e = max(worst_e, abs(energies32[i] - e64) / e64)
-    print("config-3 FP32 vs FP64 total energy after 100 steps (%d instances): worst %.3g" % (len(sub), worst_e),
+    print("uneven plasmas FP32 vs FP64 total energy after 100 steps (%d instances): worst %.3g" % (len(sub), worst_e),
           flush=True)
     assert worst_e < 1e-5
     b64.close()

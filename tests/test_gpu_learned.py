@@ -378,7 +378,7 @@ def test_denoised_step_filtered_charge_continuity(ctx, dn, prec, tol):
 
 @pytest.mark.parametrize("prec,tol", [(P.F64, 1e-11), (P.F32, 2e-4)])
 def test_learned_form_at_128cube(ctx, dn, prec, tol):
-    """config 4's grid (128^3, 4 ppc, TSC): the continuity projection of a direct-deposited current
+    """the learned form's grid (128^3, 4 ppc, TSC): the continuity projection of a direct-deposited current
     equals the oracle's FFT solve, and 5 denoised steps (filtered charge) keep Gauss's law for
     the filtered charge"""
     _say("learned form 128^3 prec=%d" % prec)

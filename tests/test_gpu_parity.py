@@ -140,9 +140,9 @@ def test_twenty_steps_parity(ctx, shape, dep):
         assert rel_err(f[c], og.f[c]) < 1e-12, "component %d" % c
 
 
-def test_config0_100_steps(ctx):
+def test_thermal32_cic_direct_100_steps(ctx):
     """32^3 periodic thermal plasma, 8 ppc, CIC, direct deposition, 100 steps, FP64."""
-    _say("config-0 100 steps")
+    _say("thermal 32^3 x 8 ppc CIC/direct 100 steps")
     ncell, ncps, ppc = (32, 32, 32), (4, 4, 4), 8
     dt = courant_dt()
     n0, w0 = plasma(dt, ppc)
@@ -166,16 +166,16 @@ def test_config0_100_steps(ctx):
     ex, eu = rel_err(gx, oe.positions()), rel_err(gu, oe.momenta())
     f = g.get_all()
     ef = [rel_err(f[c], og.f[c]) for c in range(9)]
-    print("config-0 rel errors: x %.3g u %.3g fields %s" % (ex, eu, " ".join("%.3g" % v for v in ef)), flush=True)
+    print("thermal 32^3 CIC/direct rel errors: x %.3g u %.3g fields %s" % (ex, eu, " ".join("%.3g" % v for v in ef)), flush=True)
     assert ex < 1e-12 and eu < 1e-12
     assert max(ef) < 1e-12
 
 
-def test_config1_shape_tsc_esirkepov_100_steps(ctx):
-    """Config 1's physics on a 32^3 grid: 16 ppc, Maxwellian 0.01 plus a two-stream +-0.1c drift
+def test_two_stream_32cube_tsc_esirkepov_100_steps(ctx):
+    """The headline plasma's physics on a 32^3 grid: 16 ppc, Maxwellian 0.01 plus a two-stream +-0.1c drift
     in x on alternate particles, TSC, Esirkepov, 4^3 supercells re-sorted every step, 100 FP64
     steps.  Membership, cell indices and per-cell counts bit-exact; x, u, E, B, J within 1e-12."""
-    _say("config-1 shape 32^3 x 16 ppc TSC/Esirkepov 100 steps")
+    _say("two-stream 32^3 x 16 ppc TSC/Esirkepov 100 steps")
     ncell, ncps, ppc = (32, 32, 32), (4, 4, 4), 16
     dt = courant_dt()
     n0, w0 = plasma(dt, ppc)
@@ -201,7 +201,7 @@ def test_config1_shape_tsc_esirkepov_100_steps(ctx):
     ex, eu = rel_err(gx, oe.positions()), rel_err(gu, oe.momenta())
     f = g.get_all()
     ef = [rel_err(f[c], og.f[c]) for c in range(9)]
-    print("config-1 shape rel errors: x %.3g u %.3g fields %s" % (ex, eu, " ".join("%.3g" % v for v in ef)),
+    print("two-stream 32^3 TSC/Esirkepov rel errors: x %.3g u %.3g fields %s" % (ex, eu, " ".join("%.3g" % v for v in ef)),
           flush=True)
     assert ex < 1e-12 and eu < 1e-12
     assert max(ef) < 1e-12
