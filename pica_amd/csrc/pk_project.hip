@@ -147,19 +147,43 @@ __global__ void __launch_bounds__(256) k_fft(Cx<R>* __restrict__ w, int logn, lo
         }
     }
     __syncthreads();
-    const int nb = pb << (logn - 1);
-    for (int lh = 0; lh < logn; lh++) {               // half = 2^lh, len = 2 half
+    // radix-2 stages two at a time: a thread takes the four elements i, i + h, i + 2h, i + 3h of a
+    // block of 4h and applies stage lh (pairs at distance h) and stage lh + 1 (pairs at distance 2h)
+    // in registers, with the radix-2 stages' own operations and twiddles (so the same results), one
+    // LDS round trip and one barrier per two stages; an odd last stage runs alone
+    const auto mul = [](Cx<R> x, Cx<R> w) { return Cx<R>{x.x * w.x - x.y * w.y, x.x * w.y + x.y * w.x}; };
+    int lh = 0;
+    for (; lh + 1 < logn; lh += 2) {
+        const int h = 1 << lh;
+        const int nq = pb << (logn - 2);
+        for (int t = threadIdx.x; t < nq; t += blockDim.x) {
+            const int pl = t >> (logn - 2), tt = t & ((n >> 2) - 1);
+            const int grp = tt >> lh, pos = tt & (h - 1);
+            const int i0 = pl * row + (grp << (lh + 2)) + pos;
+            const Cx<R> x0 = a[i0], x1 = a[i0 + h], x2 = a[i0 + 2 * h], x3 = a[i0 + 3 * h];
+            const Cx<R> w1 = twl[pos << (logn - 1 - lh)];
+            const Cx<R> v1 = mul(x1, w1), v3 = mul(x3, w1);
+            const Cx<R> y0{x0.x + v1.x, x0.y + v1.y}, y1{x0.x - v1.x, x0.y - v1.y};
+            const Cx<R> y2{x2.x + v3.x, x2.y + v3.y}, y3{x2.x - v3.x, x2.y - v3.y};
+            const Cx<R> w2 = twl[pos << (logn - 2 - lh)], w3 = twl[(pos + h) << (logn - 2 - lh)];
+            const Cx<R> v2 = mul(y2, w2), u3 = mul(y3, w3);
+            a[i0] = Cx<R>{y0.x + v2.x, y0.y + v2.y};
+            a[i0 + 2 * h] = Cx<R>{y0.x - v2.x, y0.y - v2.y};
+            a[i0 + h] = Cx<R>{y1.x + u3.x, y1.y + u3.y};
+            a[i0 + 3 * h] = Cx<R>{y1.x - u3.x, y1.y - u3.y};
+        }
+        __syncthreads();
+    }
+    if (lh < logn) {                                   // half = 2^lh, len = 2 half
         const int half = 1 << lh;
+        const int nb = pb << (logn - 1);
         for (int t = threadIdx.x; t < nb; t += blockDim.x) {
             const int pl = t >> (logn - 1), tt = t & ((n >> 1) - 1);
             const int grp = tt >> lh, pos = tt & (half - 1);
             const int i0 = pl * row + (grp << (lh + 1)) + pos, i1 = i0 + half;
-            const Cx<R> wt = twl[pos << (logn - 1 - lh)];
-            const R wx = wt.x, wy = wt.y;
-            const Cx<R> u = a[i0], x = a[i1];
-            const R vx = x.x * wx - x.y * wy, vy = x.x * wy + x.y * wx;
-            a[i0] = Cx<R>{u.x + vx, u.y + vy};
-            a[i1] = Cx<R>{u.x - vx, u.y - vy};
+            const Cx<R> u = a[i0], v = mul(a[i1], twl[pos << (logn - 1 - lh)]);
+            a[i0] = Cx<R>{u.x + v.x, u.y + v.y};
+            a[i1] = Cx<R>{u.x - v.x, u.y - v.y};
         }
         __syncthreads();
     }
