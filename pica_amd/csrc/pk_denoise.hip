@@ -10,8 +10,11 @@
 //   B (32 k x 16 cout):   lane l holds B[8 (l / 16) .. +7][l % 16]  -- pre-packed weights,
 //                         held in registers for the whole kernel
 //   D (16 x 16, FP32):    lane l holds D[4 (l / 16) + j][l % 16], j = 0..3
-// With 16 input channels a K-chunk is 2 taps x 16 channels (14 chunks, the 28th tap zero); with
-// 3 (padded to 4) it is 8 taps x 4 channels (4 chunks).  The input gradient is the same forward
+// With 16 input channels a K-chunk is 2 taps of one x-offset dx x 16 channels: 5 chunks per dx
+// (the 9 (dy, dz) offsets in pairs, the tenth zero), 15 in all.  A wavefront then computes the
+// four x-rows of its brick together: one LDS read of a halo x-plane feeds every row that plane is
+// a tap plane of (dx = plane - row), 30 reads for 60 MFMAs instead of 56 for 56.  With 3 (padded
+// to 4) input channels a chunk is 8 taps x 4 channels (4 chunks).  The input gradient is the same forward
 // kernel with the weights flipped (tap -> 26 - tap) and transposed; the weight gradient is a GEMM
 // with K = voxels over LDS-staged bricks on the same 16x16x32 MFMA, with the bias gradient as one
 // more tile.  Training stacks a mini-batch's samples so that each layer is one launch.
@@ -32,7 +35,8 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int HID = 16;
 constexpr int PK_DN_RADIUS = 3;   // receptive radius of the three 3x3x3 layers
 constexpr int NPARAM = 27 * 3 * HID + HID + 27 * HID * HID + HID + 27 * HID * 3 + 3;   // 9539
-constexpr int WF16 = 14 * 64 * 8, WF4 = 4 * 64 * 8;   // packed B fragments (BF16 values) per layer
+constexpr int KC16 = 15, KC4 = 4;      // K-chunks of a 16- and a 4-channel layer
+constexpr int WF16 = KC16 * 64 * 8, WF4 = KC4 * 64 * 8;   // packed B fragments (BF16 values) per layer
 constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + 27 * 3 * HID, OFF_W2 = OFF_B1 + HID, OFF_B2 = OFF_W2 + 27 * HID * HID,
               OFF_W3 = OFF_B2 + HID, OFF_B3 = OFF_W3 + 27 * HID * 3;
 
@@ -116,7 +120,9 @@ enum { OUT_RELU_BF16 = 0, OUT_F32 = 1, OUT_MASKED_BF16 = 2 };
 // one-voxel periodic halo are staged in LDS (each input voxel leaves L2 about 2.5 times instead
 // of 27), then every wavefront runs 16-voxel z-rows of the brick through the MFMA chain.  A 4 x 4
 // x 16 brick stages 648 voxels for 256 (2 x 8 x 16: 720, and a 10% slower denoiser at 128^3).
-// CIN = 16 (2 taps x 16 channels per K-chunk) or 4 (8 taps x 4 channels per chunk).
+// CIN = 16 (2 taps x 16 channels per K-chunk) or 4 (8 taps x 4 channels per chunk).  Five
+// workgroups per CU (92 VGPRs for the 16-channel form, 104 KB of LDS): at four, the 16-channel
+// layers took 63 / 58 us at 128^3 against 56 / 49 us.
 constexpr int BX = 4, BY = 4, BZ = 16, HX = BX + 2, HY = BY + 2, HZ = BZ + 2;
 constexpr int BRICK_TILES = BX * BY * (BZ / 16);
 static_assert(BRICK_TILES % 8 == 0, "4 wavefronts x 2 rows per pass");
@@ -130,11 +136,11 @@ __device__ __forceinline__ int wrap(int a, int n)
 }
 
 template <int CIN, int MODE>
-__global__ void __launch_bounds__(256) k_conv(Vol V, const uint16_t* __restrict__ in, const uint16_t* __restrict__ wf,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_conv(Vol V, const uint16_t* __restrict__ in, const uint16_t* __restrict__ wf,
                                               const float* __restrict__ bias, int cout, uint16_t* __restrict__ out_b,
                                               float* __restrict__ out_f, const uint16_t* __restrict__ mask_act)
 {
-    constexpr int KC = CIN == 16 ? 14 : 4;
+    constexpr int KC = CIN == 16 ? KC16 : KC4;
     constexpr int UNITS = CIN == 16 ? 2 : 1;              // 16- or 8-byte units per voxel
     extern __shared__ __align__(16) unsigned char smem[];
     uint16_t* halo = (uint16_t*)smem;                     // [HX][HY][HZ][CIN]
@@ -178,32 +184,67 @@ __global__ void __launch_bounds__(256) k_conv(Vol V, const uint16_t* __restrict_
     for (int kc = 0; kc < KC; kc++) B[kc] = *(const bf16x8*)(wf + ((size_t)kc * 64 + lane) * 8);
     const float bco = (bias && m < cout) ? bias[m] : 0.0f;
     __syncthreads();
-    // two z-rows per pass: independent accumulators hide the MFMA dependency latency
-    for (int t0 = wave; t0 < BRICK_TILES; t0 += 8) {
-        int tx[2], ty[2], tz[2];
+    // the bias, activation and store of one 16-voxel z-row (brick row tx, ty, tz)
+    const auto store = [&](const f32x4& acc, int tx, int ty, int tz) {
+        const int co = m;
+        const int gx = x0 + tx, gy = y0 + ty;
+        if (co >= cout || gx >= V.n0 || gy >= V.n1) return;
 #pragma unroll
-        for (int r = 0; r < 2; r++) {
-            const int t = t0 + 4 * r;
-            tx[r] = t / (BY * (BZ / 16));
-            ty[r] = (t / (BZ / 16)) % BY;
-            tz[r] = (t % (BZ / 16)) * 16;
+        for (int j = 0; j < 4; j++) {
+            const int gz = z0 + tz + 4 * q + j;
+            if (gz >= V.n2) continue;
+            const long vo = ((long)gx * V.n1 + gy) * V.n2 + gz;
+            const float zv = acc[j] + bco;
+            if (MODE == OUT_RELU_BF16) out_b[vo * 16 + co] = to_bf16(zv > 0.0f ? zv : 0.0f);
+            else if (MODE == OUT_F32) out_f[(long)co * V.nv + vo] = zv;
+            else out_b[vo * 16 + co] = to_bf16(from_bf16(mask_act[vo * 16 + co]) > 0.0f ? zv : 0.0f);
         }
-        f32x4 acc[2] = {{0.0f, 0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f, 0.0f}};
+    };
+    if constexpr (CIN == 16) {
+        // wavefront w: the z-row at brick y = w for all four x.  Halo x-plane hx is tap plane
+        // dx = hx - r - 1 of row r; chunk 5 (dx + 1) + c holds (dy, dz) offsets 2c, 2c + 1 of it
+        // (offset o at halo (ty + o / 3, m + o % 3)), lanes q / 2 taking one offset each.
+        static_assert(BY * (BZ / 16) == PK_BLOCK / 64 && BZ == 16, "one z-row of each brick y per wavefront");
+        f32x4 acc[BX];
 #pragma unroll
-        for (int kc = 0; kc < KC; kc++) {
+        for (int r = 0; r < BX; r++) acc[r] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int hx = 0; hx < HX; hx++) {
+#pragma unroll
+            for (int c = 0; c < 5; c++) {
+                const int o = 2 * c + q / 2;
+                bf16x8 a = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+                if (o < 9) {
+                    const int h = (hx * HY + wave + o / 3) * HZ + m + o % 3;
+                    a = *(const bf16x8*)(halo + h * 16 + 8 * (q % 2));
+                }
+#pragma unroll
+                for (int r = 0; r < BX; r++) {
+                    const int dxi = hx - r;
+                    if (dxi >= 0 && dxi <= 2)
+                        acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            __builtin_bit_cast(bf16v8, a), __builtin_bit_cast(bf16v8, B[5 * dxi + c]), acc[r], 0, 0, 0);
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < BX; r++) store(acc[r], r, wave, 0);
+    } else {
+        // two z-rows per pass: independent accumulators hide the MFMA dependency latency
+        for (int t0 = wave; t0 < BRICK_TILES; t0 += 8) {
+            int tx[2], ty[2], tz[2];
 #pragma unroll
             for (int r = 0; r < 2; r++) {
-                bf16x8 a;
-                if (CIN == 16) {
-                    const int tap = 2 * kc + q / 2;     // 8 channels of one tap per lane
-                    if (tap < 27) {
-                        const int h =
-                            ((tx[r] + tap / 9) * HY + (ty[r] + (tap / 3) % 3)) * HZ + (tz[r] + m + tap % 3);
-                        a = *(const bf16x8*)(halo + h * 16 + 8 * (q % 2));
-                    } else {
-                        a = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
-                    }
-                } else {
+                const int t = t0 + 4 * r;
+                tx[r] = t / (BY * (BZ / 16));
+                ty[r] = (t / (BZ / 16)) % BY;
+                tz[r] = (t % (BZ / 16)) * 16;
+            }
+            f32x4 acc[2] = {{0.0f, 0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f, 0.0f}};
+#pragma unroll
+            for (int kc = 0; kc < KC; kc++) {
+#pragma unroll
+                for (int r = 0; r < 2; r++) {
                     bf16x4 lo = bf16x4{0, 0, 0, 0}, hi = bf16x4{0, 0, 0, 0};   // 4 channels of two taps
                     const int t0 = 8 * kc + 2 * q, t1 = t0 + 1;
                     if (t0 < 27) {
@@ -214,28 +255,13 @@ __global__ void __launch_bounds__(256) k_conv(Vol V, const uint16_t* __restrict_
                         const int h = ((tx[r] + t1 / 9) * HY + (ty[r] + (t1 / 3) % 3)) * HZ + (tz[r] + m + t1 % 3);
                         hi = *(const bf16x4*)(halo + h * 4);
                     }
-                    a = bf16x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+                    const bf16x8 a = bf16x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+                    acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16v8, a),
+                                                                     __builtin_bit_cast(bf16v8, B[kc]), acc[r], 0, 0, 0);
                 }
-                acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16v8, a),
-                                                                 __builtin_bit_cast(bf16v8, B[kc]), acc[r], 0, 0, 0);
             }
-        }
-        const int co = m;
-        if (co >= cout) continue;
 #pragma unroll
-        for (int r = 0; r < 2; r++) {
-            const int gx = x0 + tx[r], gy = y0 + ty[r];
-            if (gx >= V.n0 || gy >= V.n1) continue;
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const int gz = z0 + tz[r] + 4 * q + j;
-                if (gz >= V.n2) continue;
-                const long vo = ((long)gx * V.n1 + gy) * V.n2 + gz;
-                const float zv = acc[r][j] + bco;
-                if (MODE == OUT_RELU_BF16) out_b[vo * 16 + co] = to_bf16(zv > 0.0f ? zv : 0.0f);
-                else if (MODE == OUT_F32) out_f[(long)co * V.nv + vo] = zv;
-                else out_b[vo * 16 + co] = to_bf16(from_bf16(mask_act[vo * 16 + co]) > 0.0f ? zv : 0.0f);
-            }
+            for (int r = 0; r < 2; r++) store(acc[r], tx[r], ty[r], tz[r]);
         }
     }
 }
@@ -379,17 +405,19 @@ __global__ void __launch_bounds__(256) k_wgrad_sum(int nwg, const float* __restr
 
 // weights -> MFMA B fragments of v_mfma_f32_16x16x32_bf16, wf[kc][lane][8].  Effective layer:
 // cin_e x cout_e with W_e(t, a, b) = W(t, a, b), or for the input gradient (flip) W(26 - t, b, a);
-// cin_pad 16: chunk kc holds taps 2kc, 2kc + 1 x 16 channels; cin_pad 4: taps 8kc .. 8kc + 7 x 4.
+// cin_pad 16: chunk kc = 5 (dx + 1) + c holds the taps 9 (dx + 1) + 2c, + 2c + 1 (the tenth
+// offset of an x-plane is zero) x 16 channels; cin_pad 4: taps 8kc .. 8kc + 7 x 4 channels.
 __device__ void pack_one(const float* __restrict__ W, int cin, int cout, int flip, int cin_pad, uint16_t* __restrict__ wf)
 {
-    const int kcs = cin_pad == 16 ? 14 : 4;
+    const int kcs = cin_pad == 16 ? KC16 : KC4;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= kcs * 64 * 8) return;
     const int j = i & 7, lane = (i >> 3) & 63, kc = i >> 9;
     const int n = lane & 15, k = 8 * (lane >> 4) + j;
     int tap, a;
     if (cin_pad == 16) {
-        tap = 2 * kc + k / 16;
+        const int o = 2 * (kc % 5) + k / 16;
+        tap = o < 9 ? 9 * (kc / 5) + o : 27;
         a = k % 16;
     } else {
         tap = 8 * kc + k / 4;
