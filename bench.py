@@ -405,6 +405,7 @@ def main():
         "fdtd_roofline": res.get("fdtd_roofline"),
         "atomics": res.get("atomics"),
         "kernels_ms_mean": {k: v["ms_mean"] for k, v in res["kernels"].items()},
+        "kernel_launches": {k: v["launches"] for k, v in res["kernels"].items()},
     }
     if plain is not None:
         line["plain_fp64"] = {"value": plain["particle_steps_per_s"], "ms_per_step": plain["ms_per_step"],
@@ -413,7 +414,8 @@ def main():
                               "fdtd_cell_updates_per_s": plain.get("fdtd_cell_updates_per_s"),
                               "fdtd_roofline": plain.get("fdtd_roofline"),
                               "atomics": plain.get("atomics"),
-                              "kernels_ms_mean": {k: v["ms_mean"] for k, v in plain["kernels"].items()}}
+                              "kernels_ms_mean": {k: v["ms_mean"] for k, v in plain["kernels"].items()},
+                              "kernel_launches": {k: v["launches"] for k, v in plain["kernels"].items()}}
         line["fp32_speedup_vs_fp64"] = res["particle_steps_per_s"] / plain["particle_steps_per_s"]
         # identical seeded inputs, same number of steps: total-energy difference of the FP32 form
         line["fp32_energy_rel_err_vs_fp64"] = abs(res["energy"] - plain["energy"]) / abs(plain["energy"])

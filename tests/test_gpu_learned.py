@@ -37,8 +37,10 @@ def dn(ctx):
     d.close()
 
 
+# (9, 12, 48) holds 4 x 4 x 16 bricks whose halo needs no periodic wrap (x0 = y0 = 4, z0 = 16),
+# bricks wholly inside the volume (16-byte stores) and bricks cut by its x-edge
 @pytest.mark.parametrize("layer", [1, 2, 3])
-@pytest.mark.parametrize("n", [(6, 5, 8), (5, 3, 7), (4, 4, 1), (2, 1, 3)])
+@pytest.mark.parametrize("n", [(6, 5, 8), (5, 3, 7), (4, 4, 1), (2, 1, 3), (9, 12, 48)])
 def test_conv_layer_matches_oracle(dn, layer, n):
     _say("conv layer %d n=%s" % (layer, n))
     p = _params(1)
@@ -54,11 +56,12 @@ def test_conv_layer_matches_oracle(dn, layer, n):
     assert err < 1e-5
 
 
-def test_forward_matches_oracle(dn):
-    _say("denoiser forward")
+@pytest.mark.parametrize("n", [(8, 6, 10), (9, 12, 48)])
+def test_forward_matches_oracle(dn, n):
+    _say("denoiser forward n=%s" % (n,))
     p = _params(3)
     dn.set_params(p)
-    x = np.random.default_rng(4).standard_normal((3, 8, 6, 10)).astype(np.float32)
+    x = np.random.default_rng(4).standard_normal((3,) + n).astype(np.float32)
     got = dn.forward(x)
     ref, _ = D.net_forward(p, x, emulate=True)
     err = rel_err(got, ref)
@@ -66,13 +69,14 @@ def test_forward_matches_oracle(dn):
     assert err < 5e-3
 
 
-def test_gradient_matches_oracle(dn):
-    _say("denoiser gradient")
+@pytest.mark.parametrize("n", [(8, 8, 6), (9, 12, 48)])
+def test_gradient_matches_oracle(dn, n):
+    _say("denoiser gradient n=%s" % (n,))
     p = _params(5)
     dn.set_params(p)
     rng = np.random.default_rng(6)
-    x = rng.standard_normal((3, 8, 8, 6)).astype(np.float32)
-    y = rng.standard_normal((3, 8, 8, 6)).astype(np.float32)
+    x = rng.standard_normal((3,) + n).astype(np.float32)
+    y = rng.standard_normal((3,) + n).astype(np.float32)
     loss, g = dn.grad(x, y)
     ref_loss, ref_g = D.net_grad(p, x, y, emulate=True)
     assert abs(loss - ref_loss) < 1e-4 * abs(ref_loss)
