@@ -28,13 +28,18 @@ import time
 
 import numpy as np
 
-from . import (C, DIAG_FIELD_ENERGY, ELECTRON_CHARGE, ESIRKEPOV, F32, F64, JX, JY, JZ, TSC, Denoiser,
-               EnsembleSupercells, YeeGrid, diag, plasma_parameters, step, step_denoised)
+from . import (C, DIAG_FIELD_ENERGY, ELECTRON_CHARGE, ESIRKEPOV, F32, F64, JX, JY, JZ, OPT_INTERLEAVE, TSC,
+               Denoiser, EnsembleSupercells, YeeGrid, diag, plasma_parameters, step, step_denoised)
 
 UTH = 0.01        # thermal spread of u = p/(mc)
 U_DRIFT = 0.1 / math.sqrt(1.0 - 0.1 * 0.1)    # u of the +-0.1c streams
 U_AMP = 0.05      # velocity mode amplitude scale (units of c)
 DT = 0.5 / C      # Courant 0.5 at h = 1 cm
+# FP32 interleave period of the timed low-ppc steps: at 128^3 x 4 ppc a 4^3 supercell fills two
+# wavefronts and its order decays slowly, so the particle stage plus its re-sort takes 0.82 ms per
+# step at periods 6-12 against 0.85 ms at the library's default 3, which stays the best at 64 ppc
+# (7.30 ms against 7.38 at 4 and 7.52 at 6)
+LOW_INTERLEAVE = 8
 
 
 def random_modes(rng, ncell, nmodes=3, amp=U_AMP):
@@ -224,7 +229,8 @@ def evaluate(ctx, dn, ncell=(128, 128, 128), ppc_low=4, ppc_mid=64, ppc_ref=256,
 def costs(ctx, dn, ncell=(128, 128, 128), ppc_low=4, ppc_mid=64, low_ncps=(4, 4, 4), seed=5):
     """ms per step of a mid-ppc plain step (FP32 split, as the denoised step, and FP64, pica's
     precision) and of a low-ppc denoised step (fresh plasmas), with the per-kernel HIP-event times of
-    the denoised step"""
+    the denoised step over one interleave period; the low-ppc legs run at LOW_INTERLEAVE, the
+    mid-ppc one at the context's period"""
     modes = random_modes(np.random.default_rng(seed), ncell)
     cost = {}
     mid = plasma(ctx, ncell, ppc_mid, modes, 1)
@@ -235,19 +241,22 @@ def costs(ctx, dn, ncell=(128, 128, 128), ppc_low=4, ppc_mid=64, low_ncps=(4, 4,
     cost["mid_plain_fp64_ms"] = 1e3 * timed_steps(lambda: step(*mid, DT, TSC, ESIRKEPOV), 10, ctx.sync)
     for obj in mid:
         obj.close()
+    period = ctx.get_option(OPT_INTERLEAVE)
+    ctx.set_option(OPT_INTERLEAVE, LOW_INTERLEAVE)
     low = plasma(ctx, ncell, ppc_low, modes, 2, low_ncps)
     cost["low_plain_ms"] = 1e3 * timed_steps(lambda: step(*low, DT, TSC, ESIRKEPOV), 10, ctx.sync)
     cost["low_denoised_ms"] = 1e3 * timed_steps(lambda: step_denoised(low[0], low[1], dn, DT, TSC, ESIRKEPOV), 10,
                                                 ctx.sync)
     ctx.timer_reset()
     ctx.timer_enable(True)
-    for _ in range(5):
+    for _ in range(LOW_INTERLEAVE):
         step_denoised(low[0], low[1], dn, DT, TSC, ESIRKEPOV)
     ctx.sync()
     ctx.timer_enable(False)
-    cost["low_denoised_kernels_ms"] = {k: ctx.timer(k)[0] / 5 for k in ("particles", "insert", "denoise", "project",
-                                                                        "yee_e", "yee_b", "ghost", "zero",
-                                                                        "interleave")}
+    ctx.set_option(OPT_INTERLEAVE, period)
+    cost["interleave_period"] = {"low": LOW_INTERLEAVE, "mid": period}
+    names = ("particles", "insert", "denoise", "project", "yee_e", "yee_b", "ghost", "zero", "interleave")
+    cost["low_denoised_kernels_ms"] = {k: ctx.timer(k)[0] / LOW_INTERLEAVE for k in names}
     for obj in low:
         obj.close()
     cost["mid_over_low_denoised"] = cost["mid_plain_ms"] / cost["low_denoised_ms"]
