@@ -505,8 +505,10 @@ __global__ void k_grid_crop_to_f32(GridView<T> g, int n0, int n1, int n2, Vol V,
 
 // physical J of a grid -> the network input xb[v][0..3] (BF16, scaled; the same roundings as
 // k_grid_to_f32 then k_input_f32)
+// div0 != null: also div0[v] = D J (pk_div_j) of the current as it is read
 template <class T>
-__global__ void k_grid_to_input(GridView<T> g, Vol V, double scale, uint16_t* __restrict__ xb)
+__global__ void k_grid_to_input(GridView<T> g, Vol V, double scale, uint16_t* __restrict__ xb,
+                                double* __restrict__ div0)
 {
     const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V.nv) return;
@@ -516,6 +518,10 @@ __global__ void k_grid_to_input(GridView<T> g, Vol V, double scale, uint16_t* __
 #pragma unroll
     for (int c = 0; c < 3; c++) xb[v * 4 + c] = to_bf16((float)(scale * (double)g.f[PK_JX + c][gi]));
     xb[v * 4 + 3] = 0;
+    if (div0) {
+        const int n[3] = {V.n0, V.n1, V.n2};
+        div0[v] = pk_div_j(g, n, x, y, z);
+    }
 }
 
 template <class T>
@@ -1120,8 +1126,9 @@ int pk_dn_apply(pk_dn* d, pk_grid* g) { return pk_dn_apply_impl(d, g, true); }
 }  // extern "C"
 
 // pk_dn_apply; fill = false leaves J's ghosts as they were (for a caller that reads J only through
-// the physical cells until it fills them itself)
-int pk_dn_apply_impl(pk_dn* d, pk_grid* g, bool fill)
+// the physical cells until it fills them itself); div0 != null receives D J of the input current
+// (FP64, one value per physical node)
+int pk_dn_apply_impl(pk_dn* d, pk_grid* g, bool fill, double* div0)
 {
     if (!d || !g) return PK_E_ARG;
     if (g->ctx != d->ctx) return pk_fail(d->ctx, PK_E_ARG, "pk_dn_apply: other context");
@@ -1136,10 +1143,10 @@ int pk_dn_apply_impl(pk_dn* d, pk_grid* g, bool fill)
     // which measured faster than writing the grid from the last layer's epilogue
     if (g->prec == PK_F64)
         hipLaunchKernelGGL(k_grid_to_input<double>, dim3(nblk(V.nv, 256)), dim3(256), 0, s, grid_view<double>(g), V,
-                           d->scale, d->xb);
+                           d->scale, d->xb, div0);
     else
         hipLaunchKernelGGL(k_grid_to_input<float>, dim3(nblk(V.nv, 256)), dim3(256), 0, s, grid_view<float>(g), V,
-                           d->scale, d->xb);
+                           d->scale, d->xb, div0);
     if ((r = forward(d, V))) return r;
     if (g->prec == PK_F64)
         hipLaunchKernelGGL(k_f32_to_grid<double>, dim3(nblk(V.nv, 256)), dim3(256), 0, s, grid_view<double>(g), V,

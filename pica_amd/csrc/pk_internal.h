@@ -125,6 +125,21 @@ struct GridView {
     double origin[3];
 };
 
+// forward-difference divergence of J at physical node (i, j, k) of an n[0] x n[1] x n[2] periodic
+// grid, FP64, from the physical cells alone (the neighbour past the last cell is the first one,
+// the value a filled ghost holds, so J's ghosts need not be filled)
+template <class T>
+__device__ __forceinline__ double pk_div_j(const GridView<T>& g, const int n[3], int i, int j, int k)
+{
+    const int ip = i + 1 == n[0] ? 0 : i + 1, jp = j + 1 == n[1] ? 0 : j + 1, kp = k + 1 == n[2] ? 0 : k + 1;
+    const auto at = [&](int a, int b, int c) { return ((long)(a + g.G) * g.s1 + (b + g.G)) * g.s2 + (c + g.G); };
+    const long c = at(i, j, k);
+    double r = ((double)g.f[PK_JX][at(ip, j, k)] - (double)g.f[PK_JX][c]) / g.step[0];
+    r += ((double)g.f[PK_JY][at(i, jp, k)] - (double)g.f[PK_JY][c]) / g.step[1];
+    r += ((double)g.f[PK_JZ][at(i, j, kp)] - (double)g.f[PK_JZ][c]) / g.step[2];
+    return r;
+}
+
 template <class T>
 GridView<T> grid_view(const pk_grid* g)
 {
@@ -276,7 +291,7 @@ int pk_field_zero(pk_grid* g, int comp);
 // Jx, Jy, Jz zeroed in one launch (they are consecutive components of the grid's block)
 int pk_field_zero_j(pk_grid* g);
 // pk_dn_apply; with fill = false J's ghosts are left as they were
-int pk_dn_apply_impl(pk_dn* d, pk_grid* g, bool fill);
+int pk_dn_apply_impl(pk_dn* d, pk_grid* g, bool fill, double* div0 = nullptr);
 int pk_field_yee_e(pk_grid* g, double dt);
 int pk_field_yee_b(pk_grid* g, double dt);
 int pk_field_ghost(pk_grid* g, int ops);

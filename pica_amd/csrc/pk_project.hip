@@ -58,18 +58,29 @@ __global__ void k_residual(GridView<T> g, Phys P, const double* __restrict__ rho
     w[v] = Cx<T>{(T)r, (T)0};
 }
 
-// D J at physical node v, FP64, from the physical cells alone (the periodic neighbour of the last
-// cell is the first one: the value a filled ghost holds, so J ghosts need not be filled)
+// physical node v -> (i, j, k); 32-bit division when the volume allows it
+__device__ __forceinline__ void phys_ijk(const Phys& P, long v, int& i, int& j, int& k)
+{
+    if (P.nv <= 0xffffffffl) {
+        const unsigned n12 = (unsigned)P.n[1] * (unsigned)P.n[2], u = (unsigned)v;
+        i = (int)(u / n12);
+        const unsigned r = u - (unsigned)i * n12;
+        j = (int)(r / (unsigned)P.n[2]);
+        k = (int)(r - (unsigned)j * (unsigned)P.n[2]);
+    } else {
+        i = (int)(v / ((long)P.n[1] * P.n[2]));
+        j = (int)((v / P.n[2]) % P.n[1]);
+        k = (int)(v % P.n[2]);
+    }
+}
+
+// D J at physical node v (pk_div_j)
 template <class T>
 __device__ __forceinline__ double div_j(const GridView<T>& g, const Phys& P, long v)
 {
-    const int i = (int)(v / ((long)P.n[1] * P.n[2])), j = (int)((v / P.n[2]) % P.n[1]), k = (int)(v % P.n[2]);
-    const int ip = i + 1 == P.n[0] ? 0 : i + 1, jp = j + 1 == P.n[1] ? 0 : j + 1, kp = k + 1 == P.n[2] ? 0 : k + 1;
-    const long c = grid_index(g.s1, g.s2, g.G, i, j, k);
-    double r = ((double)g.f[PK_JX][grid_index(g.s1, g.s2, g.G, ip, j, k)] - (double)g.f[PK_JX][c]) / g.step[0];
-    r += ((double)g.f[PK_JY][grid_index(g.s1, g.s2, g.G, i, jp, k)] - (double)g.f[PK_JY][c]) / g.step[1];
-    r += ((double)g.f[PK_JZ][grid_index(g.s1, g.s2, g.G, i, j, kp)] - (double)g.f[PK_JZ][c]) / g.step[2];
-    return r;
+    int i, j, k;
+    phys_ijk(P, v, i, j, k);
+    return pk_div_j(g, P.n, i, j, k);
 }
 
 // w[v] = (D J - d0[v], 0) = (D (J - Jref), 0): the residual when the current Jref, whose
@@ -85,14 +96,6 @@ __global__ void k_residual_ref(GridView<T> g, Phys P, const double* __restrict__
     w[v] = Cx<T>{(T)(div_j(g, P, v) - d0[v]), pack_ref ? (T)d0[v] : (T)0};
 }
 
-template <class T>
-__global__ void k_save_div(GridView<T> g, Phys P, double* __restrict__ d0)
-{
-    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= P.nv) return;
-    d0[v] = div_j(g, P, v);
-}
-
 // in-place radix-2 FFTs of pb pencils per workgroup: pencil p holds w[base(p) + j * stride],
 // j < n, base(p) = (p / inner) * ostride + p % inner (n, pb, inner powers of two, passed as
 // logarithms); tw[k] = exp(-2 pi i k / n), k < n / 2;
@@ -101,10 +104,18 @@ __global__ void k_save_div(GridView<T> g, Phys P, double* __restrict__ d0)
 // solve != null: each value is first divided by lambda(k) nv, the Laplacian's eigenvalue times
 // the volume (lambda = solve[i] + solve[n0 + j] + solve[n0 + n1 + k]; the mean mode is zeroed) --
 // the Poisson solve folded into the first inverse pass.
+// res.d0 != null: the pass reads no volume but forms the residual (D J - d0, pack ? d0 : 0), as
+// k_residual_ref writes it, from the grid (the first forward pass)
+template <class R>
+struct ResidualSrc {
+    GridView<R> g;
+    const double* d0 = nullptr;
+    int pack = 0;
+};
 template <class R>
 __global__ void __launch_bounds__(256) k_fft(Cx<R>* __restrict__ w, int logn, long npencil, int logpb, int loginner,
                                              long ostride, long stride, int along, const double2* __restrict__ tw,
-                                             int inverse, const double* __restrict__ solve, Phys P)
+                                             int inverse, const double* __restrict__ solve, Phys P, ResidualSrc<R> res)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const int n = 1 << logn, pb = 1 << logpb;
@@ -128,10 +139,15 @@ __global__ void __launch_bounds__(256) k_fft(Cx<R>* __restrict__ w, int logn, lo
             const long p = p0 + pl;
             if (idx < tot && p < npencil) {
                 const long gv = ((p >> loginner) * ostride + (p & inner_mask)) + (long)j * stride;
-                v[k] = w[gv];
+                if (res.d0) {
+                    const double d0 = res.d0[gv];
+                    v[k] = Cx<R>{(R)(div_j(res.g, P, gv) - d0), res.pack ? (R)d0 : (R)0};
+                } else {
+                    v[k] = w[gv];
+                }
                 if (solve) {
-                    const long n12 = (long)P.n[1] * P.n[2];
-                    const int vi = (int)(gv / n12), vj = (int)((gv / P.n[2]) % P.n[1]), vk = (int)(gv % P.n[2]);
+                    int vi, vj, vk;
+                    phys_ijk(P, gv, vi, vj, vk);
                     const double lam = solve[vi] + solve[P.n[0] + vj] + solve[P.n[0] + P.n[1] + vk];
                     const R f = gv == 0 ? (R)0 : (R)(1.0 / (lam * (double)P.nv));
                     v[k] = Cx<R>{v[k].x * f, v[k].y * f};
@@ -202,7 +218,8 @@ __global__ void k_correct(GridView<T> g, Phys P, const Cx<T>* __restrict__ w)
 {
     const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= P.nv) return;
-    const int i = (int)(v / ((long)P.n[1] * P.n[2])), j = (int)((v / P.n[2]) % P.n[1]), k = (int)(v % P.n[2]);
+    int i, j, k;
+    phys_ijk(P, v, i, j, k);
     const long c = grid_index(g.s1, g.s2, g.G, i, j, k);
     const double ph = (double)w[v].x;
     const int im = i == 0 ? P.n[0] - 1 : i - 1, jm = j == 0 ? P.n[1] - 1 : j - 1, km = k == 0 ? P.n[2] - 1 : k - 1;
@@ -305,7 +322,8 @@ int ensure_proj(pk_grid* g, const Phys& P)
 // forward: z, y, x passes; inverse: x, y, z, the first of them with the Poisson solve when
 // lam != null.  Edges of one cell (1D / 2D grids) need no pass.
 template <class R>
-int fft3(pk_grid* g, const Phys& P, Cx<R>* w, const double2* tw[3], int inverse, const double* lam)
+int fft3(pk_grid* g, const Phys& P, Cx<R>* w, const double2* tw[3], int inverse, const double* lam,
+         ResidualSrc<R> res = ResidualSrc<R>())
 {
     bool first = true;
     hipStream_t s = g->ctx->stream;
@@ -323,9 +341,11 @@ int fft3(pk_grid* g, const Phys& P, Cx<R>* w, const double2* tw[3], int inverse,
         hipLaunchKernelGGL(k_fft<R>, dim3((unsigned)((npencil + pb - 1) / pb)), dim3(256),
                            sizeof(Cx<R>) * (pb * (n + 1) + n / 2), s, w, log2_exact(n), npencil, log2_exact(pb),
                            log2_exact((int)inner[d]), ostride[d], stride[d], d == 2 ? 1 : 0, tw[d], inverse,
-                           (inverse && first) ? lam : nullptr, P);
+                           (inverse && first) ? lam : nullptr, P, first ? res : ResidualSrc<R>());
         first = false;
     }
+    if (first && res.d0)   // a grid of one cell: no pass ran, so the residual is formed alone
+        hipLaunchKernelGGL(k_residual_ref<R>, dim3(nblk(P.nv, 256)), dim3(256), 0, s, res.g, P, res.d0, w, res.pack);
     PK_HIP(g->ctx, hipGetLastError());
     return PK_OK;
 }
@@ -352,14 +372,18 @@ int project(pk_grid* g, const Phys& P, const double* rho1, double dt, ChargeGain
     tw[2] = tw[1] + P.n[1] / 2;
     // the twiddle block holds n / 2 complex values per dimension (none for an edge of one cell)
     const double* lam = tab + 2 * (P.n[0] / 2 + P.n[1] / 2 + P.n[2] / 2);
-    if (rho1)
+    // the reference-divergence residual is formed by the first forward pass as it loads
+    ResidualSrc<T> res;
+    if (rho1) {
         hipLaunchKernelGGL(k_residual<T>, dim3(nblk(P.nv, 256)), dim3(256), 0, s, grid_view<T>(g), P, rho1,
                            g->rho_prev, 1.0 / dt, w);
-    else
-        hipLaunchKernelGGL(k_residual_ref<T>, dim3(nblk(P.nv, 256)), dim3(256), 0, s, grid_view<T>(g), P,
-                           jref_of(g, P), w, cg.n > 0 ? 1 : 0);
+    } else {
+        res.g = grid_view<T>(g);
+        res.d0 = jref_of(g, P);
+        res.pack = cg.n > 0 ? 1 : 0;
+    }
     int r;
-    if ((r = fft3(g, P, w, tw, 0, nullptr))) return r;
+    if ((r = fft3(g, P, w, tw, 0, nullptr, res))) return r;
     if (!rho1 && cg.n > 0) {
         // both real fields went through the one transform; their filtered combination goes to a
         // second volume (the combination reads Z at k and -k)
@@ -454,14 +478,8 @@ int pk_step_denoised(pk_ens* e, pk_grid* g, pk_dn* d, double dt, int shape, int 
     if ((r = pk_field_fold_fill_j(g, dep != PK_ESIRKEPOV))) return r;
     if (dep == PK_ESIRKEPOV) {
         if ((r = ensure_proj(g, P))) return r;
-        if (g->prec == PK_F64)
-            hipLaunchKernelGGL(k_save_div<double>, dim3(nblk(P.nv, 256)), dim3(256), 0, g->ctx->stream,
-                               grid_view<double>(g), P, jref_of(g, P));
-        else
-            hipLaunchKernelGGL(k_save_div<float>, dim3(nblk(P.nv, 256)), dim3(256), 0, g->ctx->stream,
-                               grid_view<float>(g), P, jref_of(g, P));
-        PK_HIP(e->ctx, hipGetLastError());
-        if ((r = pk_dn_apply_impl(d, g, false))) return r;
+        // the denoiser's input pass also saves D J_esirkepov
+        if ((r = pk_dn_apply_impl(d, g, false, jref_of(g, P)))) return r;
         {
             pk_ktimer tm(e->ctx, "project");
             ChargeGain cg;
