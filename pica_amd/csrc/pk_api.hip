@@ -109,7 +109,10 @@ int pk_ctx_create(int device, pk_ctx** out)
     pk_ctx* c = new pk_ctx();
     c->device = device;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipHostMalloc((void**)&c->flags_host, sizeof(int32_t) * PK_NFLAGS, hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void**)&c->flags_host, sizeof(int32_t) * PK_NFLAGS, hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&c->flags_ev, hipEventDisableTiming) != hipSuccess) {
+        if (c->flags_host) (void)hipHostFree(c->flags_host);
+        if (c->stream) (void)hipStreamDestroy(c->stream);
         delete c;
         return PK_E_HIP;
     }
@@ -135,6 +138,7 @@ void pk_ctx_destroy(pk_ctx* c)
         (void)hipEventDestroy(p.b);
     }
     for (auto ev : c->evpool) (void)hipEventDestroy(ev);
+    if (c->flags_ev) (void)hipEventDestroy(c->flags_ev);
     if (c->flags_host) (void)hipHostFree(c->flags_host);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -414,11 +418,34 @@ int pk_ens_resort(pk_ens* e)
     int r = pk_ens_usable(e);
     if (r) return r;
     if (!e->pending_movers) return PK_OK;
-    r = pk_ens_insert_launch(e);
+    if ((r = pk_ens_resort_begin(e))) return r;
+    return pk_ens_resort_end(e);
+}
+
+}  // extern "C"
+
+int pk_ens_resort_begin(pk_ens* e)
+{
+    if (!e->pending_movers || e->resort_inflight) return PK_OK;
+    int r = pk_ens_usable(e);
     if (r) return r;
+    if ((r = pk_ens_insert_launch(e))) return r;
+    pk_ctx* c = e->ctx;
+    PK_HIP(c, hipMemcpyAsync(c->flags_host, e->flags, sizeof(int32_t) * PK_NFLAGS, hipMemcpyDeviceToHost, c->stream));
+    PK_HIP(c, hipEventRecord(c->flags_ev, c->stream));
+    e->resort_inflight = true;
+    return PK_OK;
+}
+
+int pk_ens_resort_end(pk_ens* e)
+{
+    if (!e->resort_inflight) return PK_OK;
+    e->resort_inflight = false;
+    pk_ctx* c = e->ctx;
+    PK_HIP(c, hipEventSynchronize(c->flags_ev));
     int32_t fl[PK_NFLAGS];
-    r = pk_flags_read(e, fl);
-    if (r) return r;
+    for (int i = 0; i < PK_NFLAGS; i++) fl[i] = c->flags_host[i];
+    int r;
     e->pending_movers = false;
     if (fl[PK_FLAG_OVERFLOW]) {
         r = pk_ens_rebuild_after_overflow(e);
@@ -435,6 +462,8 @@ int pk_ens_resort(pk_ens* e)
     if (e->posmode == PK_POS_SPLIT && period > 0 && e->resorts % period == 0) return pk_ens_interleave_launch(e);
     return PK_OK;
 }
+
+extern "C" {
 
 int pk_ens_download(pk_ens* e, double* x, double* y, double* z, double* ux, double* uy, double* uz, double* w,
                     int64_t* off)
@@ -503,11 +532,14 @@ int pk_step(pk_ens* e, pk_grid* g, double dt, int shape, int dep)
     if ((r = pk_field_zero_j(g))) return r;
     if ((r = pk_particle_step(e, g, dt, shape, dep))) return r;
     if ((r = pk_field_fold_fill_j(g, true))) return r;
+    // the re-sort's insert goes ahead of the field update (they touch disjoint data), so that the
+    // GPU runs the field update while the host waits for the re-sort's flags
+    if ((r = pk_ens_resort_begin(e))) return r;
     // B(dt/2), E(dt), B(dt/2) with ghost fills (the leapfrog around YeeSolver.h:28-38)
-    if ((r = pk_field_yee_periodic(g, dt * 0.5, false))) return r;
-    if ((r = pk_field_yee_periodic(g, dt, true))) return r;
-    if ((r = pk_field_yee_periodic(g, dt * 0.5, false))) return r;
-    return pk_ens_resort(e);
+    if (!(r = pk_field_yee_periodic(g, dt * 0.5, false)) && !(r = pk_field_yee_periodic(g, dt, true)))
+        r = pk_field_yee_periodic(g, dt * 0.5, false);
+    const int rs = pk_ens_resort_end(e);
+    return r ? r : rs;
 }
 
 int pk_step_species(pk_ens* const* ens, int n, pk_grid* g, double dt, int shape, int dep)

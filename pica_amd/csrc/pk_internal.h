@@ -40,6 +40,7 @@ struct pk_ctx {
     std::vector<hipEvent_t> evpool;
     std::map<std::string, pk_timer_rec> timers;
     int32_t* flags_host = nullptr;     // pinned, PK_NFLAGS
+    hipEvent_t flags_ev = nullptr;     // recorded after a re-sort's flag copy (pk_ens_resort_begin)
     int opt[PK_OPT_N] = {PK_KERNEL_AUTO, 0, 3};
 };
 
@@ -90,6 +91,7 @@ struct pk_ens {
     int maxcap = 0;
     int64_t np = 0;
     bool pending_movers = false;  // particle step done, re-sort not yet done
+    bool resort_inflight = false; // insert and flag copy launched, flags not yet read
     bool failed = false;          // a re-sort rebuild failed: contents inconsistent until upload / load
     long resorts = 0;             // re-sorts done (the interleave period counts them)
     int nfield() const { return prec == PK_F64 ? 7 : 8; }
@@ -282,6 +284,11 @@ int pk_batch_interleave_launch(pk_batch* b);
 int pk_ens_interleave_launch(pk_ens* e);   // reorder now, no re-sort first
 int pk_check_pair(pk_ens* e, pk_grid* g);   // the periodic step's grid/ensemble requirements
 int pk_flags_read(pk_ens* e, int32_t out[PK_NFLAGS]);
+// pk_ens_resort in two halves, so that independent work can be queued between them:
+// _begin launches the insert and the copy of the flags (no wait); _end waits for that copy,
+// rebuilds after an overflow, reports a failed particle and launches the interleave when due
+int pk_ens_resort_begin(pk_ens* e);
+int pk_ens_resort_end(pk_ens* e);
 int pk_flags_clear(pk_ens* e);
 int pk_ens_usable(pk_ens* e);               // PK_E_STATE after a failed re-sort rebuild
 int pk_ens_rebuild_after_overflow(pk_ens* e);   // rebuild of an overflowed re-sort; marks failure

@@ -488,14 +488,17 @@ int pk_step_denoised(pk_ens* e, pk_grid* g, pk_dn* d, double dt, int shape, int 
             if (r) return r;
         }
         if ((r = pk_field_ghost(g, PK_FILL_J))) return r;
-        if ((r = pk_ens_resort(e))) return r;
+        // the re-sort's insert goes ahead of the field update (disjoint data); its flags are
+        // read while the GPU runs the field update
+        if ((r = pk_ens_resort_begin(e))) return r;
     } else {
         if ((r = pk_dn_apply(d, g))) return r;
         if ((r = pk_project_continuity(e, g, shape, dt))) return r;
     }
-    if ((r = pk_field_yee_periodic(g, dt * 0.5, false))) return r;
-    if ((r = pk_field_yee_periodic(g, dt, true))) return r;
-    return pk_field_yee_periodic(g, dt * 0.5, false);
+    if (!(r = pk_field_yee_periodic(g, dt * 0.5, false)) && !(r = pk_field_yee_periodic(g, dt, true)))
+        r = pk_field_yee_periodic(g, dt * 0.5, false);
+    const int rs = pk_ens_resort_end(e);
+    return r ? r : rs;
 }
 
 }  // extern "C"
