@@ -120,8 +120,12 @@ __global__ void __launch_bounds__(256) k_fft(Cx<R>* __restrict__ w, int logn, lo
     extern __shared__ __align__(16) unsigned char smem[];
     const int n = 1 << logn, pb = 1 << logpb;
     Cx<R>* twl = (Cx<R>*)smem;        // n / 2 twiddles, conjugated for the inverse
-    Cx<R>* a = twl + (n >> 1);        // pb rows of n + 1 (the pad spreads a row's start over the banks)
-    const int row = n + 1;
+    // pb rows of n + n / 8 + 1 values: element i of a row sits at i + i / 8 (one pad value after every
+    // eight, so that the bit-reversed writes and the butterflies' strided accesses spread over the
+    // banks), and the last pad spreads the rows' starts
+    Cx<R>* a = twl + (n >> 1);
+    const int row = n + (n >> 3) + 1;
+    const auto at = [row](int pl, int i) { return pl * row + i + (i >> 3); };
     for (int t = threadIdx.x; t < (n >> 1); t += blockDim.x) {
         const double2 wd = tw[t];
         twl[t] = Cx<R>{(R)wd.x, inverse ? (R)-wd.y : (R)wd.y};
@@ -158,8 +162,7 @@ __global__ void __launch_bounds__(256) k_fft(Cx<R>* __restrict__ w, int logn, lo
         for (int k = 0; k < CH; k++) {
             const int idx = i0 + 256 * k;
             const int pl = along ? idx >> logn : idx & (pb - 1), j = along ? idx & (n - 1) : idx >> logpb;
-            if (idx < tot && p0 + pl < npencil)
-                a[pl * row + (int)(__brev((unsigned)j) >> (32 - logn))] = v[k];
+            if (idx < tot && p0 + pl < npencil) a[at(pl, (int)(__brev((unsigned)j) >> (32 - logn)))] = v[k];
         }
     }
     __syncthreads();
@@ -175,8 +178,9 @@ __global__ void __launch_bounds__(256) k_fft(Cx<R>* __restrict__ w, int logn, lo
         for (int t = threadIdx.x; t < nq; t += blockDim.x) {
             const int pl = t >> (logn - 2), tt = t & ((n >> 2) - 1);
             const int grp = tt >> lh, pos = tt & (h - 1);
-            const int i0 = pl * row + (grp << (lh + 2)) + pos;
-            const Cx<R> x0 = a[i0], x1 = a[i0 + h], x2 = a[i0 + 2 * h], x3 = a[i0 + 3 * h];
+            const int b0 = (grp << (lh + 2)) + pos;
+            const int i0 = at(pl, b0), i1 = at(pl, b0 + h), i2 = at(pl, b0 + 2 * h), i3 = at(pl, b0 + 3 * h);
+            const Cx<R> x0 = a[i0], x1 = a[i1], x2 = a[i2], x3 = a[i3];
             const Cx<R> w1 = twl[pos << (logn - 1 - lh)];
             const Cx<R> v1 = mul(x1, w1), v3 = mul(x3, w1);
             const Cx<R> y0{x0.x + v1.x, x0.y + v1.y}, y1{x0.x - v1.x, x0.y - v1.y};
@@ -184,9 +188,9 @@ __global__ void __launch_bounds__(256) k_fft(Cx<R>* __restrict__ w, int logn, lo
             const Cx<R> w2 = twl[pos << (logn - 2 - lh)], w3 = twl[(pos + h) << (logn - 2 - lh)];
             const Cx<R> v2 = mul(y2, w2), u3 = mul(y3, w3);
             a[i0] = Cx<R>{y0.x + v2.x, y0.y + v2.y};
-            a[i0 + 2 * h] = Cx<R>{y0.x - v2.x, y0.y - v2.y};
-            a[i0 + h] = Cx<R>{y1.x + u3.x, y1.y + u3.y};
-            a[i0 + 3 * h] = Cx<R>{y1.x - u3.x, y1.y - u3.y};
+            a[i2] = Cx<R>{y0.x - v2.x, y0.y - v2.y};
+            a[i1] = Cx<R>{y1.x + u3.x, y1.y + u3.y};
+            a[i3] = Cx<R>{y1.x - u3.x, y1.y - u3.y};
         }
         __syncthreads();
     }
@@ -196,7 +200,8 @@ __global__ void __launch_bounds__(256) k_fft(Cx<R>* __restrict__ w, int logn, lo
         for (int t = threadIdx.x; t < nb; t += blockDim.x) {
             const int pl = t >> (logn - 1), tt = t & ((n >> 1) - 1);
             const int grp = tt >> lh, pos = tt & (half - 1);
-            const int i0 = pl * row + (grp << (lh + 1)) + pos, i1 = i0 + half;
+            const int b0 = (grp << (lh + 1)) + pos;
+            const int i0 = at(pl, b0), i1 = at(pl, b0 + half);
             const Cx<R> u = a[i0], v = mul(a[i1], twl[pos << (logn - 1 - lh)]);
             a[i0] = Cx<R>{u.x + v.x, u.y + v.y};
             a[i1] = Cx<R>{u.x - v.x, u.y - v.y};
@@ -208,7 +213,7 @@ __global__ void __launch_bounds__(256) k_fft(Cx<R>* __restrict__ w, int logn, lo
         const long p = p0 + pl;
         if (p >= npencil) continue;
         const long base = (p >> loginner) * ostride + (p & inner_mask);
-        w[base + (long)j * stride] = a[pl * row + j];
+        w[base + (long)j * stride] = a[at(pl, j)];
     }
 }
 
@@ -337,9 +342,9 @@ int fft3(pk_grid* g, const Phys& P, Cx<R>* w, const double2* tw[3], int inverse,
         const int n = P.n[d];
         if (n == 1) continue;
         const long npencil = P.nv / n;
-        const int pb = n <= 256 ? 16 : 4096 / n;          // at most 74 KB of LDS (FP64, n = 1024)
+        const int pb = n <= 256 ? 16 : 4096 / n;          // at most 82 KB of LDS (FP64, n = 1024)
         hipLaunchKernelGGL(k_fft<R>, dim3((unsigned)((npencil + pb - 1) / pb)), dim3(256),
-                           sizeof(Cx<R>) * (pb * (n + 1) + n / 2), s, w, log2_exact(n), npencil, log2_exact(pb),
+                           sizeof(Cx<R>) * (pb * (n + n / 8 + 1) + n / 2), s, w, log2_exact(n), npencil, log2_exact(pb),
                            log2_exact((int)inner[d]), ostride[d], stride[d], d == 2 ? 1 : 0, tw[d], inverse,
                            (inverse && first) ? lam : nullptr, P, first ? res : ResidualSrc<R>());
         first = false;
