@@ -272,7 +272,8 @@ int pk_particle_step(pk_ens* e, pk_grid* g, double dt, int shape, int dep);
 /* moves extracted particles into their new supercells (deterministic order) */
 int pk_ens_resort(pk_ens* e);
 /* a whole periodic PIC step: J = 0, particle step, fold J, B(dt/2), E(dt), B(dt/2) with ghost
- * fills, re-sort */
+ * fills, re-sort (its insert is queued ahead of the field update, which touches no particle, and
+ * the host reads its overflow flags while the GPU runs the field update) */
 int pk_step(pk_ens* e, pk_grid* g, double dt, int shape, int dep);
 /* pk_step for several species on one grid (pica's ParticleTypes table, Particle.h:26-31, with
  * one ensemble per type set by pk_species_set): every ensemble's particle stage adds to J, then
