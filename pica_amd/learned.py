@@ -243,17 +243,19 @@ def costs(ctx, dn, ncell=(128, 128, 128), ppc_low=4, ppc_mid=64, low_ncps=(4, 4,
         obj.close()
     period = ctx.get_option(OPT_INTERLEAVE)
     ctx.set_option(OPT_INTERLEAVE, LOW_INTERLEAVE)
-    low = plasma(ctx, ncell, ppc_low, modes, 2, low_ncps)
-    cost["low_plain_ms"] = 1e3 * timed_steps(lambda: step(*low, DT, TSC, ESIRKEPOV), 10, ctx.sync)
-    cost["low_denoised_ms"] = 1e3 * timed_steps(lambda: step_denoised(low[0], low[1], dn, DT, TSC, ESIRKEPOV), 10,
-                                                ctx.sync)
-    ctx.timer_reset()
-    ctx.timer_enable(True)
-    for _ in range(LOW_INTERLEAVE):
-        step_denoised(low[0], low[1], dn, DT, TSC, ESIRKEPOV)
-    ctx.sync()
-    ctx.timer_enable(False)
-    ctx.set_option(OPT_INTERLEAVE, period)
+    try:
+        low = plasma(ctx, ncell, ppc_low, modes, 2, low_ncps)
+        cost["low_plain_ms"] = 1e3 * timed_steps(lambda: step(*low, DT, TSC, ESIRKEPOV), 10, ctx.sync)
+        cost["low_denoised_ms"] = 1e3 * timed_steps(
+            lambda: step_denoised(low[0], low[1], dn, DT, TSC, ESIRKEPOV), 10, ctx.sync)
+        ctx.timer_reset()
+        ctx.timer_enable(True)
+        for _ in range(LOW_INTERLEAVE):
+            step_denoised(low[0], low[1], dn, DT, TSC, ESIRKEPOV)
+        ctx.sync()
+        ctx.timer_enable(False)
+    finally:
+        ctx.set_option(OPT_INTERLEAVE, period)
     cost["interleave_period"] = {"low": LOW_INTERLEAVE, "mid": period}
     names = ("particles", "insert", "denoise", "project", "yee_e", "yee_b", "ghost", "zero", "interleave")
     cost["low_denoised_kernels_ms"] = {k: ctx.timer(k)[0] / LOW_INTERLEAVE for k in names}
