@@ -431,8 +431,14 @@ int pk_ens_resort_begin(pk_ens* e)
     if (r) return r;
     if ((r = pk_ens_insert_launch(e))) return r;
     pk_ctx* c = e->ctx;
-    PK_HIP(c, hipMemcpyAsync(c->flags_host, e->flags, sizeof(int32_t) * PK_NFLAGS, hipMemcpyDeviceToHost, c->stream));
-    PK_HIP(c, hipEventRecord(c->flags_ev, c->stream));
+    if (hipMemcpyAsync(c->flags_host, e->flags, sizeof(int32_t) * PK_NFLAGS, hipMemcpyDeviceToHost, c->stream) !=
+            hipSuccess ||
+        hipEventRecord(c->flags_ev, c->stream) != hipSuccess) {
+        // the insert is queued: running it again would insert its movers twice
+        e->failed = true;
+        return pk_fail(c, PK_E_HIP, "re-sort: reading the insert's flags failed; the ensemble is unusable until "
+                                    "it is uploaded or loaded again");
+    }
     e->resort_inflight = true;
     return PK_OK;
 }
@@ -442,7 +448,12 @@ int pk_ens_resort_end(pk_ens* e)
     if (!e->resort_inflight) return PK_OK;
     e->resort_inflight = false;
     pk_ctx* c = e->ctx;
-    PK_HIP(c, hipEventSynchronize(c->flags_ev));
+    if (hipEventSynchronize(c->flags_ev) != hipSuccess) {
+        // the insert ran or may have run: running it again would insert its movers twice
+        e->failed = true;
+        return pk_fail(c, PK_E_HIP, "re-sort: waiting for the insert failed; the ensemble is unusable until it "
+                                    "is uploaded or loaded again");
+    }
     int32_t fl[PK_NFLAGS];
     for (int i = 0; i < PK_NFLAGS; i++) fl[i] = c->flags_host[i];
     int r;
