@@ -88,8 +88,13 @@ int pk_timer_reset(pk_ctx* ctx);
  * the particle kernel takes two adjacent slots and deposits a same-cell pair with one set of LDS
  * atomics, and the lanes of a wavefront sit in different cells.  0 turns it off.  Supercells of
  * more than 64 cells or 4096 particles, or with a cell of more than 128 particles, are left as
- * they are.  Batched stepping (pk_batch_run) reorders at the same re-sort counts. */
-enum { PK_OPT_PARTICLE_KERNEL = 0, PK_OPT_CHANGE_QUEUE = 1, PK_OPT_INTERLEAVE = 2, PK_OPT_N };
+ * they are.  Batched stepping (pk_batch_run) reorders at the same re-sort counts.
+ * PK_OPT_INTERLEAVE_F64 (default 0, off): the same for FP64 ensembles, whose kernel takes one
+ * particle per lane: rank-major order (the r-th particles of every cell, by cell, for r = 0, 1,
+ * ...), as the loader leaves them.  Off by default so that FP64 runs keep the oracle's re-sort
+ * order (pica's own arithmetic does not depend on the order); batched stepping ignores it. */
+enum { PK_OPT_PARTICLE_KERNEL = 0, PK_OPT_CHANGE_QUEUE = 1, PK_OPT_INTERLEAVE = 2, PK_OPT_INTERLEAVE_F64 = 3,
+       PK_OPT_N };
 enum { PK_KERNEL_AUTO = 0, PK_KERNEL_PER_PARTICLE = 1, PK_KERNEL_CELL_SORTED = 2 };
 int pk_ctx_set_option(pk_ctx* ctx, int option, int value);
 int pk_ctx_get_option(pk_ctx* ctx, int option, int* value);

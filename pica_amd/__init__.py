@@ -33,7 +33,7 @@ FILL_E, FILL_B, FILL_J, FOLD_J, FOLD_RHO = 1, 2, 4, 8, 16
 (DIAG_FIELD_ENERGY, DIAG_KINETIC_ENERGY, DIAG_CHARGE, DIAG_GAUSS_MAX, DIAG_GAUSS_DRIFT, DIAG_RHO_MAX,
  DIAG_CONTINUITY_MAX, DIAG_DIVJ_MAX) = range(8)
 DIAG_N = 8
-OPT_PARTICLE_KERNEL, OPT_CHANGE_QUEUE, OPT_INTERLEAVE = 0, 1, 2
+OPT_PARTICLE_KERNEL, OPT_CHANGE_QUEUE, OPT_INTERLEAVE, OPT_INTERLEAVE_F64 = 0, 1, 2, 3
 BATCH_WG_PER_CU, BATCH_ORDER, BATCH_GRAPH, BATCH_SUPERCELL, BATCH_INTERLEAVE, BATCH_BLOCK = 0, 1, 2, 3, 4, 5
 SHAPE_PER_INSTANCE = 2
 KERNEL_AUTO, KERNEL_PER_PARTICLE, KERNEL_CELL_SORTED = 0, 1, 2

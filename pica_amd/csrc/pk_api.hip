@@ -176,7 +176,7 @@ int pk_ctx_set_option(pk_ctx* c, int option, int value)
         c->opt[option] = value;
         return PK_OK;
     }
-    if (option == PK_OPT_INTERLEAVE) {
+    if (option == PK_OPT_INTERLEAVE || option == PK_OPT_INTERLEAVE_F64) {
         if (value < 0) return pk_fail(c, PK_E_ARG, "pk_ctx_set_option: negative interleave period");
         c->opt[option] = value;
         return PK_OK;
@@ -469,8 +469,9 @@ int pk_ens_resort_end(pk_ens* e)
                        "particle step: a particle was not finite or moved further than its tile allows "
                        "(it was left unchanged)");
     e->resorts++;
-    const int period = e->ctx->opt[PK_OPT_INTERLEAVE];
-    if (e->posmode == PK_POS_SPLIT && period > 0 && e->resorts % period == 0) return pk_ens_interleave_launch(e);
+    const int period = e->ctx->opt[e->posmode == PK_POS_SPLIT ? PK_OPT_INTERLEAVE : PK_OPT_INTERLEAVE_F64];
+    if (period > 0 && e->resorts % period == 0)
+        return e->posmode == PK_POS_SPLIT ? pk_ens_interleave_launch(e) : pk_ens_interleave_f64_launch(e);
     return PK_OK;
 }
 

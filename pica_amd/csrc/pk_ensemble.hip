@@ -866,6 +866,158 @@ __device__ __forceinline__ void interleave_sc(const EnsGeom& eg, const PartSplit
     }
 }
 
+// FP64 ensembles (one particle per lane in the FP64 kernel): the order is rank-major, the particles
+// of rank r (r-th of their cell in storage order) by cell, so the lanes of a wavefront sit in
+// different cells, as the loader leaves them (slot = r * cells + cell when all counts are equal).
+// A particle's cell is floor(x / cellSize) per dimension (k_cells), relative to the supercell.
+// Ranks as in interleave_sc; the seven 64-bit fields then pass one at a time through a 64-bit
+// LDS stage, two fields ahead in flight.  Supercells of more than 64 cells or 256 IL_P particles,
+// or with a cell of more than IL_RMAX particles, are left as they are.
+template <int IL_P>
+__device__ __forceinline__ void interleave_f64_sc(const EnsGeom& eg, const StageGeom& sg, const PartF64& pm,
+                                                  const int64_t* __restrict__ off, const int32_t* __restrict__ cnt, long s)
+{
+    const int n = cnt[s];
+    const int ncl = eg.ncps[0] * eg.ncps[1] * eg.ncps[2];
+    if (n < 2 || n > 256 * IL_P || (IL_P > 8 && n <= 256 * 8) || ncl > 64) return;
+    __shared__ unsigned long long cm[64][4];
+    __shared__ int hist[64];
+    __shared__ unsigned long long M[IL_RMAX];
+    __shared__ int start[IL_RMAX];
+    __shared__ int maxc, bad;
+    __shared__ __align__(16) uint64_t stage[2][256 * IL_P];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint64_t below = (1ull << lane) - 1ull;
+    const long base = off[s];
+    const int m0[3] = {(int)(s / ((long)eg.nsc[1] * eg.nsc[2])) * eg.ncps[0],
+                       (int)((s / eg.nsc[2]) % eg.nsc[1]) * eg.ncps[1], (int)(s % eg.nsc[2]) * eg.ncps[2]};
+    uint64_t* const fld[7] = {(uint64_t*)pm.x,  (uint64_t*)pm.y,  (uint64_t*)pm.z, (uint64_t*)pm.ux,
+                              (uint64_t*)pm.uy, (uint64_t*)pm.uz, (uint64_t*)pm.w};
+    constexpr int RING = 3;            // fields held in registers: the three positions, then a ring
+    uint64_t v[RING][IL_P];
+    int key[IL_P], pos[IL_P];
+#pragma unroll
+    for (int f = 0; f < RING; f++)
+#pragma unroll
+        for (int k = 0; k < IL_P; k++)
+            if (tid + 256 * k < n) v[f][k] = fld[f][base + tid + 256 * k];
+#pragma unroll
+    for (int k = 0; k < IL_P; k++) {
+        key[k] = -1;
+        pos[k] = 0;
+        if (tid + 256 * k < n) {
+            int kk = 0;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                int l = (int)floor((__longlong_as_double((long long)v[d][k]) - 0.0) * sg.cellinv[d]) - m0[d];
+                l = l < 0 ? 0 : (l >= eg.ncps[d] ? eg.ncps[d] - 1 : l);   // a stray particle still gets a slot
+                kk = kk * eg.ncps[d] + l;
+            }
+            key[k] = kk;
+        }
+    }
+    cm[tid >> 2][tid & 3] = 0ull;
+    if (tid < 64) hist[tid] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < IL_P; k++) {
+        if (256 * k >= n) break;
+        const int kk = key[k];
+        if (kk >= 0) atomicOr(&cm[kk][wave], 1ull << lane);
+        __syncthreads();
+        if (kk >= 0) {
+            int r = hist[kk] + __popcll(cm[kk][wave] & below);
+            for (int w = 0; w < wave; w++) r += __popcll(cm[kk][w]);
+            pos[k] = r;                                   // the rank for now
+        }
+        __syncthreads();
+        if (tid < 64) {
+            hist[tid] += __popcll(cm[tid][0]) + __popcll(cm[tid][1]) + __popcll(cm[tid][2]) + __popcll(cm[tid][3]);
+            cm[tid][0] = 0ull; cm[tid][1] = 0ull; cm[tid][2] = 0ull; cm[tid][3] = 0ull;
+        }
+        __syncthreads();
+    }
+    // rank row r: M[r] bit c = cell c holds more than r particles, start[r] = slots of the rows
+    // below; lane l builds rows l and l + 64
+    if (wave == 0) {
+        const int h = lane < ncl ? hist[lane] : 0;
+        int mx = h;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const int t = __shfl_xor(mx, o);
+            mx = t > mx ? t : mx;
+        }
+        const int rmax = mx > IL_RMAX ? IL_RMAX : mx;
+        uint64_t mine0 = 0ull, mine1 = 0ull;
+        for (int r = 0; r < rmax; r++) {
+            const uint64_t b = __ballot(h > r);
+            if (r == lane) mine0 = b;
+            if (r == lane + 64) mine1 = b;
+        }
+        const int c0 = lane < rmax ? __popcll(mine0) : 0, c1 = lane + 64 < rmax ? __popcll(mine1) : 0;
+        int i0 = c0, i1 = c1;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t0 = __shfl_up(i0, o), t1 = __shfl_up(i1, o);
+            if (lane >= o) {
+                i0 += t0;
+                i1 += t1;
+            }
+        }
+        const int tot0 = __shfl(i0, 63), tot1 = __shfl(i1, 63);
+        M[lane] = mine0;
+        start[lane] = i0 - c0;
+        if (lane + 64 < IL_RMAX) {
+            M[lane + 64] = mine1;
+            start[lane + 64] = tot0 + i1 - c1;
+        }
+        if (lane == 63) {
+            maxc = mx;
+            bad = mx > IL_RMAX || tot0 + tot1 != n;
+        }
+    }
+    __syncthreads();
+    const int mx = maxc;
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < IL_P; k++)
+        if (key[k] >= 0) {
+            const int r = pos[k], h = hist[key[k]];
+            const uint64_t below_c = (1ull << key[k]) - 1ull;
+            if (r < 0 || r >= h || h > mx || mx > IL_RMAX || !((M[r] >> key[k]) & 1ull))
+                ok = false;
+            else
+                pos[k] = start[r] + __popcll(M[r] & below_c);
+        }
+    if (!ok) bad = 1;
+    __syncthreads();
+    if (bad) return;
+#pragma unroll
+    for (int f = 0; f < 7; f++) {
+        uint64_t* const st = stage[f & 1];
+#pragma unroll
+        for (int k = 0; k < IL_P; k++)
+            if (key[k] >= 0) st[pos[k]] = v[f % RING][k];
+        __syncthreads();
+        if (f + RING < 7)
+#pragma unroll
+            for (int k = 0; k < IL_P; k++)
+                if (tid + 256 * k < n) v[f % RING][k] = fld[f + RING][base + tid + 256 * k];
+#pragma unroll
+        for (int k = 0; k < IL_P; k++)
+            if (tid + 256 * k < n) fld[f][base + tid + 256 * k] = st[tid + 256 * k];
+    }
+}
+
+template <int IL_P>
+__global__ void __launch_bounds__(256) k_interleave_f64(EnsGeom eg, StageGeom sg, PartF64 pm,
+                                                        const int64_t* __restrict__ off, const int32_t* __restrict__ cnt)
+{
+    const long s = pk_xcd_supercell(blockIdx.x, eg.xcd_chunk);
+    if (s >= eg.nsct) return;
+    interleave_f64_sc<IL_P>(eg, sg, pm, off, cnt, s);
+}
+
 template <int IL_P>
 __global__ void __launch_bounds__(256) k_interleave_split(EnsGeom eg, PartSplit pm, const int64_t* __restrict__ off,
                                                           const int32_t* __restrict__ cnt)
@@ -910,6 +1062,21 @@ int pk_ens_interleave_launch(pk_ens* e)
     if (il_big(e->maxcap))
         hipLaunchKernelGGL(k_interleave_split<16>, dim3((unsigned)(PK_XCD * (long)eg.xcd_chunk)), dim3(256), 0,
                            e->ctx->stream, eg, part_split(e, e->main), e->off, e->cnt);
+    PK_HIP(e->ctx, hipGetLastError());
+    return PK_OK;
+}
+
+int pk_ens_interleave_f64_launch(pk_ens* e)
+{
+    if (e->posmode != PK_POS_ABS || e->np == 0) return PK_OK;
+    EnsGeom eg = ens_geom(e);
+    StageGeom sg = stage_geom(e);
+    pk_ktimer tm(e->ctx, "interleave");
+    hipLaunchKernelGGL(k_interleave_f64<8>, dim3((unsigned)(PK_XCD * (long)eg.xcd_chunk)), dim3(256), 0,
+                       e->ctx->stream, eg, sg, part_f64(e, e->main), e->off, e->cnt);
+    if (il_big(e->maxcap))
+        hipLaunchKernelGGL(k_interleave_f64<16>, dim3((unsigned)(PK_XCD * (long)eg.xcd_chunk)), dim3(256), 0,
+                           e->ctx->stream, eg, sg, part_f64(e, e->main), e->off, e->cnt);
     PK_HIP(e->ctx, hipGetLastError());
     return PK_OK;
 }

@@ -41,7 +41,7 @@ struct pk_ctx {
     std::map<std::string, pk_timer_rec> timers;
     int32_t* flags_host = nullptr;     // pinned, PK_NFLAGS
     hipEvent_t flags_ev = nullptr;     // recorded after a re-sort's flag copy (pk_ens_resort_begin)
-    int opt[PK_OPT_N] = {PK_KERNEL_AUTO, 0, 3};
+    int opt[PK_OPT_N] = {PK_KERNEL_AUTO, 0, 3, 0};
 };
 
 enum { PK_FLAG_ERR = 0, PK_FLAG_OVERFLOW = 1, PK_NFLAGS = 4 };
@@ -288,6 +288,8 @@ int pk_flags_read(pk_ens* e, int32_t out[PK_NFLAGS]);
 // _begin launches the insert and the copy of the flags (no wait); _end waits for that copy,
 // rebuilds after an overflow, reports a failed particle and launches the interleave when due
 int pk_ens_resort_begin(pk_ens* e);
+// the FP64 ensemble's periodic reorder (PK_OPT_INTERLEAVE_F64)
+int pk_ens_interleave_f64_launch(pk_ens* e);
 int pk_ens_resort_end(pk_ens* e);
 int pk_flags_clear(pk_ens* e);
 int pk_ens_usable(pk_ens* e);               // PK_E_STATE after a failed re-sort rebuild
