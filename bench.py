@@ -412,6 +412,7 @@ def main():
     }
     if plain is not None:
         line["plain_fp64"] = {"value": plain["particle_steps_per_s"], "ms_per_step": plain["ms_per_step"],
+                              "reorder_period": args.f64_interleave,
                               "particle_kernel_rate": plain.get("particle_kernel_rate"),
                               "roofline": plain.get("roofline"),
                               "fdtd_cell_updates_per_s": plain.get("fdtd_cell_updates_per_s"),
