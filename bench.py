@@ -89,6 +89,8 @@ def run_gpu(args, prec):
     ctx.set_option(P.OPT_PARTICLE_KERNEL, {"auto": P.KERNEL_AUTO, "per_particle": P.KERNEL_PER_PARTICLE,
                                            "cell_sorted": P.KERNEL_CELL_SORTED}[args.kernel])
     ctx.set_option(P.OPT_INTERLEAVE_F64, args.f64_interleave)
+    if args.interleave is not None:
+        ctx.set_option(P.OPT_INTERLEAVE, args.interleave)
     n = args.n
     ncell = (n, n, n)
     ncps = (args.sc, args.sc, args.sc)
@@ -366,6 +368,8 @@ def main():
                     help="128^3 run pairs the denoiser trains on (the learned form's full set is 200)")
     ap.add_argument("--batch-n", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=100)
+    ap.add_argument("--interleave", type=int, default=None,
+                    help="FP32 interleave period in re-sorts (PK_OPT_INTERLEAVE; default: the library's, 3)")
     ap.add_argument("--f64-interleave", type=int, default=8,
                     help="FP64 reorder period in re-sorts (PK_OPT_INTERLEAVE_F64; 0: off, the library's default)")
     ap.add_argument("--kernel", default="auto", choices=["auto", "per_particle", "cell_sorted"],
